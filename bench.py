@@ -1,0 +1,274 @@
+"""Benchmark: RK4 particle-steps/s at 16M particles, FP32 vs BF16 field.
+
+Workload (one GPU): 2^24 particles uniform in the unit box (seed 3), RK4,
+dt = 2e-3, x periodic / y,z reflective walls, through a 256^3 x 16-snapshot
+divergence-free random Fourier field (64 modes, |k| <= 8, unit RMS speed,
+seed 2) stored in float32 (headline) and bfloat16.  A "step" is one RK4 step
+of every particle; inputs are resident in HBM when timing starts.
+
+Prints ONE JSON line (rank 0).  Roofline: algorithmic bytes per launch
+(792 B/particle-step in FP32, 408 B in BF16: 24 B of state plus 4 stages x 2
+snapshots x 8 corners x 3 components) over the average launch duration timed
+with HIP events on the launch stream.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_PARTICLES = 1 << 24
+GRID_N = 256
+N_SNAP = 16
+DT = 2e-3
+
+
+def algorithmic_bytes_per_particle_step(elem_bytes: int) -> int:
+    return 24 + 4 * 2 * 8 * 3 * elem_bytes
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=N_PARTICLES)
+    ap.add_argument("--steps-per-launch", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=1 << 20)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    return ap.parse_args()
+
+
+def run_chunks(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events):
+    """Launch jt_track_grid over [0, n_steps) in chunks of spl steps; returns
+    per-launch event durations (ms) when record_events."""
+    durs = []
+    evs = []
+    o = L.TrackOpts()
+    o.steps_per_launch = spl
+    o.sort_every = 0
+    o.block = 0
+    s0 = 0
+    while s0 < n_steps:
+        k = min(spl, n_steps - s0)
+        st = C.c_void_p(step_times.data_ptr() + 4 * s0)
+        if record_events:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        rc = lib.jt_track_grid(C.byref(desc), C.byref(bc), L.JT_RK4, L.ptr(x), x.shape[0], st,
+                               C.c_float(np.float32(DT)), k, 1, None, C.byref(o), L.ptr(ws),
+                               ws.numel(), stream)
+        L.check(rc, "jt_track_grid")
+        if record_events:
+            e1.record()
+            evs.append((e0, e1, k))
+        s0 += k
+    if record_events:
+        torch.cuda.synchronize()
+        durs = [(a.elapsed_time(b), k) for a, b, k in evs]
+    return durs
+
+
+def measure(field, bc_obj, x0, args, torch, L, lib):
+    dev = field.device
+    n = x0.shape[0]
+    total = args.warmup + args.steps
+    times = (np.arange(total + 1, dtype=np.float64) * DT).astype(np.float32)
+    step_times = torch.from_numpy(times).to(dev)
+    desc = field.descriptor()
+    bc = bc_obj.descriptor(dev)
+    o = L.TrackOpts()
+    ws_bytes = int(lib.jt_track_grid_workspace_size(n, total, C.byref(o)))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    stream = L.stream_handle()
+    x = x0.clone()
+    run_chunks(lib, L, desc, bc, x, step_times, args.warmup, args.steps_per_launch, ws, stream, torch, False)
+    torch.cuda.synchronize()
+    x = x0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    durs = run_chunks(lib, L, desc, bc, x, step_times, args.steps, args.steps_per_launch, ws, stream, torch, True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, durs, x
+
+
+def cpu_baseline(host_data, times, grid, x0_host, n_part, n_steps):
+    """The oracle's NumPy restatement of the reference algorithm (blend every
+    node, then trilinear; rk4_step; mixed boundary), single process."""
+    from oracle import boundary as OB
+    from oracle import fields as OF
+    from oracle import integrators as OI
+    meta = OF.GridMeta(grid, grid, grid)
+    x = x0_host[:n_part].copy()
+    fn = lambda p, t: OF.structured_series_sample(host_data, times, meta, p, t, "periodic")
+    t0 = time.perf_counter()
+    for s in range(n_steps):
+        x = OI.rk4_step(x, np.float32(s * DT), DT, fn)
+        x = OB.per_axis(x, [[0, 0, 0], [1, 1, 1]], ["periodic", "reflective", "reflective"])
+    el = time.perf_counter() - t0
+    return n_part * n_steps / el, el
+
+
+def step_error_vs_fp64(host_data_f32, host_data_rounded, times, grid, states, t):
+    """Relative one-step position error of a float32 step on ``host_data_rounded``
+    against a float64 step on the float32 field, from identical states."""
+    from oracle import fields as OF
+    from oracle import integrators as OI
+    meta = OF.GridMeta(grid, grid, grid)
+    f32 = lambda p, tt: OF.structured_series_sample_corners(host_data_rounded, times, meta, p, tt, "periodic")
+    d64 = host_data_f32.astype(np.float64) if host_data_f32.size < (1 << 26) else None
+
+    def f64(p, tt):
+        # float64 twin: blend + trilinear evaluated in float64 on the stored values
+        from oracle.fields import blend_params
+        pb = blend_params(times, np.float32(tt))
+        V0 = host_data_f32[pb[1]]
+        V1 = host_data_f32[pb[2]]
+        a = float(pb[3])
+        n = grid.shape[0]
+        f = np.mod(p * n, n)
+        i0 = np.floor(f).astype(np.int64) % n
+        w = f - np.floor(f)
+        i1 = (i0 + 1) % n
+        out = np.zeros_like(p)
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    ix = i1[:, 0] if dx else i0[:, 0]
+                    iy = i1[:, 1] if dy else i0[:, 1]
+                    iz = i1[:, 2] if dz else i0[:, 2]
+                    wt = ((w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1])
+                          * (w[:, 2] if dz else 1 - w[:, 2]))
+                    v = (1 - a) * V0[ix, iy, iz].astype(np.float64) + a * V1[ix, iy, iz].astype(np.float64)
+                    out += wt[:, None] * v
+        return out
+    x32 = OI.rk4_step(states, t, DT, f32)
+    x64 = OI.rk4_step(states.astype(np.float64), float(np.float32(t)), float(np.float32(DT)), f64,
+                      dtype=np.float64)
+    rel = np.abs(x32 - x64) / np.maximum(np.abs(x64), 1e-3)
+    return float(rel.max())
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(tracking shards particles with no exchange, so the "
+                                   "one-GPU number is one shard at its real shape)"}))
+        sys.exit(2)
+    import torch
+    import jaxtrace_amd as jt
+    from jaxtrace_amd import _lib as L
+    from jaxtrace_amd import workloads as W
+    lib = L.load()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(0)
+
+    t_gen = time.perf_counter()
+    data, times, grid = W.random_fourier(n=GRID_N, n_snap=N_SNAP, t_end=2.0, seed=2, device=dev)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    x0_host = torch.rand((args.particles, 3), generator=g, dtype=torch.float32).numpy()
+    x0 = torch.from_numpy(x0_host).to(dev)
+    field32 = jt.StructuredTimeSeriesField(data, times, grid, grid, grid, bounds_handling="periodic")
+    bc = jt.mixed_boundary([[0, 0, 0], [1, 1, 1]], {"x": "periodic", "y": "reflective", "z": "reflective"})
+    gen_s = time.perf_counter() - t_gen
+
+    wall32, durs32, xf32 = measure(field32, bc, x0, args, torch, L, lib)
+    n = args.particles
+    rate32 = n * args.steps / wall32
+    ms_step = wall32 * 1e3 / args.steps
+    launch_ms = [d / k for d, k in durs32]           # per step within each launch
+    spl = args.steps_per_launch
+    full = [(d, k) for d, k in durs32 if k == spl] or durs32
+    avg_launch_ms = float(np.mean([d for d, _ in full]))
+    steps_per = full[0][1]
+    bpps32 = algorithmic_bytes_per_particle_step(4)
+    achieved = bpps32 * n * steps_per / (avg_launch_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "RK4 particle-steps/s at 16M particles, FP32 vs BF16 field; adjoint/fwd ratio",
+        "value": rate32,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random Fourier field generated on device, uniform particles)",
+        "config": {"workload": "random-Fourier 256^3 x 16 snapshots, RK4, dt=2e-3, "
+                               "x periodic / y,z reflective, FP32 field",
+                   "particles": n, "grid": [GRID_N] * 3, "snapshots": N_SNAP, "dt": DT,
+                   "steps_per_launch": spl},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_track<RK4,f32x3>", "launch_ms": avg_launch_ms,
+                     "particle_steps_per_launch": n * steps_per,
+                     "bytes_per_particle_step": bpps32},
+    }
+    prof = os.path.join(ROOT, "profiles", "track_grid_rk4_f32_traffic.json")
+    if os.path.exists(prof):
+        try:
+            tr = json.load(open(prof))
+            result["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
+            result["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+        except Exception:
+            pass
+
+    if not args.no_bf16:
+        field16 = field32.with_storage("bfloat16")
+        wall16, durs16, xf16 = measure(field16, bc, x0, args, torch, L, lib)
+        rate16 = n * args.steps / wall16
+        full16 = [(d, k) for d, k in durs16 if k == spl] or durs16
+        avg16 = float(np.mean([d for d, _ in full16]))
+        ach16 = algorithmic_bytes_per_particle_step(2) * n * full16[0][1] / (avg16 * 1e-3) / 1e9
+        result["bf16"] = {"value": rate16, "speedup_vs_f32": rate16 / rate32, "launch_ms": avg16,
+                          "roofline_achieved_GBs": ach16, "roofline_frac": ach16 / HBM_PEAK_GBS,
+                          "bytes_per_particle_step": algorithmic_bytes_per_particle_step(2)}
+        del field16
+
+    # one-step error vs a float64 step on a 4096-particle sample of the final state
+    try:
+        host = data.cpu().numpy()
+        hb = data.to(torch.bfloat16).to(torch.float32).cpu().numpy()
+        idx = np.random.default_rng(11).choice(n, min(4096, n), replace=False)
+        st = xf32[idx].cpu().numpy()
+        result["step_rel_err_vs_fp64"] = {
+            "f32": step_error_vs_fp64(host, host, times, grid, st, np.float32(0.5)),
+            "bf16": step_error_vs_fp64(host, hb, times, grid, st, np.float32(0.5)),
+            "sample": "4096 particles, one RK4 step at t=0.5 from identical states"}
+    except Exception as e:  # reported, never fatal
+        result["step_rel_err_vs_fp64"] = {"error": str(e)}
+
+    if not args.no_cpu_baseline:
+        host = data.cpu().numpy()
+        npart = min(args.cpu_particles, n)
+        rate_cpu, el = cpu_baseline(host, times, grid, x0_host, npart, args.cpu_steps)
+        result["cpu_baseline"] = {
+            "value": rate_cpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{npart} particles x {args.cpu_steps} RK4 steps of the same workload "
+                      f"(NumPy restatement of the reference algorithm: every node blended per "
+                      f"stage, then trilinear), {el:.1f} s",
+            "host_cpus": len(os.sched_getaffinity(0))}
+    result["adjoint_fwd_ratio"] = None
+    result["setup_s"] = gen_s
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,32 @@
+"""jaxtrace_amd: MI355X (gfx950) implementation of JAXTrace's particle-advection
+and density hot path, behind the reference's own Python API.
+
+Host code holds torch tensors on the ROCm device and calls hand-written HIP
+kernels through the C ABI declared in ``include/jaxtrace_amd.h``.
+"""
+from ._lib import load as _load_library  # noqa: F401
+from .fields import StructuredGridSampler, StructuredTimeSeriesField
+from .boundary import (
+    BoundaryCondition,
+    periodic_boundary,
+    reflective_boundary,
+    clamping_boundary,
+    absorbing_boundary_factory,
+    mixed_boundary,
+    continuous_inlet_boundary_factory,
+    apply_periodic,
+    clamp_to_bounds,
+    reflect_boundary,
+    absorbing_boundary,
+)
+from .tracking import TrackerOptions, Trajectory, ParticleTracker, create_tracker
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "StructuredGridSampler", "StructuredTimeSeriesField",
+    "BoundaryCondition", "periodic_boundary", "reflective_boundary", "clamping_boundary",
+    "absorbing_boundary_factory", "mixed_boundary", "continuous_inlet_boundary_factory",
+    "apply_periodic", "clamp_to_bounds", "reflect_boundary", "absorbing_boundary",
+    "TrackerOptions", "Trajectory", "ParticleTracker", "create_tracker",
+]

@@ -1,0 +1,122 @@
+"""ctypes binding of the in-tree HIP library ``_jt.so`` (C ABI in
+``include/jaxtrace_amd.h``).
+
+``torch`` is imported before the library is loaded so that the HIP runtime
+torch already holds is the one the library binds to (one runtime, one set of
+device pointers and streams per process).  There is no CPU fallback: if the
+library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch  # noqa: F401  (must precede loading _jt.so)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_jt.so")
+
+JT_OK, JT_EINVAL, JT_ENOMEM, JT_EHIP, JT_EUNSUPPORTED = 0, 1, 2, 3, 4
+
+JT_EULER, JT_RK2, JT_RK4 = 0, 1, 2
+JT_DTYPE_F32, JT_DTYPE_BF16, JT_DTYPE_F16 = 0, 1, 2
+JT_BOUNDS = {"clamp": 0, "zero": 1, "nan": 2, "periodic": 3}
+JT_TIME = {"linear": 0, "nearest": 1}
+JT_EXTRAP = {"constant": 0, "linear": 1, "nan": 2, "zero": 3}
+JT_BC_NONE, JT_BC_PERIODIC, JT_BC_REFLECTIVE, JT_BC_CLAMP, JT_BC_ABSORB = 0, 1, 2, 3, 4
+JT_INLET_OFF, JT_INLET_GRID, JT_INLET_ZERO = 0, 1, 2
+
+c_float3 = C.c_float * 3
+c_int3 = C.c_int32 * 3
+
+
+class GridField(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p), ("times", C.c_void_p), ("n_snap", C.c_int32),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("dtype", C.c_int32), ("ncomp", C.c_int32),
+        ("origin", c_float3), ("spacing", c_float3),
+        ("bounds_mode", C.c_int32), ("time_interp", C.c_int32), ("time_extrap", C.c_int32),
+    ]
+
+
+class Boundary(C.Structure):
+    _fields_ = [
+        ("axis_kind", c_int3), ("lo", c_float3), ("hi", c_float3),
+        ("absorb_rows", C.c_int32), ("outside_value", C.c_float),
+        ("inlet", C.c_int32), ("inlet_axis", C.c_int32), ("inlet_positive", C.c_int32),
+        ("inlet_backstop", C.c_float),
+        ("inlet_coords", C.c_void_p * 2), ("inlet_ncoords", C.c_int32 * 2),
+    ]
+
+
+class TrackOpts(C.Structure):
+    _fields_ = [("steps_per_launch", C.c_int32), ("sort_every", C.c_int32), ("block", C.c_int32)]
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol the header declares
+_SIGS = {
+    "jt_last_error": (C.c_char_p, []),
+    "jt_version": (C.c_int32, []),
+    "jt_grid_sample": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_int64, C.c_float,
+                                 C.c_void_p, C.c_void_p]),
+    "jt_apply_boundary": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p]),
+    "jt_track_grid_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.POINTER(TrackOpts)]),
+    "jt_track_grid": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
+                                C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
+                                C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
+}
+
+
+def load():
+    """Load ``_jt.so`` (once) and declare every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"jaxtrace_amd HIP library not built ({LIB_PATH} missing); "
+            "run `python -m jaxtrace_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+_EXC = {JT_EINVAL: ValueError, JT_ENOMEM: MemoryError, JT_EHIP: RuntimeError,
+        JT_EUNSUPPORTED: TypeError}
+
+
+def check(rc: int, what: str):
+    """Map a status code to the reference's exception types."""
+    if rc == JT_OK:
+        return
+    msg = load().jt_last_error().decode(errors="replace")
+    raise _EXC.get(rc, RuntimeError)(f"{what}: {msg}")
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("jaxtrace_amd requires a ROCm GPU (torch.cuda.is_available() is False)")
+
+
+def device():
+    require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)

@@ -1,0 +1,193 @@
+"""Boundary conditions (jaxtrace/tracking/boundary.py) as device descriptors.
+
+Each factory returns a ``BoundaryCondition``: a callable with the reference's
+closure name (``_periodic_bc`` ...) that applies the condition on the GPU
+(``jt_apply_boundary``), and whose descriptor the tracking kernels fuse into
+every step.  Factories, argument meaning and errors follow the reference:
+
+* periodic_boundary / reflective_boundary / clamping_boundary /
+  absorbing_boundary_factory   -- boundary.py:215-337
+* mixed_boundary               -- boundary.py:377-438 (per axis, x then y then z)
+* continuous_inlet_boundary_factory -- boundary.py:1444-1601 with the
+  grid-preserving replacement of :1249-1318
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Union
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+F32 = np.float32
+
+
+def _ensure_bounds_shape(bounds) -> np.ndarray:
+    """boundary.py:39-86."""
+    b = np.asarray(bounds, dtype=F32)
+    if b.ndim == 1:
+        if len(b) == 4:
+            b = np.array([[b[0], b[2], 0.0], [b[1], b[3], 0.0]], dtype=F32)
+        elif len(b) == 6:
+            b = np.array([[b[0], b[2], b[4]], [b[1], b[3], b[5]]], dtype=F32)
+        else:
+            raise ValueError(f"1D bounds must have 4 or 6 elements, got {len(b)}")
+    elif b.ndim == 2:
+        if b.shape == (2, 2):
+            b = np.array([[b[0, 0], b[1, 0], 0.0], [b[0, 1], b[1, 1], 0.0]], dtype=F32)
+        elif b.shape == (2, 3):
+            pass
+        elif b.shape == (3, 2):
+            b = b.T
+        else:
+            raise ValueError(f"2D bounds must have shape (2,2), (2,3), or (3,2), got {b.shape}")
+    else:
+        raise ValueError(f"Bounds must be 1D or 2D array, got {b.ndim}D")
+    if not np.all(b[0] < b[1]):
+        raise ValueError(f"Invalid bounds: min {b[0]} >= max {b[1]}")
+    return np.ascontiguousarray(b.astype(F32))
+
+
+class BoundaryCondition:
+    """A boundary condition the tracking kernels can fuse into each step."""
+
+    def __init__(self, name: str, bounds: np.ndarray, axis_kind=(0, 0, 0), absorb_rows=False,
+                 outside_value=np.nan, inlet=None):
+        self.__name__ = name
+        self.bounds = bounds
+        self.axis_kind = tuple(int(k) for k in axis_kind)
+        self.absorb_rows = bool(absorb_rows)
+        self.outside_value = F32(outside_value)
+        self.inlet = inlet           # dict or None
+        self._dev_cache = {}
+
+    def descriptor(self, device) -> L.Boundary:
+        key = str(device)
+        if key in self._dev_cache:
+            return self._dev_cache[key][0]
+        d = L.Boundary()
+        d.axis_kind[:] = list(self.axis_kind)
+        d.lo[:] = [F32(v) for v in self.bounds[0]]
+        d.hi[:] = [F32(v) for v in self.bounds[1]]
+        d.absorb_rows = int(self.absorb_rows)
+        d.outside_value = self.outside_value
+        keep = []
+        if self.inlet is not None:
+            inl = self.inlet
+            d.inlet = inl["mode"]
+            d.inlet_axis = inl["axis"]
+            d.inlet_positive = int(inl["positive"])
+            d.inlet_backstop = inl["backstop"]
+            for j, coords in enumerate(inl["coords"]):
+                t = torch.from_numpy(np.ascontiguousarray(coords, dtype=F32)).to(device)
+                keep.append(t)
+                d.inlet_coords[j] = t.data_ptr()
+                d.inlet_ncoords[j] = t.numel()
+        self._dev_cache[key] = (d, keep)
+        return d
+
+    def __call__(self, positions):
+        """Apply on the GPU; returns the same array type it was given."""
+        lib = L.load()
+        dev = positions.device if isinstance(positions, torch.Tensor) and positions.is_cuda else L.device()
+        is_torch = isinstance(positions, torch.Tensor)
+        x = (positions.detach().to(device=dev, dtype=torch.float32) if is_torch
+             else torch.from_numpy(np.asarray(positions, dtype=F32)).to(dev))
+        x = x.reshape(-1, 3).clone().contiguous()
+        L.check(lib.jt_apply_boundary(C.byref(self.descriptor(dev)), L.ptr(x), x.shape[0],
+                                      L.stream_handle()), "jt_apply_boundary")
+        return x if is_torch else x.cpu().numpy()
+
+    def __repr__(self):
+        return f"BoundaryCondition({self.__name__}, kinds={self.axis_kind}, inlet={self.inlet is not None})"
+
+
+def periodic_boundary(bounds) -> BoundaryCondition:
+    """boundary.py:215-240."""
+    return BoundaryCondition("_periodic_bc", _ensure_bounds_shape(bounds), (L.JT_BC_PERIODIC,) * 3)
+
+
+def reflective_boundary(bounds) -> BoundaryCondition:
+    """boundary.py:243-268."""
+    return BoundaryCondition("_reflective_bc", _ensure_bounds_shape(bounds), (L.JT_BC_REFLECTIVE,) * 3)
+
+
+def clamping_boundary(bounds) -> BoundaryCondition:
+    """boundary.py:271-296."""
+    return BoundaryCondition("_clamping_bc", _ensure_bounds_shape(bounds), (L.JT_BC_CLAMP,) * 3)
+
+
+def absorbing_boundary_factory(bounds, outside_value: float = np.nan) -> BoundaryCondition:
+    """boundary.py:299-337: a particle with any coordinate outside gets
+    ``outside_value`` in all three coordinates."""
+    return BoundaryCondition("_absorbing_bc", _ensure_bounds_shape(bounds), (L.JT_BC_NONE,) * 3,
+                             absorb_rows=True, outside_value=outside_value)
+
+
+_MIXED = {"periodic": L.JT_BC_PERIODIC, "reflective": L.JT_BC_REFLECTIVE,
+          "clamping": L.JT_BC_CLAMP, "absorbing": L.JT_BC_ABSORB}
+
+
+def mixed_boundary(bounds, conditions: Dict[str, str]) -> BoundaryCondition:
+    """boundary.py:377-438: per-axis kinds, missing axes default to periodic."""
+    b = _ensure_bounds_shape(bounds)
+    kinds = []
+    for axis in ("x", "y", "z"):
+        c = conditions.get(axis, "periodic")
+        if c not in _MIXED:
+            raise ValueError(f"Unknown condition type: {c}")
+        kinds.append(_MIXED[c])
+    return BoundaryCondition("_mixed_bc", b, kinds, outside_value=np.nan)
+
+
+def continuous_inlet_boundary_factory(bounds, flow_axis: str = "x", flow_direction: str = "positive",
+                                      inlet_distribution: str = "uniform",
+                                      concentrations: Optional[Dict[str, int]] = None) -> BoundaryCondition:
+    """boundary.py:1444-1601.  The 'grid' distribution (with concentrations)
+    snaps an exited particle's off-axis coordinates to the nearest node of
+    ``np.linspace(lo, hi, conc, dtype=float32)`` (:1249-1318)."""
+    b = _ensure_bounds_shape(bounds)
+    axis_map = {"x": 0, "y": 1, "z": 2}
+    if flow_axis not in axis_map:
+        raise ValueError(f"flow_axis must be 'x', 'y', or 'z', got {flow_axis}")
+    a = axis_map[flow_axis]
+    positive = flow_direction == "positive"
+    amin, amax = b[0, a], b[1, a]
+    # :1580-1587, NumPy scalar arithmetic exactly as the reference evaluates it
+    backstop = F32(amin + 0.001 * (amax - amin)) if positive else F32(amax - 0.001 * (amax - amin))
+    others = [i for i in range(3) if i != a]
+    if inlet_distribution == "grid" and concentrations is not None:
+        names = "xyz"
+        coords = [np.linspace(b[0, i], b[1, i], max(1, int(concentrations[names[i]])), dtype=np.float32)
+                  for i in others]
+        mode = L.JT_INLET_GRID
+    elif inlet_distribution in ("uniform", "gaussian"):
+        raise NotImplementedError(
+            "continuous inlet with random replacement ('uniform'/'gaussian') draws from the host "
+            "np.random state every step; the fused GPU path supports the reproducible 'grid' "
+            "distribution")
+    else:
+        coords = [np.zeros(1, F32), np.zeros(1, F32)]
+        mode = L.JT_INLET_ZERO
+    inlet = {"mode": mode, "axis": a, "positive": positive, "backstop": backstop, "coords": coords}
+    return BoundaryCondition("_continuous_inlet_bc", b, (L.JT_BC_NONE,) * 3, inlet=inlet)
+
+
+# ---- array functions of the reference (boundary.py:101-210) -----------------
+
+def apply_periodic(x, bounds):
+    return periodic_boundary(bounds)(x)
+
+
+def clamp_to_bounds(x, bounds):
+    return clamping_boundary(bounds)(x)
+
+
+def reflect_boundary(x, bounds):
+    return reflective_boundary(bounds)(x)
+
+
+def absorbing_boundary(x, bounds, outside_value: float = np.nan):
+    return absorbing_boundary_factory(bounds, outside_value)(x)

@@ -1,0 +1,288 @@
+// Device-side temporal blend + trilinear sampling on a structured grid, and
+// the boundary epilogue.  Shared by the tracking, sampling and adjoint
+// kernels.  All arithmetic is float32, one IEEE operation at a time, in the
+// reference's evaluation order (the library is built with -ffp-contract=off).
+#pragma once
+#include "jt_common.h"
+
+namespace jt {
+
+// Snapshot storage layouts (element type x components per node).
+enum StoreKind { ST_F32x3 = 0, ST_F32x4 = 1, ST_BF16x4 = 2, ST_F16x4 = 3 };
+
+// Result of the scalar half of time_series.py:155-245 for one stage time.
+enum BlendMode { BL_INTERP = 0, BL_COPY = 1, BL_LEFT = 2, BL_RIGHT = 3, BL_ZERO = 4, BL_NAN = 5 };
+struct Blend {
+    int i0, i1;   // snapshot indices
+    float a;      // weight
+    int mode;     // BlendMode
+};
+
+struct GridDev {
+    const void* data;
+    long long snap_stride;   // elements per snapshot (nodes * ncomp)
+    const float* times;
+    int T;
+    int nx, ny, nz;
+    float o[3], d[3];        // origin, spacing
+    float vhi[3];            // origin + f32(n-1)*d: inclusive validity bound
+    float cmax[3];           // f32(max(n-1, 0)): clip bound of the grid coordinate
+    int bmode;               // JT_BOUNDS_*
+    int interp, extrap;      // JT_TIME_*, JT_EXTRAP_*
+};
+
+// searchsorted(times, t, side='right') for non-decreasing times.
+__device__ __forceinline__ int search_right(const float* times, int T, float t) {
+    int lo = 0, hi = T;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (times[mid] <= t) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// time_series.py:155-245, scalar part (which snapshots and which weight).
+__device__ inline Blend time_blend(const float* times, int T, float t, int interp, int extrap) {
+    Blend b;
+    b.i0 = 0; b.i1 = 0; b.a = 0.0f; b.mode = BL_COPY;
+    if (interp == JT_TIME_NEAREST) {                  // :171-174, first minimum
+        int best = 0;
+        float bd = fabsf(times[0] - t);
+        for (int k = 1; k < T; ++k) {
+            float dk = fabsf(times[k] - t);
+            if (dk < bd) { bd = dk; best = k; }
+        }
+        b.i0 = b.i1 = best;
+        return b;
+    }
+    int ir = search_right(times, T, t);               // :162-164
+    int il = min(max(ir - 1, 0), T - 1);
+    float tmin = times[0], tmax = times[T - 1];
+    bool left = t < tmin, right = t > tmax;
+    if (left || right) {
+        if (extrap == JT_EXTRAP_CONSTANT) {           // :217-219
+            b.i0 = b.i1 = left ? 0 : T - 1;
+            return b;
+        }
+        if (extrap == JT_EXTRAP_ZERO) { b.mode = BL_ZERO; return b; }
+        if (extrap == JT_EXTRAP_NAN) { b.mode = BL_NAN; return b; }
+        if (T <= 1) { b.i0 = b.i1 = left ? 0 : T - 1; return b; }
+        if (left) {                                    // :190-199
+            float t0 = times[0], t1 = times[1];
+            float dd = t1 - t0;
+            dd = dd < 1e-12f ? 1e-12f : dd;
+            b.i0 = 0; b.i1 = 1; b.a = (t - t0) / dd; b.mode = BL_LEFT;
+        } else {                                       // :201-210
+            float t0 = times[T - 2], t1 = times[T - 1];
+            float dd = t1 - t0;
+            dd = dd < 1e-12f ? 1e-12f : dd;
+            b.i0 = T - 2; b.i1 = T - 1; b.a = (t - t1) / dd; b.mode = BL_RIGHT;
+        }
+        return b;
+    }
+    int irc = min(max(ir, 0), T - 1);                  // :178-188
+    float t0 = times[il], t1 = times[irc];
+    float dd = t1 - t0;
+    dd = dd < 1e-12f ? 1e-12f : dd;
+    b.i0 = il; b.i1 = irc; b.a = clipf((t - t0) / dd, 0.0f, 1.0f); b.mode = BL_INTERP;
+    return b;
+}
+
+template <int ST>
+__device__ __forceinline__ void load_node(const void* base, int node, float& a, float& b, float& c) {
+    if constexpr (ST == ST_F32x3) {
+        const float* p = (const float*)base + (size_t)node * 3;
+        a = p[0]; b = p[1]; c = p[2];
+    } else if constexpr (ST == ST_F32x4) {
+        float4 v = ((const float4*)base)[node];
+        a = v.x; b = v.y; c = v.z;
+    } else if constexpr (ST == ST_BF16x4) {
+        uint2 v = ((const uint2*)base)[node];
+        a = bf16_to_f32((uint16_t)(v.x & 0xffffu));
+        b = bf16_to_f32((uint16_t)(v.x >> 16));
+        c = bf16_to_f32((uint16_t)(v.y & 0xffffu));
+    } else {
+        uint2 v = ((const uint2*)base)[node];
+        a = f16_to_f32((uint16_t)(v.x & 0xffffu));
+        b = f16_to_f32((uint16_t)(v.x >> 16));
+        c = f16_to_f32((uint16_t)(v.y & 0xffffu));
+    }
+}
+
+template <int ST> struct ElemBytes { static constexpr int v = 4; };
+template <> struct ElemBytes<ST_BF16x4> { static constexpr int v = 2; };
+template <> struct ElemBytes<ST_F16x4> { static constexpr int v = 2; };
+
+// Blended value of one node (time_series.py:186-188, :199, :210).
+template <int ST>
+__device__ __forceinline__ void corner(const void* b0, const void* b1, const Blend& bl,
+                                       float oma, int node, float v[3]) {
+    if (bl.mode == BL_ZERO) { v[0] = v[1] = v[2] = 0.0f; return; }
+    if (bl.mode == BL_NAN) { v[0] = v[1] = v[2] = __int_as_float(0x7fc00000); return; }
+    float a0, a1, a2;
+    load_node<ST>(b0, node, a0, a1, a2);
+    if (bl.mode == BL_COPY) { v[0] = a0; v[1] = a1; v[2] = a2; return; }
+    float c0, c1, c2;
+    load_node<ST>(b1, node, c0, c1, c2);
+    const float al = bl.a;
+    if (bl.mode == BL_INTERP) {
+        v[0] = oma * a0 + al * c0;
+        v[1] = oma * a1 + al * c1;
+        v[2] = oma * a2 + al * c2;
+    } else if (bl.mode == BL_LEFT) {
+        v[0] = a0 + al * (c0 - a0);
+        v[1] = a1 + al * (c1 - a1);
+        v[2] = a2 + al * (c2 - a2);
+    } else {
+        v[0] = c0 + al * (c0 - a0);
+        v[1] = c1 + al * (c1 - a1);
+        v[2] = c2 + al * (c2 - a2);
+    }
+}
+
+// structured.py:176-220 for one axis.
+__device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, int n,
+                                           int& i0, int& i1, float& w, bool& valid) {
+    float f = (x - g.o[ax]) / g.d[ax];
+    if (g.bmode == JT_BOUNDS_PERIODIC) {
+        f = jmodf(f, (float)n);
+    } else {
+        valid = valid && (x >= g.o[ax]) && (x <= g.vhi[ax]);
+        f = clipf(f, 0.0f, g.cmax[ax]);
+    }
+    float fl = floorf(f);
+    int ii = (fl != fl) ? 0 : (int)fl;                 // NaN -> 0
+    int jj = (g.bmode == JT_BOUNDS_PERIODIC) ? (ii + 1) % n : min(max(ii + 1, 0), n - 1);
+    w = f - (float)ii;
+    i0 = min(max(ii, 0), n - 1);                       // gather index clamp
+    i1 = min(max(jj, 0), n - 1);
+}
+
+// Blended trilinear sample u(x, t) (structured.py:167-250 after the blend).
+template <int ST>
+__device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
+                                       float px, float py, float pz, float out[3]) {
+    int i0, i1, j0, j1, k0, k1;
+    float wx, wy, wz;
+    bool valid = true;
+    axis_coord(g, 0, px, g.nx, i0, i1, wx, valid);
+    axis_coord(g, 1, py, g.ny, j0, j1, wy, valid);
+    axis_coord(g, 2, pz, g.nz, k0, k1, wz, valid);
+    const char* base = (const char*)g.data;
+    const size_t sb = (size_t)g.snap_stride * ElemBytes<ST>::v;
+    const void* b0 = base + (size_t)bl.i0 * sb;
+    const void* b1 = base + (size_t)bl.i1 * sb;
+    const float oma = 1.0f - bl.a;
+    const int r0 = i0 * g.ny, r1 = i1 * g.ny;
+    const int n00 = (r0 + j0) * g.nz, n10 = (r1 + j0) * g.nz;
+    const int n01 = (r0 + j1) * g.nz, n11 = (r1 + j1) * g.nz;
+    float v000[3], v100[3], v010[3], v110[3], v001[3], v101[3], v011[3], v111[3];
+    corner<ST>(b0, b1, bl, oma, n00 + k0, v000);
+    corner<ST>(b0, b1, bl, oma, n10 + k0, v100);
+    corner<ST>(b0, b1, bl, oma, n01 + k0, v010);
+    corner<ST>(b0, b1, bl, oma, n11 + k0, v110);
+    corner<ST>(b0, b1, bl, oma, n00 + k1, v001);
+    corner<ST>(b0, b1, bl, oma, n10 + k1, v101);
+    corner<ST>(b0, b1, bl, oma, n01 + k1, v011);
+    corner<ST>(b0, b1, bl, oma, n11 + k1, v111);
+    const float ax = 1.0f - wx, ay = 1.0f - wy, az = 1.0f - wz;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float c00 = v000[c] * ax + v100[c] * wx;       // :234-237
+        float c10 = v010[c] * ax + v110[c] * wx;
+        float c01 = v001[c] * ax + v101[c] * wx;
+        float c11 = v011[c] * ax + v111[c] * wx;
+        float c0 = c00 * ay + c10 * wy;                 // :239-240
+        float c1 = c01 * ay + c11 * wy;
+        out[c] = c0 * az + c1 * wz;                     // :242
+    }
+    if (!valid) {                                       // :244-247
+        if (g.bmode == JT_BOUNDS_ZERO) { out[0] = out[1] = out[2] = 0.0f; }
+        else if (g.bmode == JT_BOUNDS_NAN) { out[0] = out[1] = out[2] = __int_as_float(0x7fc00000); }
+    }
+}
+
+// ---- boundary epilogue (boundary.py) ----------------------------------------
+struct BcDev {
+    int kind[3];
+    float lo[3], hi[3];
+    int absorb_rows;
+    float outside;
+    int inlet, ia, ipos;
+    float backstop;
+    const float* coords[2];
+    int nc[2];
+};
+
+__device__ __forceinline__ float bc_axis(int kind, float x, float lo, float hi, float outside) {
+    if (kind == JT_BC_PERIODIC) {                       // :117-123
+        float w = hi - lo;
+        w = w < 0.0f ? 0.0f : w;
+        float ws = w > 0.0f ? w : 1.0f;
+        return lo + jmodf(x - lo, ws);
+    } else if (kind == JT_BC_REFLECTIVE) {              // :165-172
+        float w = hi - lo;
+        w = w < 0.0f ? 0.0f : w;
+        float y = jmodf(x - lo, 2.0f * (w > 0.0f ? w : 1.0f));
+        float yr = (y <= w) ? y : 2.0f * w - y;
+        return lo + yr;
+    } else if (kind == JT_BC_CLAMP) {                   // :143
+        return clipf(x, lo, hi);
+    } else if (kind == JT_BC_ABSORB) {                  // per-axis, as mixed_boundary applies it
+        return (x < lo || x > hi) ? outside : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ float snap_nearest(const float* c, int nc, float v) {
+    int best = 0;
+    float bd = fabsf(c[0] - v);
+    for (int k = 1; k < nc; ++k) {
+        float dk = fabsf(c[k] - v);
+        if (dk < bd) { bd = dk; best = k; }
+    }
+    return c[best];
+}
+
+__device__ __forceinline__ void apply_bc(const BcDev& b, float p[3]) {
+    if (b.inlet != JT_INLET_OFF) {                      // boundary.py:1515-1599
+        const int a = b.ia;
+        const int o0 = (a == 0) ? 1 : 0;
+        const int o1 = (a == 2) ? 1 : 2;
+        const bool exited = b.ipos ? (p[a] > b.hi[a]) : (p[a] < b.lo[a]);
+        if (exited) {
+            float g0 = 0.0f, g1 = 0.0f;
+            if (b.inlet == JT_INLET_GRID) {             // :1301-1317
+                g0 = snap_nearest(b.coords[0], b.nc[0], p[o0]);
+                g1 = snap_nearest(b.coords[1], b.nc[1], p[o1]);
+            }
+            p[a] = b.ipos ? b.lo[a] : b.hi[a];
+            p[o0] = g0;
+            p[o1] = g1;
+        }
+        if (b.ipos) { if (p[a] < b.lo[a]) p[a] = b.backstop; }   // :1580-1587
+        else        { if (p[a] > b.hi[a]) p[a] = b.backstop; }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {                   // :1590-1597
+            if (i == a) continue;
+            float v = p[i];
+            if (v < b.lo[i]) p[i] = 2.0f * b.lo[i] - v;
+            else if (v > b.hi[i]) p[i] = 2.0f * b.hi[i] - v;
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) p[i] = bc_axis(b.kind[i], p[i], b.lo[i], b.hi[i], b.outside);
+    if (b.absorb_rows) {                                // :190-210
+        bool out = false;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out = out || (p[i] < b.lo[i]) || (p[i] > b.hi[i]);
+        if (out) { p[0] = p[1] = p[2] = b.outside; }
+    }
+}
+
+// Host-side translation of the public descriptors.
+int make_grid_dev(const jt_grid_field* f, GridDev& g, int& store_kind);
+int make_bc_dev(const jt_boundary* b, BcDev& d);
+
+}  // namespace jt

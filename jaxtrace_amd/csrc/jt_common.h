@@ -1,0 +1,60 @@
+// Shared host/device helpers for the jaxtrace_amd HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "../../include/jaxtrace_amd.h"
+
+namespace jt {
+
+// Record a failure message for jt_last_error() and return `code`.
+int fail(int code, const char* fmt, ...);
+
+// Check a HIP call; on error record it and return JT_EHIP from the caller.
+#define JT_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess)                                               \
+            return ::jt::fail(JT_EHIP, "%s failed: %s (%s:%d)", #call,      \
+                              hipGetErrorString(_e), __FILE__, __LINE__);   \
+    } while (0)
+
+#define JT_REQUIRE(cond, ...)                                               \
+    do {                                                                    \
+        if (!(cond)) return ::jt::fail(JT_EINVAL, __VA_ARGS__);             \
+    } while (0)
+
+// Carve aligned sub-buffers out of a caller-provided workspace.
+struct Arena {
+    char* base;
+    size_t cap, used;
+    explicit Arena(void* p, size_t c) : base((char*)p), cap(c), used(0) {}
+    template <class T> T* take(size_t count) {
+        size_t off = (used + 255) & ~(size_t)255;
+        used = off + count * sizeof(T);
+        return (base && used <= cap) ? (T*)(base + off) : nullptr;
+    }
+    bool ok() const { return used <= cap; }
+};
+
+// ---- device helpers ----------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+    return __uint_as_float(((uint32_t)h) << 16);
+}
+__device__ __forceinline__ float f16_to_f32(uint16_t h) {
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+
+// jnp.mod on floats: fmod, then add the divisor when the signs differ.
+__device__ __forceinline__ float jmodf(float a, float n) {
+    float r = fmodf(a, n);
+    if (r != 0.0f && ((r < 0.0f) != (n < 0.0f))) r = r + n;
+    return r;
+}
+// jnp.clip (minimum(maximum(x, lo), hi)) keeping NaN.
+__device__ __forceinline__ float clipf(float x, float lo, float hi) {
+    float y = x < lo ? lo : x;
+    return y > hi ? hi : y;
+}
+
+}  // namespace jt

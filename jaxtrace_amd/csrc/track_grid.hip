@@ -1,0 +1,297 @@
+// Fused multi-step particle tracking through a time-blended structured grid.
+//
+// One thread owns one particle and keeps its position in registers across all
+// stages and all steps of a launch: the reference's lax.scan over
+// rk4_step -> field_fn -> boundary_fn (jaxtrace/tracking/tracker.py:228-246)
+// materialises k1..k4 and the blended node arrays in memory every stage; here
+// each stage is 8 corner gathers from each of the two bracketing snapshots.
+#include "grid_sampler.h"
+#include <hipcub/hipcub.hpp>
+
+namespace jt {
+
+int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
+    JT_REQUIRE(f != nullptr, "field descriptor is NULL");
+    JT_REQUIRE(f->data != nullptr && f->times != nullptr, "field data/times pointer is NULL");
+    JT_REQUIRE(f->n_snap >= 1, "n_snap must be >= 1 (got %d)", f->n_snap);
+    JT_REQUIRE(f->nx >= 1 && f->ny >= 1 && f->nz >= 1, "grid dims must be >= 1");
+    JT_REQUIRE(f->ncomp == 3 || f->ncomp == 4, "ncomp must be 3 or 4 (got %d)", f->ncomp);
+    JT_REQUIRE(f->bounds_mode >= JT_BOUNDS_CLAMP && f->bounds_mode <= JT_BOUNDS_PERIODIC,
+               "bad bounds_mode %d", f->bounds_mode);
+    JT_REQUIRE(f->time_interp == JT_TIME_LINEAR || f->time_interp == JT_TIME_NEAREST,
+               "bad time_interp %d", f->time_interp);
+    JT_REQUIRE(f->time_extrap >= JT_EXTRAP_CONSTANT && f->time_extrap <= JT_EXTRAP_ZERO,
+               "bad time_extrap %d", f->time_extrap);
+    long long nodes = (long long)f->nx * f->ny * f->nz;
+    JT_REQUIRE(nodes * f->ncomp < (1LL << 31), "grid too large for 32-bit node offsets");
+    if (f->dtype == JT_DTYPE_F32) st = (f->ncomp == 3) ? ST_F32x3 : ST_F32x4;
+    else if (f->dtype == JT_DTYPE_BF16) st = ST_BF16x4;
+    else if (f->dtype == JT_DTYPE_F16) st = ST_F16x4;
+    else return fail(JT_EINVAL, "bad dtype %d", f->dtype);
+    if (f->dtype != JT_DTYPE_F32 && f->ncomp != 4)
+        return fail(JT_EUNSUPPORTED, "16-bit snapshots must be stored with ncomp=4");
+    g.data = f->data;
+    g.snap_stride = nodes * f->ncomp;
+    g.times = f->times;
+    g.T = f->n_snap;
+    g.nx = f->nx; g.ny = f->ny; g.nz = f->nz;
+    const int n[3] = {f->nx, f->ny, f->nz};
+    for (int a = 0; a < 3; ++a) {
+        g.o[a] = f->origin[a];
+        g.d[a] = f->spacing[a];
+        // structured.py:188-190: x_min + (N-1)*dx, int32*f32 -> f32
+        float nm1 = (float)(n[a] - 1);
+        float span = nm1 * f->spacing[a];
+        g.vhi[a] = f->origin[a] + span;
+        g.cmax[a] = (float)(n[a] - 1 > 0 ? n[a] - 1 : 0);
+    }
+    g.bmode = f->bounds_mode;
+    g.interp = f->time_interp;
+    g.extrap = f->time_extrap;
+    return JT_OK;
+}
+
+int make_bc_dev(const jt_boundary* b, BcDev& d) {
+    if (b == nullptr) {
+        for (int i = 0; i < 3; ++i) { d.kind[i] = JT_BC_NONE; d.lo[i] = 0; d.hi[i] = 0; }
+        d.absorb_rows = 0; d.outside = 0; d.inlet = JT_INLET_OFF; d.ia = 0; d.ipos = 1;
+        d.backstop = 0; d.coords[0] = d.coords[1] = nullptr; d.nc[0] = d.nc[1] = 0;
+        return JT_OK;
+    }
+    for (int i = 0; i < 3; ++i) {
+        JT_REQUIRE(b->axis_kind[i] >= JT_BC_NONE && b->axis_kind[i] <= JT_BC_ABSORB,
+                   "bad boundary kind %d on axis %d", b->axis_kind[i], i);
+        d.kind[i] = b->axis_kind[i];
+        d.lo[i] = b->lo[i];
+        d.hi[i] = b->hi[i];
+    }
+    d.absorb_rows = b->absorb_rows;
+    d.outside = b->outside_value;
+    JT_REQUIRE(b->inlet >= JT_INLET_OFF && b->inlet <= JT_INLET_ZERO, "bad inlet mode %d", b->inlet);
+    d.inlet = b->inlet;
+    d.ia = b->inlet_axis;
+    d.ipos = b->inlet_positive ? 1 : 0;
+    d.backstop = b->inlet_backstop;
+    if (b->inlet != JT_INLET_OFF)
+        JT_REQUIRE(b->inlet_axis >= 0 && b->inlet_axis <= 2, "bad inlet axis %d", b->inlet_axis);
+    for (int j = 0; j < 2; ++j) {
+        d.coords[j] = b->inlet_coords[j];
+        d.nc[j] = b->inlet_ncoords[j];
+        if (b->inlet == JT_INLET_GRID)
+            JT_REQUIRE(d.coords[j] != nullptr && d.nc[j] >= 1, "inlet grid coords missing");
+    }
+    return JT_OK;
+}
+
+// ---- kernels -------------------------------------------------------------
+
+// Stage-time blend table: 3 entries per step (stage origin, midpoint, end).
+// rk4.py:88-92 (f32 t + 0.5*dt, t + dt); rk2.py:165 (float64 midpoint sum).
+__global__ void k_blend_table(GridDev g, const float* __restrict__ step_times, int n_steps,
+                              float dt, int integrator, Blend* __restrict__ tab) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_steps) return;
+    const float t = step_times[s];
+    float t1 = t, t2 = t;
+    if (integrator == JT_RK4) {
+        const float dth = 0.5f * dt;
+        t1 = t + dth;
+        t2 = t + dt;
+    } else if (integrator == JT_RK2) {
+        const float dth = dt * 0.5f;
+        t1 = (float)((double)t + (double)dth);
+    }
+    tab[3 * s + 0] = time_blend(g.times, g.T, t, g.interp, g.extrap);
+    tab[3 * s + 1] = time_blend(g.times, g.T, t1, g.interp, g.extrap);
+    tab[3 * s + 2] = time_blend(g.times, g.T, t2, g.interp, g.extrap);
+}
+
+template <int INTEG, int ST>
+__global__ void __launch_bounds__(256)
+k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
+        const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
+        int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
+    const size_t dst = ids ? (size_t)ids[i] : (size_t)i;
+    for (int s = s0; s < s1; ++s) {
+        const Blend b0 = tab[3 * s];
+        if constexpr (INTEG == JT_RK4) {               // rk4.py:84-106
+            const Blend b1 = tab[3 * s + 1];
+            const Blend b2 = tab[3 * s + 2];
+            float k1[3], k2[3], k3[3], k4[3], q[3];
+            sample<ST>(g, b0, p[0], p[1], p[2], k1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k1[c];
+            sample<ST>(g, b1, q[0], q[1], q[2], k2);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k2[c];
+            sample<ST>(g, b1, q[0], q[1], q[2], k3);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dt * k3[c];
+            sample<ST>(g, b2, q[0], q[1], q[2], k4);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)                 // rk4.py:44
+                p[c] = p[c] + dt6 * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
+        } else if constexpr (INTEG == JT_RK2) {        // rk2.py:157-177
+            const Blend b1 = tab[3 * s + 1];
+            float v1[3], v2[3], q[3];
+            sample<ST>(g, b0, p[0], p[1], p[2], v1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * v1[c];
+            sample<ST>(g, b1, q[0], q[1], q[2], v2);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v2[c];
+        } else {                                        // euler.py:150-157
+            float v[3];
+            sample<ST>(g, b0, p[0], p[1], p[2], v);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
+        }
+        apply_bc(bc, p);                                // tracker.py:238-239
+        if (traj != nullptr && (s + 1) % rec_every == 0) {
+            const size_t r = (size_t)((s + 1) / rec_every);
+            float* o = traj + (r * (size_t)n + dst) * 3;
+            o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+        }
+    }
+    xs[3 * (size_t)i] = p[0];
+    xs[3 * (size_t)i + 1] = p[1];
+    xs[3 * (size_t)i + 2] = p[2];
+}
+
+template <int ST>
+__global__ void __launch_bounds__(256)
+k_sample(GridDev g, const float* __restrict__ x, int n, float t, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Blend b = time_blend(g.times, g.T, t, g.interp, g.extrap);
+    float v[3];
+    sample<ST>(g, b, x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], v);
+    out[3 * (size_t)i] = v[0];
+    out[3 * (size_t)i + 1] = v[1];
+    out[3 * (size_t)i + 2] = v[2];
+}
+
+__global__ void __launch_bounds__(256) k_boundary(BcDev bc, float* __restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2]};
+    apply_bc(bc, p);
+    x[3 * (size_t)i] = p[0];
+    x[3 * (size_t)i + 1] = p[1];
+    x[3 * (size_t)i + 2] = p[2];
+}
+
+template <int INTEG>
+static void launch_track_st(int st, dim3 grid, dim3 block, hipStream_t s, const GridDev& g,
+                            const BcDev& bc, float* xs, int n, const Blend* tab, int s0, int s1,
+                            float dt, float dth, float dt6, int rec, float* traj, const int* ids) {
+    switch (st) {
+    case ST_F32x3: hipLaunchKernelGGL((k_track<INTEG, ST_F32x3>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    case ST_F32x4: hipLaunchKernelGGL((k_track<INTEG, ST_F32x4>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    case ST_BF16x4: hipLaunchKernelGGL((k_track<INTEG, ST_BF16x4>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    default: hipLaunchKernelGGL((k_track<INTEG, ST_F16x4>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    }
+}
+
+}  // namespace jt
+
+using namespace jt;
+
+extern "C" int jt_grid_sample(const jt_grid_field* field, const float* x, int64_t n, float t,
+                              float* out, void* stream) {
+    GridDev g;
+    int st;
+    int rc = make_grid_dev(field, g, st);
+    if (rc) return rc;
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(x != nullptr && out != nullptr, "NULL x/out");
+    hipStream_t s = (hipStream_t)stream;
+    dim3 block(256), grid((unsigned)((n + 255) / 256));
+    switch (st) {
+    case ST_F32x3: hipLaunchKernelGGL(k_sample<ST_F32x3>, grid, block, 0, s, g, x, (int)n, t, out); break;
+    case ST_F32x4: hipLaunchKernelGGL(k_sample<ST_F32x4>, grid, block, 0, s, g, x, (int)n, t, out); break;
+    case ST_BF16x4: hipLaunchKernelGGL(k_sample<ST_BF16x4>, grid, block, 0, s, g, x, (int)n, t, out); break;
+    default: hipLaunchKernelGGL(k_sample<ST_F16x4>, grid, block, 0, s, g, x, (int)n, t, out); break;
+    }
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+extern "C" int jt_apply_boundary(const jt_boundary* bc, float* x, int64_t n, void* stream) {
+    BcDev d;
+    JT_REQUIRE(bc != nullptr, "boundary descriptor is NULL");
+    int rc = make_bc_dev(bc, d);
+    if (rc) return rc;
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(x != nullptr, "NULL x");
+    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, d, x, (int)n);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+static int default_steps_per_launch(int64_t n) {
+    // Keep one launch to a fraction of a second at the largest sizes.
+    if (n >= (1 << 22)) return 50;
+    return 1000;
+}
+
+extern "C" size_t jt_track_grid_workspace_size(int64_t n, int32_t n_steps, const jt_track_opts* opts) {
+    (void)n; (void)opts;
+    Arena a(nullptr, 0);
+    a.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
+    return a.used + 256;
+}
+
+extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, int32_t integrator,
+                             float* x, int64_t n, const float* step_times, float dt, int32_t n_steps,
+                             int32_t record_every, float* traj, const jt_track_opts* opts,
+                             void* ws, size_t ws_bytes, void* stream) {
+    GridDev g;
+    int st;
+    int rc = make_grid_dev(field, g, st);
+    if (rc) return rc;
+    BcDev d;
+    rc = make_bc_dev(bc, d);
+    if (rc) return rc;
+    JT_REQUIRE(integrator >= JT_EULER && integrator <= JT_RK4, "bad integrator %d", integrator);
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    JT_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
+    JT_REQUIRE(record_every >= 1, "record_every must be >= 1");
+    JT_REQUIRE(n == 0 || x != nullptr, "NULL x");
+    JT_REQUIRE(n_steps == 0 || step_times != nullptr, "NULL step_times");
+    hipStream_t s = (hipStream_t)stream;
+    if (traj != nullptr && n > 0)
+        JT_HIP(hipMemcpyAsync(traj, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (n == 0 || n_steps == 0) return JT_OK;
+
+    Arena arena(ws, ws_bytes);
+    Blend* tab = arena.take<Blend>((size_t)3 * n_steps);
+    if (tab == nullptr || !arena.ok())
+        return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu",
+                    jt_track_grid_workspace_size(n, n_steps, opts), ws_bytes);
+
+    hipLaunchKernelGGL(k_blend_table, dim3((n_steps + 127) / 128), dim3(128), 0, s, g, step_times,
+                       n_steps, dt, integrator, tab);
+    JT_HIP(hipGetLastError());
+
+    int spl = (opts && opts->steps_per_launch > 0) ? opts->steps_per_launch : default_steps_per_launch(n);
+    const float dth = 0.5f * dt;       // rk4.py:90 (rk2.py:169 is dt*0.5, the same value)
+    const float dt6 = dt / 6.0f;       // rk4.py:44
+    dim3 block(256), grid((unsigned)((n + 255) / 256));
+    for (int s0 = 0; s0 < n_steps; s0 += spl) {
+        int s1 = s0 + spl < n_steps ? s0 + spl : n_steps;
+        switch (integrator) {
+        case JT_RK4: launch_track_st<JT_RK4>(st, grid, block, s, g, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, nullptr); break;
+        case JT_RK2: launch_track_st<JT_RK2>(st, grid, block, s, g, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, nullptr); break;
+        default: launch_track_st<JT_EULER>(st, grid, block, s, g, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, nullptr); break;
+        }
+        JT_HIP(hipGetLastError());
+    }
+    return JT_OK;
+}

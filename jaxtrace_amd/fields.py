@@ -1,0 +1,208 @@
+"""Velocity fields backed by device-resident snapshots and HIP sampling kernels.
+
+Mirrors the reference's field classes (same constructor arguments, same
+errors) while holding torch tensors on the ROCm device:
+
+* ``StructuredGridSampler``      -- jaxtrace/fields/structured.py:65-357
+* ``StructuredTimeSeriesField``  -- the configs' time-dependent structured
+  field: TimeSeriesField's temporal blend (time_series.py:155-245) composed
+  with StructuredGridSampler's trilinear interpolation (structured.py:167-250).
+  The reference has no single class for it; this is that composition.
+
+Snapshots may be stored in float32 (zero-copy layout (T,Nx,Ny,Nz,3)) or in
+bfloat16/float16 (padded to 4 components, 8 bytes per node) to halve the
+gather traffic; all arithmetic stays float32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+F32 = np.float32
+_STORAGE = {"float32": (L.JT_DTYPE_F32, torch.float32), "bfloat16": (L.JT_DTYPE_BF16, torch.bfloat16),
+            "float16": (L.JT_DTYPE_F16, torch.float16)}
+
+
+def _as_f32_device(a, dev) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        return a.detach().to(device=dev, dtype=torch.float32).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=F32))).to(dev)
+
+
+def _ensure_positions(positions, dev):
+    """(N,2|3) -> (N,3) float32 on device (structured.py:49-61, tracker.py:72-87)."""
+    is_torch = isinstance(positions, torch.Tensor)
+    x = _as_f32_device(positions, dev)
+    if x.ndim == 1:
+        x = x.reshape(1, -1)
+    if x.ndim != 2:
+        raise ValueError(f"Positions must be 2D array, got shape {tuple(x.shape)}")
+    if x.shape[1] == 2:
+        x = torch.cat([x, torch.zeros((x.shape[0], 1), dtype=torch.float32, device=dev)], dim=1)
+    elif x.shape[1] != 3:
+        raise ValueError(f"Positions must have 2 or 3 columns, got {x.shape[1]}")
+    return x.contiguous(), is_torch
+
+
+class _GridSeries:
+    """Shared machinery: a (T, Nx, Ny, Nz, 3) snapshot series on a uniform grid."""
+
+    kind = "grid"
+
+    def _setup(self, data, times, grid_x, grid_y, grid_z, bounds_handling, interpolation,
+               extrapolation, storage, layout, device):
+        self.device = torch.device(device) if device is not None else L.device()
+        mode = str(bounds_handling).lower()
+        if mode not in L.JT_BOUNDS:
+            raise ValueError(f"Unsupported bounds_handling: {bounds_handling}")
+        interp = str(interpolation).lower()
+        extra = str(extrapolation).lower()
+        if interp not in ("nearest", "linear", "cubic"):
+            raise ValueError(f"Unsupported interpolation: {interpolation}")
+        if extra not in L.JT_EXTRAP:
+            raise ValueError(f"Unsupported extrapolation: {extrapolation}")
+        if storage not in _STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_STORAGE)}, got {storage!r}")
+        if layout not in ("packed", "padded"):
+            raise ValueError("layout must be 'packed' or 'padded'")
+        gx = np.asarray(grid_x.cpu() if isinstance(grid_x, torch.Tensor) else grid_x, dtype=F32)
+        gy = np.asarray(grid_y.cpu() if isinstance(grid_y, torch.Tensor) else grid_y, dtype=F32)
+        gz = np.asarray(grid_z.cpu() if isinstance(grid_z, torch.Tensor) else grid_z, dtype=F32)
+        T, Nx, Ny, Nz, D = data.shape
+        if D != 3:
+            raise ValueError(f"velocity_data must have 3 components, got {D}")
+        for name, g, n in (("grid_x", gx, Nx), ("grid_y", gy, Ny), ("grid_z", gz, Nz)):
+            if g.shape != (n,):
+                raise ValueError(f"{name} shape {g.shape} doesn't match velocity_data ({n})")
+        tt = np.asarray(times.cpu() if isinstance(times, torch.Tensor) else times, dtype=F32).reshape(-1)
+        if tt.shape != (T,):
+            raise ValueError(f"times shape {tt.shape} != (T,) with T={T}")
+        self.Nx, self.Ny, self.Nz, self.T = Nx, Ny, Nz, T
+        # structured.py:123-130: dx = float(grid_x[1] - grid_x[0]) in f32
+        self.dx = F32(gx[1] - gx[0]) if Nx > 1 else F32(1.0)
+        self.dy = F32(gy[1] - gy[0]) if Ny > 1 else F32(1.0)
+        self.dz = F32(gz[1] - gz[0]) if Nz > 1 else F32(1.0)
+        self.x_min, self.x_max = float(gx[0]), float(gx[-1])
+        self.y_min, self.y_max = float(gy[0]), float(gy[-1])
+        self.z_min, self.z_max = float(gz[0]), float(gz[-1])
+        self.grid_x, self.grid_y, self.grid_z = gx, gy, gz
+        self.bounds_handling, self.interpolation, self.extrapolation = mode, interp, extra
+        self.storage, self.layout = storage, layout
+        dt_code, tdt = _STORAGE[storage]
+        ncomp = 3 if (storage == "float32" and layout == "packed") else 4
+        if ncomp == 3:
+            snaps = data.to(device=self.device, dtype=torch.float32).contiguous()
+        else:
+            snaps = torch.zeros((T, Nx, Ny, Nz, 4), dtype=tdt, device=self.device)
+            snaps[..., :3] = data.to(device=self.device, dtype=torch.float32).to(tdt)
+        self._snaps = snaps
+        self._times_dev = torch.from_numpy(tt.copy()).to(self.device)
+        self.times = tt
+        self.t_min, self.t_max = float(tt[0]), float(tt[-1])
+        d = L.GridField()
+        d.data = snaps.data_ptr()
+        d.times = self._times_dev.data_ptr()
+        d.n_snap, d.nx, d.ny, d.nz = T, Nx, Ny, Nz
+        d.dtype, d.ncomp = dt_code, ncomp
+        d.origin[:] = [F32(gx[0]), F32(gy[0]), F32(gz[0])]
+        d.spacing[:] = [self.dx, self.dy, self.dz]
+        d.bounds_mode = L.JT_BOUNDS[mode]
+        d.time_interp = L.JT_TIME.get(interp, 0)
+        d.time_extrap = L.JT_EXTRAP[extra]
+        self._desc = d
+
+    # -- kernel descriptor ---------------------------------------------------
+    def descriptor(self) -> L.GridField:
+        if self.interpolation == "cubic":
+            raise NotImplementedError("cubic interpolation not supported with JAX")
+        return self._desc
+
+    @property
+    def snapshot_bytes_per_node(self) -> int:
+        return self._snaps.element_size() * 3
+
+    def _sample(self, positions, t):
+        lib = L.load()
+        x, is_torch = _ensure_positions(positions, self.device)
+        out = torch.empty_like(x)
+        L.check(lib.jt_grid_sample(C.byref(self.descriptor()), L.ptr(x), x.shape[0], C.c_float(F32(t)),
+                                   L.ptr(out), L.stream_handle()), "jt_grid_sample")
+        if is_torch:
+            return out
+        return out.cpu().numpy()
+
+    def get_spatial_bounds(self):
+        return (np.array([self.x_min, self.y_min, self.z_min], F32),
+                np.array([self.x_max, self.y_max, self.z_max], F32))
+
+
+class StructuredGridSampler(_GridSeries):
+    """structured.py:65-357: time-independent trilinear sampler.
+
+    ``sample(positions)`` returns trilinear values with 'clamp' | 'zero' |
+    'nan' | 'periodic' bounds handling.  Stored as a one-snapshot series
+    whose temporal blend is an exact copy."""
+
+    def __init__(self, velocity_data, grid_x, grid_y, grid_z, bounds_handling: str = "clamp",
+                 grid_meta=None, storage: str = "float32", layout: str = "packed", device=None):
+        v = velocity_data if isinstance(velocity_data, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(np.asarray(velocity_data, dtype=F32)))
+        if v.ndim != 4:
+            raise ValueError(f"velocity_data must be 4D (Nx,Ny,Nz,3), got {tuple(v.shape)}")
+        self._setup(v[None], np.zeros(1, F32), grid_x, grid_y, grid_z, bounds_handling, "linear",
+                    "constant", storage, layout, device)
+        self.grid_meta = grid_meta
+
+    def sample(self, positions):
+        return self._sample(positions, 0.0)
+
+
+class StructuredTimeSeriesField(_GridSeries):
+    """Time-dependent structured field: temporal blend (time_series.py:155-245)
+    of (T, Nx, Ny, Nz, 3) snapshots composed with trilinear interpolation in
+    space (structured.py:167-250).
+
+    ``sample_at_positions(positions, t)`` is the reference composition
+    ``_trilinear_jax(_sample_all_nodes_at_time_jax(t))``; the kernel blends the
+    8 gathered corners with the identical per-node arithmetic."""
+
+    def __init__(self, data, times, grid_x, grid_y, grid_z, bounds_handling: str = "clamp",
+                 interpolation: str = "linear", extrapolation: str = "constant",
+                 storage: str = "float32", layout: str = "packed", device=None,
+                 _source_info: Optional[dict] = None):
+        v = data if isinstance(data, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(np.asarray(data, dtype=F32)))
+        if v.ndim == 3:
+            # TimeSeriesField layout (T, N, 3) with N = Nx*Ny*Nz in C order
+            n = (len(grid_x), len(grid_y), len(grid_z))
+            if v.shape[1] != n[0] * n[1] * n[2]:
+                raise ValueError(f"data has {v.shape[1]} nodes, grid has {n[0] * n[1] * n[2]}")
+            v = v.reshape(v.shape[0], *n, 3)
+        if v.ndim != 5:
+            raise ValueError(f"data must be (T,Nx,Ny,Nz,3) or (T,N,3), got {tuple(v.shape)}")
+        self._setup(v, times, grid_x, grid_y, grid_z, bounds_handling, interpolation,
+                    extrapolation, storage, layout, device)
+        self._source_info = _source_info or {}
+
+    def sample_at_positions(self, positions, t: float):
+        return self._sample(positions, t)
+
+    def __call__(self, a, b):
+        if np.ndim(a) == 0:
+            return self.sample_at_positions(b, float(a))
+        return self.sample_at_positions(a, float(b))
+
+    def get_time_bounds(self):
+        return (self.t_min, self.t_max)
+
+    def with_storage(self, storage: str, layout: str = "packed") -> "StructuredTimeSeriesField":
+        """Same field, snapshots re-stored as ``storage`` (RNE from float32)."""
+        src = self._snaps[..., :3].to(torch.float32)
+        return StructuredTimeSeriesField(src, self.times, self.grid_x, self.grid_y, self.grid_z,
+                                         self.bounds_handling, self.interpolation,
+                                         self.extrapolation, storage, layout, self.device)

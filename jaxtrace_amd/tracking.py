@@ -1,0 +1,200 @@
+"""Particle tracking driver: ``create_tracker`` / ``ParticleTracker.track_particles``.
+
+Mirrors jaxtrace/tracking/tracker.py (``TrackerOptions`` :107-153,
+``track_particles`` :320-358, ``create_tracker`` :810-855) and
+jaxtrace/tracking/particles.py (``Trajectory`` :82-131).  The whole time loop
+runs in one C-ABI call (``jt_track_grid``) whose kernels keep each particle's
+state in registers across stages and steps, with the boundary fused into the
+step.  Fields and boundaries must be this package's device-backed kinds;
+arbitrary Python callables are rejected with ``TypeError`` (there is no
+second, host-side code path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Any, Callable, Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .boundary import BoundaryCondition
+from .fields import _GridSeries, _ensure_positions
+
+F32 = np.float32
+_INTEGRATORS = {"euler": L.JT_EULER, "rk2": L.JT_RK2, "rk4": L.JT_RK4}
+
+
+@dataclass
+class TrackerOptions:
+    """tracker.py:107-153 plus device-side extensions.
+
+    Reference fields are accepted with their meaning where it applies to a
+    single fused GPU program (the JIT/scan switches have nothing to select).
+    Extensions:
+      record_every       keep x0 and every k-th step (compacted output)
+      record_final_only  keep only x0 and the final positions
+      return_device      Trajectory arrays stay torch tensors on the GPU
+      steps_per_launch   steps fused into one kernel launch (0 = library default)
+      sort_every         re-sort particles by grid cell every k steps (0 = off)
+    """
+    max_memory_gb: float = 8.0
+    batch_size: Optional[int] = None
+    oom_recovery: bool = True
+    use_jax_jit: bool = True
+    static_compilation: bool = True
+    use_scan_jit: bool = True
+    device_put_inputs: bool = True
+    record_on_device: bool = True
+    max_batch_size: int = 100_000
+    min_batch_size: int = 100
+    record_velocities: bool = False
+    recording_interval: int = 1
+    progress_callback: Optional[Callable[[float], None]] = None
+    progress_style: str = "auto"
+    progress_desc: str = "Tracking"
+    progress_update_every: float = 0.05
+    adaptive_dt: bool = False
+    error_tolerance: float = 1e-6
+    # extensions
+    record_every: int = 1
+    record_final_only: bool = False
+    return_device: bool = False
+    steps_per_launch: int = 0
+    sort_every: int = 0
+
+
+class Trajectory:
+    """particles.py:82-131: positions (T,N,3) f32, times (T,), velocities, metadata."""
+
+    def __init__(self, positions, times, velocities=None, metadata: Optional[Dict[str, Any]] = None):
+        self.positions = positions
+        self.times = np.asarray(times, dtype=F32)
+        self.velocities = velocities
+        self.metadata = dict(metadata or {})
+        shp = tuple(positions.shape)
+        if len(shp) != 3 or shp[2] != 3:
+            raise ValueError(f"Positions must have shape (T,N,3), got {shp}")
+        self.T, self.N, self.D = shp
+        if self.times.shape != (self.T,):
+            raise ValueError(f"Times shape {self.times.shape} doesn't match trajectory length {self.T}")
+        if velocities is not None and tuple(velocities.shape) != shp:
+            raise ValueError("Velocities shape doesn't match positions shape")
+        self.metadata.setdefault("format_version", "1.0")
+        self.metadata.setdefault("dtype", "float32")
+        self.metadata.setdefault("coordinate_system", "cartesian")
+
+    def __len__(self):
+        return self.T
+
+    @property
+    def num_particles(self):
+        return self.N
+
+    @property
+    def num_timesteps(self):
+        return self.T
+
+    @property
+    def duration(self):
+        return float(self.times[-1] - self.times[0]) if self.T > 1 else 0.0
+
+    def get_positions_at_time(self, t_idx: int):
+        return self.positions[t_idx]
+
+    def get_final_positions(self):
+        return self.positions[-1]
+
+
+@dataclass
+class ParticleTracker:
+    """tracker.py:155-535, driving the fused HIP kernels."""
+    integrator_name: str
+    field: Any
+    boundary: BoundaryCondition
+    options: TrackerOptions = field(default_factory=TrackerOptions)
+
+    def __post_init__(self):
+        if self.integrator_name not in _INTEGRATORS:
+            raise ValueError(f"Unknown integrator: {self.integrator_name}. "
+                             f"Available: {list(_INTEGRATORS)}")
+
+    @property
+    def integrator(self):
+        return self.integrator_name
+
+    def _opts(self):
+        o = L.TrackOpts()
+        o.steps_per_launch = int(self.options.steps_per_launch)
+        o.sort_every = int(self.options.sort_every)
+        o.block = 0
+        return o
+
+    def track_particles(self, initial_positions, time_span: Tuple[float, float], n_timesteps: int,
+                        dt: Optional[float] = None, **kwargs) -> Trajectory:
+        """tracker.py:320-358."""
+        lib = L.load()
+        fld = self.field
+        dev = fld.device
+        x, _ = _ensure_positions(initial_positions, dev)
+        n = x.shape[0]
+        t_start, t_end = time_span
+        times = np.linspace(float(t_start), float(t_end), int(n_timesteps), dtype=np.float32)
+        if dt is None:
+            dt = float(times[1] - times[0]) if n_timesteps > 1 else 0.0
+        else:
+            dt = float(dt)
+        n_steps = max(int(n_timesteps) - 1, 0)
+        opt = self.options
+        rec = n_steps if (opt.record_final_only and n_steps > 0) else max(int(opt.record_every), 1)
+        n_rec = 1 + n_steps // rec
+        rec_idx = np.arange(n_rec) * rec
+        out_times = times[rec_idx] if n_steps > 0 else times[:1]
+
+        step_times = torch.from_numpy(times[:-1].copy() if n_steps > 0 else np.zeros(1, F32)).to(dev)
+        desc = fld.descriptor()
+        bc = self.boundary.descriptor(dev)
+        o = self._opts()
+        ws_bytes = int(lib.jt_track_grid_workspace_size(n, n_steps, C.byref(o)))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        traj = torch.empty((n_rec, n, 3), dtype=torch.float32, device=dev)
+        state = x.clone()
+        t0 = time.perf_counter()
+        L.check(lib.jt_track_grid(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
+                                  L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
+                                  rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
+                                  L.stream_handle()), "jt_track_grid")
+        velocities = None
+        if opt.record_velocities:
+            velocities = torch.empty_like(traj)
+            for r in range(n_rec):
+                velocities[r] = fld._sample(traj[r], float(out_times[r]))
+        if opt.progress_callback is not None:
+            opt.progress_callback(1.0)
+        meta = {"integrator": self.integrator_name, "batch_processing": "single_batch",
+                "n_particles": n, "n_timesteps": int(n_timesteps), "dt": dt,
+                "record_every": rec, "backend": "hip-gfx950", "field_storage": fld.storage}
+        if not opt.return_device:
+            traj = traj.cpu().numpy()
+            if velocities is not None:
+                velocities = velocities.cpu().numpy()
+        meta["wall_s"] = time.perf_counter() - t0
+        return Trajectory(traj, out_times, velocities, meta)
+
+
+def create_tracker(integrator_name: str, field, boundary_condition, **options) -> ParticleTracker:
+    """tracker.py:810-855."""
+    if integrator_name not in _INTEGRATORS:
+        raise ValueError(f"Unknown integrator: {integrator_name}. Available: {list(_INTEGRATORS)}")
+    if not isinstance(field, _GridSeries):
+        raise TypeError("field must be a device-backed jaxtrace_amd field "
+                        "(StructuredGridSampler, StructuredTimeSeriesField); "
+                        f"got {type(field).__name__}")
+    if not isinstance(boundary_condition, BoundaryCondition):
+        if callable(boundary_condition):
+            raise TypeError("boundary_condition must come from a jaxtrace_amd boundary factory "
+                            "(periodic/reflective/clamping/absorbing/mixed/continuous_inlet)")
+        raise TypeError("boundary_condition must be callable")
+    return ParticleTracker(integrator_name, field, boundary_condition, TrackerOptions(**options))

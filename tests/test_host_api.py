@@ -1,0 +1,56 @@
+"""Host-side behaviour of the reference-shaped API that needs no GPU:
+argument checking and error types (tracker.py:810-855, boundary.py:39-86,
+377-438, 1444-1601)."""
+import numpy as np
+import pytest
+
+import jaxtrace_amd as jt
+from jaxtrace_amd import _lib as L
+
+
+def test_unknown_integrator_is_value_error():
+    with pytest.raises(ValueError):
+        jt.create_tracker("rk5", object(), jt.periodic_boundary([[0, 0, 0], [1, 1, 1]]))
+
+
+def test_python_callables_are_rejected_with_type_error():
+    bc = jt.periodic_boundary([[0, 0, 0], [1, 1, 1]])
+    with pytest.raises(TypeError):
+        jt.create_tracker("rk4", lambda x, t: x, bc)
+
+
+def test_bounds_validation():
+    with pytest.raises(ValueError):
+        jt.periodic_boundary([[0, 0, 0], [1, 0, 1]])
+    with pytest.raises(ValueError):
+        jt.periodic_boundary([0, 1, 2])
+    b = jt.reflective_boundary([0, 1, 0, 2, 0, 3])
+    np.testing.assert_array_equal(b.bounds, [[0, 0, 0], [1, 2, 3]])
+    assert b.__name__ == "_reflective_bc"
+
+
+def test_mixed_boundary_kinds_and_errors():
+    b = jt.mixed_boundary([[0, 0, 0], [1, 1, 1]], {"y": "reflective", "z": "clamping"})
+    assert b.axis_kind == (L.JT_BC_PERIODIC, L.JT_BC_REFLECTIVE, L.JT_BC_CLAMP)
+    with pytest.raises(ValueError):
+        jt.mixed_boundary([[0, 0, 0], [1, 1, 1]], {"x": "sticky"})
+
+
+def test_continuous_inlet_descriptor():
+    with pytest.raises(ValueError):
+        jt.continuous_inlet_boundary_factory([[0, 0, 0], [1, 1, 1]], flow_axis="w")
+    b = jt.continuous_inlet_boundary_factory([[0, 0, 0], [1, 1, 1]], "x", "positive", "grid",
+                                             {"x": 64, "y": 64, "z": 32})
+    assert b.inlet["mode"] == L.JT_INLET_GRID and b.inlet["axis"] == 0
+    # the reference passes f32 bound scalars, so NumPy 2 evaluates linspace in f32
+    f0, f1 = np.float32(0), np.float32(1)
+    np.testing.assert_array_equal(b.inlet["coords"][1], np.linspace(f0, f1, 32, dtype=np.float32))
+    assert b.inlet["backstop"] == np.float32(0.001)
+    with pytest.raises(NotImplementedError):
+        jt.continuous_inlet_boundary_factory([[0, 0, 0], [1, 1, 1]], inlet_distribution="uniform")
+
+
+def test_tracker_options_defaults_match_reference():
+    o = jt.TrackerOptions()
+    assert o.max_batch_size == 100_000 and o.min_batch_size == 100 and o.max_memory_gb == 8.0
+    assert o.record_every == 1 and not o.return_device

@@ -79,12 +79,16 @@ extern "C" {
  * TimeSeriesField's temporal blend (time_series.py:155-245) with
  * StructuredGridSampler's trilinear interpolation (structured.py:167-250). */
 typedef struct jt_grid_field {
-    const void*  data;        /* device (n_snap, nx, ny, nz, ncomp), C order   */
+    const void*  data;        /* device (n_snap, nx, ny, nz(+1), ncomp), C order */
     const float* times;       /* device (n_snap,) f32, non-decreasing          */
     int32_t n_snap;
     int32_t nx, ny, nz;
     int32_t dtype;            /* JT_DTYPE_*                                    */
     int32_t ncomp;            /* 3 (packed xyz) or 4 (xyz + pad)               */
+    int32_t z_halo;           /* 1: each snapshot holds nz+1 z-planes; plane nz
+                                 repeats plane 0 (periodic bounds) or plane
+                                 nz-1 (other bounds), so the two z-corners of a
+                                 cell are one contiguous load                  */
     float   origin[3];        /* f32 grid_x[0], grid_y[0], grid_z[0]           */
     float   spacing[3];       /* f32 grid[1]-grid[0] (1 when n == 1)           */
     int32_t bounds_mode;      /* JT_BOUNDS_*                                   */
@@ -145,6 +149,123 @@ int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc,
                   int32_t record_every, float* traj,
                   const jt_track_opts* opts, void* ws, size_t ws_bytes,
                   void* stream);
+
+/* ---- hash-grid neighbours and SPH density ------------------------------- */
+/* cell-key packing: neighbors.py:83-91.  The reference never enables
+ * jax_enable_x64, so its int64 keys are int32 and `cx << 42` is 0: the
+ * as-shipped 3-D key is (cy<<21)|cz (JT_KEY_X32).  JT_KEY_X64 is the
+ * intended (cx<<42)|(cy<<21)|cz. */
+#define JT_KEY_X32 0
+#define JT_KEY_X64 1
+
+/* SPH kernels: sph.py:227-262 */
+#define JT_SPH_CUBIC    0
+#define JT_SPH_WENDLAND 1
+#define JT_SPH_GAUSSIAN 2
+
+/* CSR hash grid of n points (HashGridNeighbors, neighbors.py:39-103).
+ * origin = per-axis min of the points, inv = f32(1/max(cell_size,1e-12)),
+ * dims = floor((max-origin)*inv)+1 per axis.  Cells are dense ids:
+ * X64: (cx*dims[1]+cy)*dims[2]+cz; X32: cy*dims[2]+cz (2-D: cx*dims[1]+cy
+ * and cy).  The caller allocates the arrays (jt_hashgrid_num_cells). */
+typedef struct jt_hashgrid {
+    int32_t dim;              /* 2 or 3                                        */
+    int32_t key_mode;         /* JT_KEY_*                                      */
+    float   origin[3];
+    float   inv;
+    int32_t dims[3];
+    int64_t n_points;
+    int32_t* order;           /* device (n): stable argsort of keys (pt_indices_sorted) */
+    float*   sorted_pos;      /* device (n, dim): positions in cell order      */
+    int32_t* cell_start;      /* device (n_cells)                              */
+    int32_t* cell_end;        /* device (n_cells), exclusive                   */
+} jt_hashgrid;
+
+/* Per-axis min then max of points (n, dim) into out_minmax (2*dim floats,
+ * device).  ws: >= 24 bytes of device scratch. */
+int jt_points_bounds(const float* P, int64_t n, int32_t dim, float* out_minmax,
+                     void* ws, size_t ws_bytes, void* stream);
+int64_t jt_hashgrid_num_cells(const jt_hashgrid* g);
+size_t  jt_hashgrid_workspace_size(int64_t n);
+/* Replaces HashGridNeighbors.__post_init__ (neighbors.py:61-103). */
+int jt_hashgrid_build(const jt_hashgrid* g, const float* P, void* ws, size_t ws_bytes,
+                      void* stream);
+/* Replaces HashGridNeighbors.query_many (neighbors.py:109-168): (m, K) int32
+ * indices and f32 distances, nearest first, stable in candidate order;
+ * invalid slots carry pt_indices_sorted[0] and +inf. */
+int jt_hashgrid_query(const jt_hashgrid* g, const float* Q, int64_t m, float radius,
+                      int32_t per_cell_cap, int32_t max_neighbors, int32_t* out_idx,
+                      float* out_dist, void* stream);
+/* Replaces SPHDensityEstimator._evaluate_density (sph.py:269-279): the sum of
+ * m_j W(d_j, h_j) over the query's first max_neighbors neighbours within
+ * radius.  masses/smoothing may be NULL (1 and h_mean). */
+int jt_sph_eval(const jt_hashgrid* g, const float* Q, int64_t m, float radius,
+                int32_t per_cell_cap, int32_t max_neighbors, const float* masses,
+                const float* smoothing, float h_mean, int32_t kernel, float* out_rho,
+                void* stream);
+
+/* ---- Gaussian KDE --------------------------------------------------------- */
+/* Replaces KDEEstimator._evaluate_grid's all-pairs eval_chunk
+ * (kde.py:244-270): out[i] = mean_n (2pi)^(-dim/2) exp(-0.5|Q_i-P_n|^2 f32(1/h^2)),
+ * exhaustive over all n points (the reference algorithm). */
+int jt_kde_eval(const float* P, int64_t n, int32_t dim, const float* Q, int64_t m,
+                float h, float* out, void* stream);
+/* The same density on the structured grid of KDEEstimator.evaluate_3d
+ * (kde.py:188-240; out is (ny, nx, nz), 'xy' meshgrid order), summing only
+ * points within `cutoff` of each grid point, found through a 3-D JT_KEY_X64
+ * hash grid of the points whose cell size is >= cutoff.  n_total is the
+ * number of points (the mean's divisor). */
+int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
+                       const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
+                       float cutoff, float* out, void* stream);
+
+/* ---- unstructured FEM field: element octree + P1 tetrahedra ---------------- */
+/* Opaque host octree (OctreeMeshOptimized, octree_fem_interpolator_optimized.py:18-38). */
+typedef struct jt_octree jt_octree;
+
+/* Replaces build_octree_mesh_optimized (octree_fem_interpolator_optimized.py:41-215),
+ * on the HOST: points (n_points,3) f32 and conn (n_elems,4) int32 are host
+ * arrays.  The arrays it produces are bit-identical to the reference's. */
+int  jt_octree_build(const float* points, int64_t n_points, const int32_t* conn, int64_t n_elems,
+                     int32_t max_per_leaf, int32_t max_depth, jt_octree** out);
+int  jt_octree_info(const jt_octree* t, int64_t* n_nodes, int32_t* max_elems, int32_t* tree_depth);
+/* Copies the flattened arrays into caller-allocated HOST buffers (any may be NULL):
+ * nodes_min/max (n_nodes,3) f32, nodes_elements (n_nodes,max_elems) int32 (-1 pad),
+ * counts (n_nodes) int32, children (n_nodes,8) int32, is_leaf (n_nodes) u8,
+ * element_bounds (n_elems,2,3) f32, centroids (n_elems,3) f32. */
+int  jt_octree_export(const jt_octree* t, float* nodes_min, float* nodes_max, int32_t* nodes_elements,
+                      int32_t* counts, int32_t* children, uint8_t* is_leaf, float* element_bounds,
+                      float* centroids);
+void jt_octree_free(jt_octree* t);
+
+/* Device-resident octree field: the exported arrays plus nodal snapshots. */
+typedef struct jt_octree_field {
+    const float*   points;            /* (n_points, 3)             */
+    const int32_t* conn;              /* (n_elems, 4)              */
+    const float*   element_bounds;    /* (n_elems, 2, 3)           */
+    const float*   nodes_min;         /* (n_nodes, 3)              */
+    const float*   nodes_max;
+    const int32_t* nodes_elements;    /* (n_nodes, max_elems)      */
+    const int32_t* nodes_elem_counts; /* (n_nodes)                 */
+    const int32_t* nodes_children;    /* (n_nodes, 8)              */
+    const uint8_t* nodes_is_leaf;     /* (n_nodes)                 */
+    int64_t n_points, n_elems, n_nodes;
+    int32_t max_elems, max_depth;
+    const void*  data;                /* (n_snap, n_points, 3) f32 */
+    const float* times;               /* (n_snap,) f32             */
+    int32_t n_snap, time_interp, time_extrap;
+} jt_octree_field;
+
+/* Replaces OctreeFEMTimeSeriesFieldOptimized.sample_at_positions
+ * (octree_fem_time_series_optimized.py:87-113).  found (nullable) receives 1
+ * where a containing element was found, 0 where the fallback was used. */
+int jt_octree_sample(const jt_octree_field* field, const float* x, int64_t n, float t, float* out,
+                     int32_t* found, void* stream);
+size_t jt_track_octree_workspace_size(int64_t n, int32_t n_steps);
+/* jt_track_grid for an octree field (same arguments and recording). */
+int jt_track_octree(const jt_octree_field* field, const jt_boundary* bc, int32_t integrator, float* x,
+                    int64_t n, const float* step_times, float dt, int32_t n_steps, int32_t record_every,
+                    float* traj, const jt_track_opts* opts, void* ws, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

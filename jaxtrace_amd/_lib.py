@@ -34,7 +34,7 @@ class GridField(C.Structure):
     _fields_ = [
         ("data", C.c_void_p), ("times", C.c_void_p), ("n_snap", C.c_int32),
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-        ("dtype", C.c_int32), ("ncomp", C.c_int32),
+        ("dtype", C.c_int32), ("ncomp", C.c_int32), ("z_halo", C.c_int32),
         ("origin", c_float3), ("spacing", c_float3),
         ("bounds_mode", C.c_int32), ("time_interp", C.c_int32), ("time_extrap", C.c_int32),
     ]
@@ -47,6 +47,30 @@ class Boundary(C.Structure):
         ("inlet", C.c_int32), ("inlet_axis", C.c_int32), ("inlet_positive", C.c_int32),
         ("inlet_backstop", C.c_float),
         ("inlet_coords", C.c_void_p * 2), ("inlet_ncoords", C.c_int32 * 2),
+    ]
+
+
+class HashGridDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("key_mode", C.c_int32), ("origin", c_float3), ("inv", C.c_float),
+        ("dims", c_int3), ("n_points", C.c_int64), ("order", C.c_void_p), ("sorted_pos", C.c_void_p),
+        ("cell_start", C.c_void_p), ("cell_end", C.c_void_p),
+    ]
+
+
+JT_KEY_X32, JT_KEY_X64 = 0, 1
+JT_SPH = {"cubic_spline": 0, "wendland": 1, "gaussian": 2}
+
+
+class OctreeField(C.Structure):
+    _fields_ = [
+        ("points", C.c_void_p), ("conn", C.c_void_p), ("element_bounds", C.c_void_p),
+        ("nodes_min", C.c_void_p), ("nodes_max", C.c_void_p), ("nodes_elements", C.c_void_p),
+        ("nodes_elem_counts", C.c_void_p), ("nodes_children", C.c_void_p), ("nodes_is_leaf", C.c_void_p),
+        ("n_points", C.c_int64), ("n_elems", C.c_int64), ("n_nodes", C.c_int64),
+        ("max_elems", C.c_int32), ("max_depth", C.c_int32),
+        ("data", C.c_void_p), ("times", C.c_void_p),
+        ("n_snap", C.c_int32), ("time_interp", C.c_int32), ("time_extrap", C.c_int32),
     ]
 
 
@@ -67,6 +91,34 @@ _SIGS = {
     "jt_track_grid": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
                                 C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                 C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "jt_points_bounds": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.c_size_t, C.c_void_p]),
+    "jt_hashgrid_num_cells": (C.c_int64, [C.POINTER(HashGridDesc)]),
+    "jt_hashgrid_workspace_size": (C.c_size_t, [C.c_int64]),
+    "jt_hashgrid_build": (C.c_int, [C.POINTER(HashGridDesc), C.c_void_p, C.c_void_p, C.c_size_t,
+                                    C.c_void_p]),
+    "jt_hashgrid_query": (C.c_int, [C.POINTER(HashGridDesc), C.c_void_p, C.c_int64, C.c_float,
+                                    C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "jt_sph_eval": (C.c_int, [C.POINTER(HashGridDesc), C.c_void_p, C.c_int64, C.c_float, C.c_int32,
+                              C.c_int32, C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_void_p,
+                              C.c_void_p]),
+    "jt_kde_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64, C.c_float,
+                              C.c_void_p, C.c_void_p]),
+    "jt_kde_grid_cutoff": (C.c_int, [C.POINTER(HashGridDesc), C.c_int64, C.c_void_p, C.c_int32,
+                                     C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, C.c_float,
+                                     C.c_void_p, C.c_void_p]),
+    "jt_octree_build": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_void_p)]),
+    "jt_octree_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]),
+    "jt_octree_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 8),
+    "jt_octree_free": (None, [C.c_void_p]),
+    "jt_octree_sample": (C.c_int, [C.POINTER(OctreeField), C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
+    "jt_track_octree_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "jt_track_octree": (C.c_int, [C.POINTER(OctreeField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
+                                  C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
+                                  C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
 }
 
 

@@ -24,6 +24,7 @@ struct GridDev {
     const float* times;
     int T;
     int nx, ny, nz;
+    int nzs;                 // allocated z extent (nz, or nz+1 with a z halo)
     float o[3], d[3];        // origin, spacing
     float vhi[3];            // origin + f32(n-1)*d: inclusive validity bound
     float cmax[3];           // f32(max(n-1, 0)): clip bound of the grid coordinate
@@ -140,6 +141,69 @@ __device__ __forceinline__ void corner(const void* b0, const void* b1, const Ble
     }
 }
 
+// Two z-adjacent nodes (node, node+1) in one contiguous load (z-halo layout).
+struct alignas(4) F6 { float v[6]; };
+struct alignas(4) F8 { float v[8]; };
+struct alignas(4) U4 { uint32_t v[4]; };
+
+template <int ST>
+__device__ __forceinline__ void load_pair(const void* base, int node, float a[6]) {
+    if constexpr (ST == ST_F32x3) {
+        F6 t = *(const F6*)((const float*)base + (size_t)node * 3);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[c] = t.v[c];
+    } else if constexpr (ST == ST_F32x4) {
+        F8 t = *(const F8*)((const float*)base + (size_t)node * 4);
+        a[0] = t.v[0]; a[1] = t.v[1]; a[2] = t.v[2];
+        a[3] = t.v[4]; a[4] = t.v[5]; a[5] = t.v[6];
+    } else {
+        U4 t = *(const U4*)((const char*)base + (size_t)node * 8);
+        if constexpr (ST == ST_BF16x4) {
+            a[0] = bf16_to_f32((uint16_t)(t.v[0] & 0xffffu)); a[1] = bf16_to_f32((uint16_t)(t.v[0] >> 16));
+            a[2] = bf16_to_f32((uint16_t)(t.v[1] & 0xffffu));
+            a[3] = bf16_to_f32((uint16_t)(t.v[2] & 0xffffu)); a[4] = bf16_to_f32((uint16_t)(t.v[2] >> 16));
+            a[5] = bf16_to_f32((uint16_t)(t.v[3] & 0xffffu));
+        } else {
+            a[0] = f16_to_f32((uint16_t)(t.v[0] & 0xffffu)); a[1] = f16_to_f32((uint16_t)(t.v[0] >> 16));
+            a[2] = f16_to_f32((uint16_t)(t.v[1] & 0xffffu));
+            a[3] = f16_to_f32((uint16_t)(t.v[2] & 0xffffu)); a[4] = f16_to_f32((uint16_t)(t.v[2] >> 16));
+            a[5] = f16_to_f32((uint16_t)(t.v[3] & 0xffffu));
+        }
+    }
+}
+
+// Blended values of nodes (node, node+1): same per-node arithmetic as corner().
+template <int ST>
+__device__ __forceinline__ void corner_pair(const void* b0, const void* b1, const Blend& bl,
+                                            float oma, int node, float v[3], float w[3]) {
+    if (bl.mode == BL_ZERO) { v[0] = v[1] = v[2] = w[0] = w[1] = w[2] = 0.0f; return; }
+    if (bl.mode == BL_NAN) {
+        v[0] = v[1] = v[2] = w[0] = w[1] = w[2] = __int_as_float(0x7fc00000);
+        return;
+    }
+    float a[6];
+    load_pair<ST>(b0, node, a);
+    if (bl.mode == BL_COPY) {
+        v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; w[0] = a[3]; w[1] = a[4]; w[2] = a[5];
+        return;
+    }
+    float c[6];
+    load_pair<ST>(b1, node, c);
+    const float al = bl.a;
+    float r[6];
+    if (bl.mode == BL_INTERP) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r[k] = oma * a[k] + al * c[k];
+    } else if (bl.mode == BL_LEFT) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r[k] = a[k] + al * (c[k] - a[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r[k] = c[k] + al * (c[k] - a[k]);
+    }
+    v[0] = r[0]; v[1] = r[1]; v[2] = r[2]; w[0] = r[3]; w[1] = r[4]; w[2] = r[5];
+}
+
 // structured.py:176-220 for one axis.
 __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, int n,
                                            int& i0, int& i1, float& w, bool& valid) {
@@ -159,7 +223,8 @@ __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, in
 }
 
 // Blended trilinear sample u(x, t) (structured.py:167-250 after the blend).
-template <int ST>
+// ZH: z-halo layout, the (k0, k0+1) corners are one contiguous pair.
+template <int ST, bool ZH = false>
 __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
                                        float px, float py, float pz, float out[3]) {
     int i0, i1, j0, j1, k0, k1;
@@ -174,17 +239,24 @@ __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
     const void* b1 = base + (size_t)bl.i1 * sb;
     const float oma = 1.0f - bl.a;
     const int r0 = i0 * g.ny, r1 = i1 * g.ny;
-    const int n00 = (r0 + j0) * g.nz, n10 = (r1 + j0) * g.nz;
-    const int n01 = (r0 + j1) * g.nz, n11 = (r1 + j1) * g.nz;
+    const int n00 = (r0 + j0) * g.nzs, n10 = (r1 + j0) * g.nzs;
+    const int n01 = (r0 + j1) * g.nzs, n11 = (r1 + j1) * g.nzs;
     float v000[3], v100[3], v010[3], v110[3], v001[3], v101[3], v011[3], v111[3];
-    corner<ST>(b0, b1, bl, oma, n00 + k0, v000);
-    corner<ST>(b0, b1, bl, oma, n10 + k0, v100);
-    corner<ST>(b0, b1, bl, oma, n01 + k0, v010);
-    corner<ST>(b0, b1, bl, oma, n11 + k0, v110);
-    corner<ST>(b0, b1, bl, oma, n00 + k1, v001);
-    corner<ST>(b0, b1, bl, oma, n10 + k1, v101);
-    corner<ST>(b0, b1, bl, oma, n01 + k1, v011);
-    corner<ST>(b0, b1, bl, oma, n11 + k1, v111);
+    if constexpr (ZH) {
+        corner_pair<ST>(b0, b1, bl, oma, n00 + k0, v000, v001);
+        corner_pair<ST>(b0, b1, bl, oma, n10 + k0, v100, v101);
+        corner_pair<ST>(b0, b1, bl, oma, n01 + k0, v010, v011);
+        corner_pair<ST>(b0, b1, bl, oma, n11 + k0, v110, v111);
+    } else {
+        corner<ST>(b0, b1, bl, oma, n00 + k0, v000);
+        corner<ST>(b0, b1, bl, oma, n10 + k0, v100);
+        corner<ST>(b0, b1, bl, oma, n01 + k0, v010);
+        corner<ST>(b0, b1, bl, oma, n11 + k0, v110);
+        corner<ST>(b0, b1, bl, oma, n00 + k1, v001);
+        corner<ST>(b0, b1, bl, oma, n10 + k1, v101);
+        corner<ST>(b0, b1, bl, oma, n01 + k1, v011);
+        corner<ST>(b0, b1, bl, oma, n11 + k1, v111);
+    }
     const float ax = 1.0f - wx, ay = 1.0f - wy, az = 1.0f - wz;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -279,6 +351,27 @@ __device__ __forceinline__ void apply_bc(const BcDev& b, float p[3]) {
         for (int i = 0; i < 3; ++i) out = out || (p[i] < b.lo[i]) || (p[i] > b.hi[i]);
         if (out) { p[0] = p[1] = p[2] = b.outside; }
     }
+}
+
+// Stage-time blend table: 3 entries per step (stage origin, midpoint, end).
+// rk4.py:88-92 (f32 t + 0.5*dt, t + dt); rk2.py:165 (float64 midpoint sum).
+static __global__ void k_blend_table(GridDev g, const float* __restrict__ step_times, int n_steps,
+                              float dt, int integrator, Blend* __restrict__ tab) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_steps) return;
+    const float t = step_times[s];
+    float t1 = t, t2 = t;
+    if (integrator == JT_RK4) {
+        const float dth = 0.5f * dt;
+        t1 = t + dth;
+        t2 = t + dt;
+    } else if (integrator == JT_RK2) {
+        const float dth = dt * 0.5f;
+        t1 = (float)((double)t + (double)dth);
+    }
+    tab[3 * s + 0] = time_blend(g.times, g.T, t, g.interp, g.extrap);
+    tab[3 * s + 1] = time_blend(g.times, g.T, t1, g.interp, g.extrap);
+    tab[3 * s + 2] = time_blend(g.times, g.T, t2, g.interp, g.extrap);
 }
 
 // Host-side translation of the public descriptors.

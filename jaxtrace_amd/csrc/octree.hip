@@ -1,0 +1,258 @@
+// Octree point location + P1 tetrahedral interpolation of a time series of
+// nodal fields, fused into the tracking step
+// (jaxtrace/fields/octree_fem_interpolator_optimized.py:218-380,
+//  jaxtrace/fields/octree_fem_time_series_optimized.py:87-113).
+//
+// Per evaluation: fixed-depth descent over the f32 node boxes, first hit in
+// the leaf's stored element order (AABB test, then Cramer barycentrics with
+// the reference's tolerances), value = sum_i b_i * F(node_i) where F is the
+// time blend of the two bracketing snapshots evaluated only at the 4 nodes
+// used (the reference blends every node; the per-node arithmetic is the
+// same).  Fallback: nearest of the 4 nodes of leaf slot 0 (slot value -1
+// indexes the last element, as JAX negative indexing does).
+#include "grid_sampler.h"
+
+namespace jt {
+
+struct OctDev {
+    const float* points;      // (Np, 3)
+    const int* conn;          // (M, 4)
+    const float* ebounds;     // (M, 2, 3)
+    const float* nmin;        // (Nn, 3)
+    const float* nmax;
+    const int* nelem;         // (Nn, max_elems)
+    const int* ncount;        // (Nn)
+    const int* nchild;        // (Nn, 8)
+    const unsigned char* nleaf;
+    int max_elems, max_depth;
+    int n_elems;
+    const float* data;        // (T, Np, 3) f32
+    long long snap_stride;    // Np * 3
+    const float* times;
+    int T, interp, extrap;
+};
+
+__device__ __forceinline__ float det3(const float c0[3], const float c1[3], const float c2[3]) {
+    // jnp.linalg.det of the 3x3 matrix with columns c0, c1, c2 (cofactor sum, left to right)
+    float a00 = c0[0], a10 = c0[1], a20 = c0[2];
+    float a01 = c1[0], a11 = c1[1], a21 = c1[2];
+    float a02 = c2[0], a12 = c2[1], a22 = c2[2];
+    float t = (a00 * a11) * a22;
+    t = t + (a01 * a12) * a20;
+    t = t + (a02 * a10) * a21;
+    t = t - (a02 * a11) * a20;
+    t = t - (a00 * a12) * a21;
+    t = t - (a01 * a10) * a22;
+    return t;
+}
+
+__device__ __forceinline__ void blended_node(const OctDev& o, const Blend& bl, float oma, int node, float v[3]) {
+    const float* b0 = o.data + (size_t)bl.i0 * o.snap_stride;
+    const float* b1 = o.data + (size_t)bl.i1 * o.snap_stride;
+    corner<ST_F32x3>(b0, b1, bl, oma, node, v);
+}
+
+// Returns 1 if an element was found (value in out), else 0 (fallback value in out).
+__device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q[3], float out[3]) {
+    const float oma = 1.0f - bl.a;
+    int node = 0;
+    for (int it = 0; it < o.max_depth; ++it) {             // :278-302
+        if (o.nleaf[node]) break;                           // leaves never move again
+        float c[3];
+        for (int a = 0; a < 3; ++a) c[a] = (o.nmin[node * 3 + a] + o.nmax[node * 3 + a]) / 2.0f;
+        int oc = (q[0] >= c[0] ? 1 : 0) + (q[1] >= c[1] ? 2 : 0) + (q[2] >= c[2] ? 4 : 0);
+        int ch = o.nchild[node * 8 + oc];
+        if (ch < 0) break;                                  // stays here for the remaining levels
+        node = ch;
+    }
+    const int* cand = o.nelem + (size_t)node * o.max_elems;
+    const int cnt = o.ncount[node];
+    for (int k = 0; k < cnt; ++k) {                         // :309-360, first hit wins
+        const int e = cand[k];
+        const float* eb = o.ebounds + (size_t)e * 6;
+        if (!(q[0] >= eb[0] && q[1] >= eb[1] && q[2] >= eb[2] &&
+              q[0] <= eb[3] && q[1] <= eb[4] && q[2] <= eb[5])) continue;
+        const int* ni = o.conn + (size_t)e * 4;
+        float v[4][3];
+        for (int j = 0; j < 4; ++j)
+            for (int a = 0; a < 3; ++a) v[j][a] = o.points[(size_t)ni[j] * 3 + a];
+        float v0p[3], v01[3], v02[3], v03[3];
+        for (int a = 0; a < 3; ++a) {
+            v0p[a] = q[a] - v[0][a];
+            v01[a] = v[1][a] - v[0][a];
+            v02[a] = v[2][a] - v[0][a];
+            v03[a] = v[3][a] - v[0][a];
+        }
+        float d = det3(v01, v02, v03);                     // :226-241
+        d = fabsf(d) < 1e-10f ? 1.0f : d;
+        float b1 = det3(v0p, v02, v03) / d;
+        float b2 = det3(v01, v0p, v03) / d;
+        float b3 = det3(v01, v02, v0p) / d;
+        float b0 = ((1.0f - b1) - b2) - b3;
+        const float lo = -1e-6f, hi = 0.999999f;           // :246-248
+        if (b0 >= lo && b1 >= lo && b2 >= lo && b3 >= lo && b0 <= hi && b1 <= hi && b2 <= hi && b3 <= hi) {
+            float F[4][3];
+            for (int j = 0; j < 4; ++j) blended_node(o, bl, oma, ni[j], F[j]);
+            for (int a = 0; a < 3; ++a)
+                out[a] = ((b0 * F[0][a] + b1 * F[1][a]) + b2 * F[2][a]) + b3 * F[3][a];
+            return 1;
+        }
+    }
+    int first = cand[0];                                   // :364-376
+    if (first < 0) first = o.n_elems + first;
+    const int* ni = o.conn + (size_t)first * 4;
+    int best = 0;
+    float bd = 0.0f;
+    for (int j = 0; j < 4; ++j) {
+        float dx = o.points[(size_t)ni[j] * 3] - q[0];
+        float dy = o.points[(size_t)ni[j] * 3 + 1] - q[1];
+        float dz = o.points[(size_t)ni[j] * 3 + 2] - q[2];
+        float dd = (dx * dx + dy * dy) + dz * dz;
+        if (j == 0 || dd < bd) { bd = dd; best = j; }
+    }
+    blended_node(o, bl, oma, ni[best], out);
+    return 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_oct_sample(OctDev o, const float* __restrict__ x, int n, float t, float* __restrict__ out,
+             int* __restrict__ found) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Blend b = time_blend(o.times, o.T, t, o.interp, o.extrap);
+    float q[3] = {x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2]};
+    float v[3];
+    int f = oct_sample(o, b, q, v);
+    out[3 * (size_t)i] = v[0];
+    out[3 * (size_t)i + 1] = v[1];
+    out[3 * (size_t)i + 2] = v[2];
+    if (found) found[i] = f;
+}
+
+template <int INTEG>
+__global__ void __launch_bounds__(256)
+k_oct_track(OctDev o, BcDev bc, float* __restrict__ xs, int n, const Blend* __restrict__ tab, int s0, int s1,
+            float dt, float dth, float dt6, int rec_every, float* __restrict__ traj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
+    for (int s = s0; s < s1; ++s) {
+        const Blend b0 = tab[3 * s];
+        if constexpr (INTEG == JT_RK4) {
+            const Blend b1 = tab[3 * s + 1], b2 = tab[3 * s + 2];
+            float k1[3], k2[3], k3[3], k4[3], q[3];
+            oct_sample(o, b0, p, k1);
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k1[c];
+            oct_sample(o, b1, q, k2);
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k2[c];
+            oct_sample(o, b1, q, k3);
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dt * k3[c];
+            oct_sample(o, b2, q, k4);
+            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt6 * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
+        } else if constexpr (INTEG == JT_RK2) {
+            const Blend b1 = tab[3 * s + 1];
+            float v1[3], v2[3], q[3];
+            oct_sample(o, b0, p, v1);
+            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * v1[c];
+            oct_sample(o, b1, q, v2);
+            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v2[c];
+        } else {
+            float v[3];
+            oct_sample(o, b0, p, v);
+            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
+        }
+        apply_bc(bc, p);
+        if (traj != nullptr && (s + 1) % rec_every == 0) {
+            float* out = traj + ((size_t)((s + 1) / rec_every) * (size_t)n + i) * 3;
+            out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
+        }
+    }
+    xs[3 * (size_t)i] = p[0];
+    xs[3 * (size_t)i + 1] = p[1];
+    xs[3 * (size_t)i + 2] = p[2];
+}
+
+static int make_oct(const jt_octree_field* f, OctDev& o) {
+    JT_REQUIRE(f != nullptr, "octree field descriptor is NULL");
+    JT_REQUIRE(f->points && f->conn && f->element_bounds && f->nodes_min && f->nodes_max && f->nodes_elements &&
+               f->nodes_elem_counts && f->nodes_children && f->nodes_is_leaf && f->data && f->times,
+               "NULL pointer in octree field");
+    JT_REQUIRE(f->n_snap >= 1 && f->n_points >= 1 && f->n_elems >= 1 && f->max_elems >= 1 && f->max_depth >= 0,
+               "bad octree field sizes");
+    JT_REQUIRE(f->time_interp == JT_TIME_LINEAR || f->time_interp == JT_TIME_NEAREST, "bad time_interp");
+    JT_REQUIRE(f->time_extrap >= JT_EXTRAP_CONSTANT && f->time_extrap <= JT_EXTRAP_ZERO, "bad time_extrap");
+    o.points = f->points; o.conn = f->conn; o.ebounds = f->element_bounds;
+    o.nmin = f->nodes_min; o.nmax = f->nodes_max; o.nelem = f->nodes_elements; o.ncount = f->nodes_elem_counts;
+    o.nchild = f->nodes_children; o.nleaf = f->nodes_is_leaf;
+    o.max_elems = f->max_elems; o.max_depth = f->max_depth; o.n_elems = (int)f->n_elems;
+    o.data = (const float*)f->data; o.snap_stride = f->n_points * 3;
+    o.times = f->times; o.T = f->n_snap; o.interp = f->time_interp; o.extrap = f->time_extrap;
+    return JT_OK;
+}
+
+}  // namespace jt
+
+using namespace jt;
+
+extern "C" int jt_octree_sample(const jt_octree_field* field, const float* x, int64_t n, float t, float* out,
+                                int32_t* found, void* stream) {
+    OctDev o;
+    int rc = make_oct(field, o);
+    if (rc) return rc;
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(x && out, "NULL x/out");
+    hipLaunchKernelGGL(k_oct_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, o, x,
+                       (int)n, t, out, found);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+extern "C" size_t jt_track_octree_workspace_size(int64_t n, int32_t n_steps) {
+    (void)n;
+    Arena a(nullptr, 0);
+    a.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
+    return a.used + 256;
+}
+
+extern "C" int jt_track_octree(const jt_octree_field* field, const jt_boundary* bc, int32_t integrator, float* x,
+                               int64_t n, const float* step_times, float dt, int32_t n_steps, int32_t record_every,
+                               float* traj, const jt_track_opts* opts, void* ws, size_t ws_bytes, void* stream) {
+    OctDev o;
+    int rc = make_oct(field, o);
+    if (rc) return rc;
+    BcDev d;
+    rc = make_bc_dev(bc, d);
+    if (rc) return rc;
+    JT_REQUIRE(integrator >= JT_EULER && integrator <= JT_RK4, "bad integrator %d", integrator);
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    JT_REQUIRE(n_steps >= 0 && record_every >= 1, "bad n_steps / record_every");
+    JT_REQUIRE(n == 0 || x != nullptr, "NULL x");
+    JT_REQUIRE(n_steps == 0 || step_times != nullptr, "NULL step_times");
+    hipStream_t s = (hipStream_t)stream;
+    if (traj != nullptr && n > 0)
+        JT_HIP(hipMemcpyAsync(traj, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (n == 0 || n_steps == 0) return JT_OK;
+    Arena arena(ws, ws_bytes);
+    Blend* tab = arena.take<Blend>((size_t)3 * n_steps);
+    if (!tab || !arena.ok()) return fail(JT_ENOMEM, "workspace too small");
+    GridDev g{};
+    g.times = o.times; g.T = o.T; g.interp = o.interp; g.extrap = o.extrap;
+    hipLaunchKernelGGL(k_blend_table, dim3((n_steps + 127) / 128), dim3(128), 0, s, g, step_times, n_steps, dt,
+                       integrator, tab);
+    JT_HIP(hipGetLastError());
+    int spl = (opts && opts->steps_per_launch > 0) ? opts->steps_per_launch : (n >= (1 << 20) ? 10 : 500);
+    const float dth = 0.5f * dt, dt6 = dt / 6.0f;
+    dim3 block(256), grid((unsigned)((n + 255) / 256));
+    for (int s0 = 0; s0 < n_steps; s0 += spl) {
+        int s1 = s0 + spl < n_steps ? s0 + spl : n_steps;
+        if (integrator == JT_RK4)
+            hipLaunchKernelGGL(k_oct_track<JT_RK4>, grid, block, 0, s, o, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+        else if (integrator == JT_RK2)
+            hipLaunchKernelGGL(k_oct_track<JT_RK2>, grid, block, 0, s, o, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+        else
+            hipLaunchKernelGGL(k_oct_track<JT_EULER>, grid, block, 0, s, o, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+        JT_HIP(hipGetLastError());
+    }
+    return JT_OK;
+}

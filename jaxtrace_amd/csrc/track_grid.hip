@@ -4,7 +4,16 @@
 // stages and all steps of a launch: the reference's lax.scan over
 // rk4_step -> field_fn -> boundary_fn (jaxtrace/tracking/tracker.py:228-246)
 // materialises k1..k4 and the blended node arrays in memory every stage; here
-// each stage is 8 corner gathers from each of the two bracketing snapshots.
+// each stage is 8 corner gathers from each of the two bracketing snapshots
+// (4 paired loads with the z-halo layout).
+//
+// Locality: optionally (sort_every > 0) particles are re-ordered every k steps
+// by the Morton code of their grid cell, so the 64 lanes of a wavefront gather
+// from one small block of cells; trajectories are written back in the
+// caller's particle order through the permutation.  Blocks are mapped to
+// particles XCD-contiguously: the hardware deals workgroups round-robin over
+// the 8 XCDs, so block b takes chunk (b % 8) * (grid / 8) + b / 8 and each
+// XCD's L2 sees one contiguous (spatially compact, once sorted) particle range.
 #include "grid_sampler.h"
 #include <hipcub/hipcub.hpp>
 
@@ -22,7 +31,8 @@ int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
                "bad time_interp %d", f->time_interp);
     JT_REQUIRE(f->time_extrap >= JT_EXTRAP_CONSTANT && f->time_extrap <= JT_EXTRAP_ZERO,
                "bad time_extrap %d", f->time_extrap);
-    long long nodes = (long long)f->nx * f->ny * f->nz;
+    JT_REQUIRE(f->z_halo == 0 || f->z_halo == 1, "z_halo must be 0 or 1");
+    long long nodes = (long long)f->nx * f->ny * (f->nz + f->z_halo);
     JT_REQUIRE(nodes * f->ncomp < (1LL << 31), "grid too large for 32-bit node offsets");
     if (f->dtype == JT_DTYPE_F32) st = (f->ncomp == 3) ? ST_F32x3 : ST_F32x4;
     else if (f->dtype == JT_DTYPE_BF16) st = ST_BF16x4;
@@ -35,6 +45,7 @@ int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
     g.times = f->times;
     g.T = f->n_snap;
     g.nx = f->nx; g.ny = f->ny; g.nz = f->nz;
+    g.nzs = f->nz + f->z_halo;
     const int n[3] = {f->nx, f->ny, f->nz};
     for (int a = 0; a < 3; ++a) {
         g.o[a] = f->origin[a];
@@ -85,33 +96,15 @@ int make_bc_dev(const jt_boundary* b, BcDev& d) {
 
 // ---- kernels -------------------------------------------------------------
 
-// Stage-time blend table: 3 entries per step (stage origin, midpoint, end).
-// rk4.py:88-92 (f32 t + 0.5*dt, t + dt); rk2.py:165 (float64 midpoint sum).
-__global__ void k_blend_table(GridDev g, const float* __restrict__ step_times, int n_steps,
-                              float dt, int integrator, Blend* __restrict__ tab) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_steps) return;
-    const float t = step_times[s];
-    float t1 = t, t2 = t;
-    if (integrator == JT_RK4) {
-        const float dth = 0.5f * dt;
-        t1 = t + dth;
-        t2 = t + dt;
-    } else if (integrator == JT_RK2) {
-        const float dth = dt * 0.5f;
-        t1 = (float)((double)t + (double)dth);
-    }
-    tab[3 * s + 0] = time_blend(g.times, g.T, t, g.interp, g.extrap);
-    tab[3 * s + 1] = time_blend(g.times, g.T, t1, g.interp, g.extrap);
-    tab[3 * s + 2] = time_blend(g.times, g.T, t2, g.interp, g.extrap);
-}
 
-template <int INTEG, int ST>
+template <int INTEG, int ST, bool ZH>
 __global__ void __launch_bounds__(256)
 k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
         const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
         int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = gridDim.x >> 3;                     // gridDim.x is a multiple of 8
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int i = lb * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
     const size_t dst = ids ? (size_t)ids[i] : (size_t)i;
@@ -121,31 +114,31 @@ k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
             const Blend b1 = tab[3 * s + 1];
             const Blend b2 = tab[3 * s + 2];
             float k1[3], k2[3], k3[3], k4[3], q[3];
-            sample<ST>(g, b0, p[0], p[1], p[2], k1);
+            sample<ST, ZH>(g, b0, p[0], p[1], p[2], k1);
 #pragma unroll
             for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k1[c];
-            sample<ST>(g, b1, q[0], q[1], q[2], k2);
+            sample<ST, ZH>(g, b1, q[0], q[1], q[2], k2);
 #pragma unroll
             for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k2[c];
-            sample<ST>(g, b1, q[0], q[1], q[2], k3);
+            sample<ST, ZH>(g, b1, q[0], q[1], q[2], k3);
 #pragma unroll
             for (int c = 0; c < 3; ++c) q[c] = p[c] + dt * k3[c];
-            sample<ST>(g, b2, q[0], q[1], q[2], k4);
+            sample<ST, ZH>(g, b2, q[0], q[1], q[2], k4);
 #pragma unroll
             for (int c = 0; c < 3; ++c)                 // rk4.py:44
                 p[c] = p[c] + dt6 * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
         } else if constexpr (INTEG == JT_RK2) {        // rk2.py:157-177
             const Blend b1 = tab[3 * s + 1];
             float v1[3], v2[3], q[3];
-            sample<ST>(g, b0, p[0], p[1], p[2], v1);
+            sample<ST, ZH>(g, b0, p[0], p[1], p[2], v1);
 #pragma unroll
             for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * v1[c];
-            sample<ST>(g, b1, q[0], q[1], q[2], v2);
+            sample<ST, ZH>(g, b1, q[0], q[1], q[2], v2);
 #pragma unroll
             for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v2[c];
         } else {                                        // euler.py:150-157
             float v[3];
-            sample<ST>(g, b0, p[0], p[1], p[2], v);
+            sample<ST, ZH>(g, b0, p[0], p[1], p[2], v);
 #pragma unroll
             for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
         }
@@ -161,14 +154,14 @@ k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
     xs[3 * (size_t)i + 2] = p[2];
 }
 
-template <int ST>
+template <int ST, bool ZH>
 __global__ void __launch_bounds__(256)
 k_sample(GridDev g, const float* __restrict__ x, int n, float t, float* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Blend b = time_blend(g.times, g.T, t, g.interp, g.extrap);
     float v[3];
-    sample<ST>(g, b, x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], v);
+    sample<ST, ZH>(g, b, x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], v);
     out[3 * (size_t)i] = v[0];
     out[3 * (size_t)i + 1] = v[1];
     out[3 * (size_t)i + 2] = v[2];
@@ -184,16 +177,84 @@ __global__ void __launch_bounds__(256) k_boundary(BcDev bc, float* __restrict__ 
     x[3 * (size_t)i + 2] = p[2];
 }
 
-template <int INTEG>
+// ---- particle re-ordering by grid cell ------------------------------------
+__device__ __forceinline__ unsigned spread3(unsigned v) {     // 10 bits -> every third bit
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ unsigned cell_bits(float x, float o, float d, int n, int shift) {
+    float f = (x - o) / d;
+    int c = (f != f) ? 0 : (int)fminf(fmaxf(floorf(f), 0.0f), (float)(n - 1));
+    return (unsigned)c >> shift;
+}
+
+__global__ void __launch_bounds__(256)
+k_morton(GridDev g, const float* __restrict__ xs, int n, int shift, unsigned* __restrict__ keys,
+         int* __restrict__ perm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned cx = cell_bits(xs[3 * (size_t)i], g.o[0], g.d[0], g.nx, shift);
+    unsigned cy = cell_bits(xs[3 * (size_t)i + 1], g.o[1], g.d[1], g.ny, shift);
+    unsigned cz = cell_bits(xs[3 * (size_t)i + 2], g.o[2], g.d[2], g.nz, shift);
+    keys[i] = (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
+    perm[i] = i;
+}
+
+__global__ void __launch_bounds__(256)
+k_permute(const float* __restrict__ xin, const int* __restrict__ idin, const int* __restrict__ perm, int n,
+          float* __restrict__ xout, int* __restrict__ idout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = perm[i];
+    xout[3 * (size_t)i] = xin[3 * (size_t)j];
+    xout[3 * (size_t)i + 1] = xin[3 * (size_t)j + 1];
+    xout[3 * (size_t)i + 2] = xin[3 * (size_t)j + 2];
+    idout[i] = idin ? idin[j] : j;
+}
+
+__global__ void __launch_bounds__(256)
+k_unpermute(const float* __restrict__ xs, const int* __restrict__ ids, int n, float* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t j = (size_t)ids[i];
+    x[3 * j] = xs[3 * (size_t)i];
+    x[3 * j + 1] = xs[3 * (size_t)i + 1];
+    x[3 * j + 2] = xs[3 * (size_t)i + 2];
+}
+
+template <int INTEG, bool ZH>
 static void launch_track_st(int st, dim3 grid, dim3 block, hipStream_t s, const GridDev& g,
                             const BcDev& bc, float* xs, int n, const Blend* tab, int s0, int s1,
                             float dt, float dth, float dt6, int rec, float* traj, const int* ids) {
     switch (st) {
-    case ST_F32x3: hipLaunchKernelGGL((k_track<INTEG, ST_F32x3>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
-    case ST_F32x4: hipLaunchKernelGGL((k_track<INTEG, ST_F32x4>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
-    case ST_BF16x4: hipLaunchKernelGGL((k_track<INTEG, ST_BF16x4>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
-    default: hipLaunchKernelGGL((k_track<INTEG, ST_F16x4>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    case ST_F32x3: hipLaunchKernelGGL((k_track<INTEG, ST_F32x3, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    case ST_F32x4: hipLaunchKernelGGL((k_track<INTEG, ST_F32x4, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    case ST_BF16x4: hipLaunchKernelGGL((k_track<INTEG, ST_BF16x4, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    default: hipLaunchKernelGGL((k_track<INTEG, ST_F16x4, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
     }
+}
+
+template <bool ZH>
+static void launch_sample(int st, dim3 grid, dim3 block, hipStream_t s, const GridDev& g, const float* x, int n,
+                          float t, float* out) {
+    switch (st) {
+    case ST_F32x3: hipLaunchKernelGGL((k_sample<ST_F32x3, ZH>), grid, block, 0, s, g, x, n, t, out); break;
+    case ST_F32x4: hipLaunchKernelGGL((k_sample<ST_F32x4, ZH>), grid, block, 0, s, g, x, n, t, out); break;
+    case ST_BF16x4: hipLaunchKernelGGL((k_sample<ST_BF16x4, ZH>), grid, block, 0, s, g, x, n, t, out); break;
+    default: hipLaunchKernelGGL((k_sample<ST_F16x4, ZH>), grid, block, 0, s, g, x, n, t, out); break;
+    }
+}
+
+static size_t sort_temp_bytes(int n) {
+    size_t temp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                       (const int*)nullptr, (int*)nullptr, n, 0, 30);
+    return temp;
 }
 
 }  // namespace jt
@@ -211,12 +272,8 @@ extern "C" int jt_grid_sample(const jt_grid_field* field, const float* x, int64_
     JT_REQUIRE(x != nullptr && out != nullptr, "NULL x/out");
     hipStream_t s = (hipStream_t)stream;
     dim3 block(256), grid((unsigned)((n + 255) / 256));
-    switch (st) {
-    case ST_F32x3: hipLaunchKernelGGL(k_sample<ST_F32x3>, grid, block, 0, s, g, x, (int)n, t, out); break;
-    case ST_F32x4: hipLaunchKernelGGL(k_sample<ST_F32x4>, grid, block, 0, s, g, x, (int)n, t, out); break;
-    case ST_BF16x4: hipLaunchKernelGGL(k_sample<ST_BF16x4>, grid, block, 0, s, g, x, (int)n, t, out); break;
-    default: hipLaunchKernelGGL(k_sample<ST_F16x4>, grid, block, 0, s, g, x, (int)n, t, out); break;
-    }
+    if (g.nzs != g.nz) launch_sample<true>(st, grid, block, s, g, x, (int)n, t, out);
+    else launch_sample<false>(st, grid, block, s, g, x, (int)n, t, out);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }
@@ -242,9 +299,15 @@ static int default_steps_per_launch(int64_t n) {
 }
 
 extern "C" size_t jt_track_grid_workspace_size(int64_t n, int32_t n_steps, const jt_track_opts* opts) {
-    (void)n; (void)opts;
     Arena a(nullptr, 0);
     a.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
+    if (opts && opts->sort_every > 0 && n > 0) {
+        a.take<unsigned>(n); a.take<unsigned>(n);      // keys, sorted keys
+        a.take<int>(n); a.take<int>(n);                // permutation in / out
+        a.take<int>(n); a.take<int>(n);                // particle ids (double buffer)
+        a.take<float>((size_t)3 * n); a.take<float>((size_t)3 * n);   // sorted state (double buffer)
+        a.take<char>(sort_temp_bytes((int)n));
+    }
     return a.used + 256;
 }
 
@@ -265,16 +328,31 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     JT_REQUIRE(record_every >= 1, "record_every must be >= 1");
     JT_REQUIRE(n == 0 || x != nullptr, "NULL x");
     JT_REQUIRE(n_steps == 0 || step_times != nullptr, "NULL step_times");
+    JT_REQUIRE(opts == nullptr || (opts->sort_every >= 0 && opts->steps_per_launch >= 0), "bad opts");
     hipStream_t s = (hipStream_t)stream;
     if (traj != nullptr && n > 0)
         JT_HIP(hipMemcpyAsync(traj, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
     if (n == 0 || n_steps == 0) return JT_OK;
 
+    const int sort_every = opts ? opts->sort_every : 0;
+    const size_t need = jt_track_grid_workspace_size(n, n_steps, opts);
     Arena arena(ws, ws_bytes);
     Blend* tab = arena.take<Blend>((size_t)3 * n_steps);
-    if (tab == nullptr || !arena.ok())
-        return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu",
-                    jt_track_grid_workspace_size(n, n_steps, opts), ws_bytes);
+    unsigned *keys = nullptr, *skeys = nullptr;
+    int *perm = nullptr, *sperm = nullptr, *ids = nullptr, *ids2 = nullptr;
+    float *xa = nullptr, *xb = nullptr;
+    char* tmp = nullptr;
+    size_t temp = 0;
+    if (sort_every > 0) {
+        keys = arena.take<unsigned>(n); skeys = arena.take<unsigned>(n);
+        perm = arena.take<int>(n); sperm = arena.take<int>(n);
+        ids = arena.take<int>(n); ids2 = arena.take<int>(n);
+        xa = arena.take<float>((size_t)3 * n); xb = arena.take<float>((size_t)3 * n);
+        temp = sort_temp_bytes((int)n);
+        tmp = arena.take<char>(temp);
+    }
+    if (tab == nullptr || !arena.ok() || (sort_every > 0 && tmp == nullptr))
+        return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu", need, ws_bytes);
 
     hipLaunchKernelGGL(k_blend_table, dim3((n_steps + 127) / 128), dim3(128), 0, s, g, step_times,
                        n_steps, dt, integrator, tab);
@@ -283,14 +361,51 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     int spl = (opts && opts->steps_per_launch > 0) ? opts->steps_per_launch : default_steps_per_launch(n);
     const float dth = 0.5f * dt;       // rk4.py:90 (rk2.py:169 is dt*0.5, the same value)
     const float dt6 = dt / 6.0f;       // rk4.py:44
-    dim3 block(256), grid((unsigned)((n + 255) / 256));
-    for (int s0 = 0; s0 < n_steps; s0 += spl) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    dim3 block(256), grid((nb + 7) / 8 * 8);
+    dim3 eb(256), eg(nb);
+    int maxn = g.nx > g.ny ? g.nx : g.ny;
+    maxn = maxn > g.nz ? maxn : g.nz;
+    int shift = 0;
+    while ((maxn - 1) >> shift >= 1024) ++shift;       // 10 bits per axis
+    float* cur = x;
+    const int* cur_ids = nullptr;
+    for (int s0 = 0; s0 < n_steps;) {
         int s1 = s0 + spl < n_steps ? s0 + spl : n_steps;
-        switch (integrator) {
-        case JT_RK4: launch_track_st<JT_RK4>(st, grid, block, s, g, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, nullptr); break;
-        case JT_RK2: launch_track_st<JT_RK2>(st, grid, block, s, g, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, nullptr); break;
-        default: launch_track_st<JT_EULER>(st, grid, block, s, g, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, nullptr); break;
+        if (sort_every > 0) {
+            int next_sort = (s0 / sort_every + 1) * sort_every;
+            if (s1 > next_sort) s1 = next_sort;
+            if (s0 % sort_every == 0) {
+                hipLaunchKernelGGL(k_morton, eg, eb, 0, s, g, cur, (int)n, shift, keys, perm);
+                JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0, 30, s));
+                float* nx = (cur == xa) ? xb : xa;
+                int* nid = (cur_ids == ids) ? ids2 : ids;
+                hipLaunchKernelGGL(k_permute, eg, eb, 0, s, cur, cur_ids, sperm, (int)n, nx, nid);
+                JT_HIP(hipGetLastError());
+                cur = nx;
+                cur_ids = nid;
+            }
         }
+        const bool zh = g.nzs != g.nz;
+        switch (integrator) {
+        case JT_RK4:
+            if (zh) launch_track_st<JT_RK4, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            else launch_track_st<JT_RK4, false>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            break;
+        case JT_RK2:
+            if (zh) launch_track_st<JT_RK2, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            else launch_track_st<JT_RK2, false>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            break;
+        default:
+            if (zh) launch_track_st<JT_EULER, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            else launch_track_st<JT_EULER, false>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            break;
+        }
+        JT_HIP(hipGetLastError());
+        s0 = s1;
+    }
+    if (cur != x) {
+        hipLaunchKernelGGL(k_unpermute, eg, eb, 0, s, cur, cur_ids, (int)n, x);
         JT_HIP(hipGetLastError());
     }
     return JT_OK;

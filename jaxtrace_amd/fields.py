@@ -68,8 +68,8 @@ class _GridSeries:
             raise ValueError(f"Unsupported extrapolation: {extrapolation}")
         if storage not in _STORAGE:
             raise ValueError(f"storage must be one of {sorted(_STORAGE)}, got {storage!r}")
-        if layout not in ("packed", "padded"):
-            raise ValueError("layout must be 'packed' or 'padded'")
+        if layout not in ("packed", "padded", "halo", "halo4"):
+            raise ValueError("layout must be 'packed', 'padded', 'halo' or 'halo4'")
         gx = np.asarray(grid_x.cpu() if isinstance(grid_x, torch.Tensor) else grid_x, dtype=F32)
         gy = np.asarray(grid_y.cpu() if isinstance(grid_y, torch.Tensor) else grid_y, dtype=F32)
         gz = np.asarray(grid_z.cpu() if isinstance(grid_z, torch.Tensor) else grid_z, dtype=F32)
@@ -94,12 +94,22 @@ class _GridSeries:
         self.bounds_handling, self.interpolation, self.extrapolation = mode, interp, extra
         self.storage, self.layout = storage, layout
         dt_code, tdt = _STORAGE[storage]
-        ncomp = 3 if (storage == "float32" and layout == "packed") else 4
-        if ncomp == 3:
+        # packed: (T,Nx,Ny,Nz,3) f32 as given; padded: 4 components per node;
+        # halo/halo4: one extra z-plane (plane Nz repeats plane 0 for periodic
+        # bounds, plane Nz-1 otherwise) so both z-corners of a cell are one load
+        ncomp = 3 if (storage == "float32" and layout in ("packed", "halo")) else 4
+        z_halo = 1 if layout in ("halo", "halo4") else 0
+        if ncomp == 3 and not z_halo:
             snaps = data.to(device=self.device, dtype=torch.float32).contiguous()
         else:
-            snaps = torch.zeros((T, Nx, Ny, Nz, 4), dtype=tdt, device=self.device)
-            snaps[..., :3] = data.to(device=self.device, dtype=torch.float32).to(tdt)
+            snaps = torch.zeros((T, Nx, Ny, Nz + z_halo, ncomp), dtype=tdt, device=self.device)
+            src = data.to(device=self.device, dtype=torch.float32)
+            snaps[:, :, :, :Nz, :3] = src.to(tdt)
+            if z_halo:
+                edge = 0 if mode == "periodic" else Nz - 1
+                snaps[:, :, :, Nz, :3] = src[:, :, :, edge, :].to(tdt)
+            del src
+        self.z_halo = z_halo
         self._snaps = snaps
         self._times_dev = torch.from_numpy(tt.copy()).to(self.device)
         self.times = tt
@@ -109,6 +119,7 @@ class _GridSeries:
         d.times = self._times_dev.data_ptr()
         d.n_snap, d.nx, d.ny, d.nz = T, Nx, Ny, Nz
         d.dtype, d.ncomp = dt_code, ncomp
+        d.z_halo = z_halo
         d.origin[:] = [F32(gx[0]), F32(gy[0]), F32(gz[0])]
         d.spacing[:] = [self.dx, self.dy, self.dz]
         d.bounds_mode = L.JT_BOUNDS[mode]
@@ -202,7 +213,109 @@ class StructuredTimeSeriesField(_GridSeries):
 
     def with_storage(self, storage: str, layout: str = "packed") -> "StructuredTimeSeriesField":
         """Same field, snapshots re-stored as ``storage`` (RNE from float32)."""
-        src = self._snaps[..., :3].to(torch.float32)
+        src = self._snaps[:, :, :, :self.Nz, :3].to(torch.float32)
         return StructuredTimeSeriesField(src, self.times, self.grid_x, self.grid_y, self.grid_z,
                                          self.bounds_handling, self.interpolation,
                                          self.extrapolation, storage, layout, self.device)
+
+
+class OctreeFEMTimeSeriesFieldOptimized:
+    """octree_fem_time_series_optimized.py:17-127: nodal snapshots on a
+    tetrahedral mesh, located through the element octree of
+    build_octree_mesh_optimized (built on the host by the library, arrays
+    identical to the reference's) and interpolated with P1 shape functions."""
+
+    kind = "octree"
+
+    def __init__(self, data, times, positions, connectivity, interpolation: str = "linear",
+                 extrapolation: str = "constant", _source_info: Optional[dict] = None,
+                 max_elements_per_leaf: int = 32, max_depth: int = 12, device=None):
+        lib = L.load()
+        self.device = torch.device(device) if device is not None else L.device()
+        data = np.ascontiguousarray(np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data, F32))
+        times = np.asarray(times.cpu() if isinstance(times, torch.Tensor) else times, F32).reshape(-1)
+        pts = np.ascontiguousarray(np.asarray(positions.cpu() if isinstance(positions, torch.Tensor)
+                                              else positions, F32))
+        conn = np.ascontiguousarray(np.asarray(connectivity.cpu() if isinstance(connectivity, torch.Tensor)
+                                               else connectivity, np.int32))
+        if data.ndim != 3 or data.shape[2] != 3:                               # time_series.py:120-128
+            raise ValueError(f"data must have 3 dims (T,N,3), got {data.shape}")
+        T, N, _ = data.shape
+        if times.shape != (T,):
+            raise ValueError(f"times shape {times.shape} != (T,) with T={T}")
+        if pts.shape != (N, 3):
+            raise ValueError(f"positions shape {pts.shape} != (N,3) with N={N}")
+        if conn.ndim != 2 or conn.shape[1] != 4:
+            raise ValueError("connectivity must be (M,4)")
+        interp, extra = interpolation.lower(), extrapolation.lower()
+        if interp not in ("nearest", "linear", "cubic"):
+            raise ValueError(f"Unsupported interpolation: {interpolation}")
+        if extra not in L.JT_EXTRAP:
+            raise ValueError(f"Unsupported extrapolation: {extrapolation}")
+        self.interpolation, self.extrapolation = interp, extra
+        self.data, self.times, self.positions, self.connectivity = data, times, pts, conn
+        self.T, self.N = T, N
+        self.t_min, self.t_max = float(times[0]), float(times[-1])
+        self._source_info = _source_info or {}
+        self.storage = "float32"
+        h = C.c_void_p()
+        L.check(lib.jt_octree_build(pts.ctypes.data, N, conn.ctypes.data, conn.shape[0],
+                                    int(max_elements_per_leaf), int(max_depth), C.byref(h)), "jt_octree_build")
+        try:
+            nn, me, dep = C.c_int64(), C.c_int32(), C.c_int32()
+            L.check(lib.jt_octree_info(h, C.byref(nn), C.byref(me), C.byref(dep)), "jt_octree_info")
+            n_nodes, max_elems = int(nn.value), int(me.value)
+            M = conn.shape[0]
+            tree = {
+                "nodes_min": np.empty((n_nodes, 3), F32), "nodes_max": np.empty((n_nodes, 3), F32),
+                "nodes_elements": np.empty((n_nodes, max_elems), np.int32),
+                "nodes_elem_counts": np.empty(n_nodes, np.int32),
+                "nodes_children": np.empty((n_nodes, 8), np.int32),
+                "nodes_is_leaf": np.empty(n_nodes, np.uint8),
+                "element_bounds": np.empty((M, 2, 3), F32), "element_centroids": np.empty((M, 3), F32),
+            }
+            L.check(lib.jt_octree_export(h, *[tree[k].ctypes.data for k in (
+                "nodes_min", "nodes_max", "nodes_elements", "nodes_elem_counts", "nodes_children",
+                "nodes_is_leaf", "element_bounds", "element_centroids")]), "jt_octree_export")
+        finally:
+            lib.jt_octree_free(h)
+        self.octree = tree
+        self.tree_depth = int(dep.value)
+        self.max_depth = int(max_depth)
+        dev = self.device
+        self._dev = {k: torch.from_numpy(v).to(dev) for k, v in tree.items() if k != "element_centroids"}
+        self._dev["points"] = torch.from_numpy(pts).to(dev)
+        self._dev["conn"] = torch.from_numpy(conn).to(dev)
+        self._dev["data"] = torch.from_numpy(data).to(dev)
+        self._dev["times"] = torch.from_numpy(times.copy()).to(dev)
+        d = L.OctreeField()
+        for k in ("points", "conn", "element_bounds", "nodes_min", "nodes_max", "nodes_elements",
+                  "nodes_elem_counts", "nodes_children", "nodes_is_leaf", "data", "times"):
+            setattr(d, k, self._dev[k].data_ptr())
+        d.n_points, d.n_elems, d.n_nodes = N, M, n_nodes
+        d.max_elems, d.max_depth = max_elems, int(max_depth)
+        d.n_snap = T
+        d.time_interp = L.JT_TIME.get(interp, 0)
+        d.time_extrap = L.JT_EXTRAP[extra]
+        self._desc = d
+
+    def descriptor(self):
+        if self.interpolation == "cubic":
+            raise NotImplementedError("cubic interpolation not supported with JAX")
+        return self._desc
+
+    def sample_at_positions(self, query_positions, t: float, return_found: bool = False):
+        lib = L.load()
+        x, is_torch = _ensure_positions(query_positions, self.device)
+        out = torch.empty_like(x)
+        found = torch.empty(x.shape[0], dtype=torch.int32, device=self.device)
+        L.check(lib.jt_octree_sample(C.byref(self.descriptor()), L.ptr(x), x.shape[0], C.c_float(F32(t)),
+                                     L.ptr(out), L.ptr(found), L.stream_handle()), "jt_octree_sample")
+        if not is_torch:
+            out, found = out.cpu().numpy(), found.cpu().numpy()
+        return (out, found) if return_found else out
+
+    _sample = sample_at_positions
+
+    def get_time_bounds(self):
+        return (self.t_min, self.t_max)

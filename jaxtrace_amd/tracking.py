@@ -21,7 +21,7 @@ import torch
 
 from . import _lib as L
 from .boundary import BoundaryCondition
-from .fields import _GridSeries, _ensure_positions
+from .fields import OctreeFEMTimeSeriesFieldOptimized, _GridSeries, _ensure_positions
 
 F32 = np.float32
 _INTEGRATORS = {"euler": L.JT_EULER, "rk2": L.JT_RK2, "rk4": L.JT_RK4}
@@ -157,15 +157,20 @@ class ParticleTracker:
         desc = fld.descriptor()
         bc = self.boundary.descriptor(dev)
         o = self._opts()
-        ws_bytes = int(lib.jt_track_grid_workspace_size(n, n_steps, C.byref(o)))
+        octree = getattr(fld, "kind", "grid") == "octree"
+        if octree:
+            ws_bytes = int(lib.jt_track_octree_workspace_size(n, n_steps))
+        else:
+            ws_bytes = int(lib.jt_track_grid_workspace_size(n, n_steps, C.byref(o)))
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         traj = torch.empty((n_rec, n, 3), dtype=torch.float32, device=dev)
         state = x.clone()
         t0 = time.perf_counter()
-        L.check(lib.jt_track_grid(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
-                                  L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
-                                  rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
-                                  L.stream_handle()), "jt_track_grid")
+        entry = lib.jt_track_octree if octree else lib.jt_track_grid
+        L.check(entry(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
+                      L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
+                      rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
+                      L.stream_handle()), "jt_track_octree" if octree else "jt_track_grid")
         velocities = None
         if opt.record_velocities:
             velocities = torch.empty_like(traj)
@@ -188,10 +193,10 @@ def create_tracker(integrator_name: str, field, boundary_condition, **options) -
     """tracker.py:810-855."""
     if integrator_name not in _INTEGRATORS:
         raise ValueError(f"Unknown integrator: {integrator_name}. Available: {list(_INTEGRATORS)}")
-    if not isinstance(field, _GridSeries):
+    if not isinstance(field, (_GridSeries, OctreeFEMTimeSeriesFieldOptimized)):
         raise TypeError("field must be a device-backed jaxtrace_amd field "
-                        "(StructuredGridSampler, StructuredTimeSeriesField); "
-                        f"got {type(field).__name__}")
+                        "(StructuredGridSampler, StructuredTimeSeriesField, "
+                        f"OctreeFEMTimeSeriesFieldOptimized); got {type(field).__name__}")
     if not isinstance(boundary_condition, BoundaryCondition):
         if callable(boundary_condition):
             raise TypeError("boundary_condition must come from a jaxtrace_amd boundary factory "

@@ -86,3 +86,69 @@ def random_fourier(n: int = 256, n_snap: int = 16, t_end: float = 2.0, n_modes: 
 
 def uniform_particles(n: int, seed: int) -> np.ndarray:
     return np.random.default_rng(seed).random((n, 3)).astype(F32)
+
+
+def gaussian_mixture(n: int, n_comp: int = 64, seed: int = 6) -> np.ndarray:
+    """n points from an isotropic Gaussian mixture in the unit cube: Dirichlet(1)
+    weights, means U[0.1,0.9]^3, sigma U[0.01,0.05]; points outside [0,1]^3
+    are redrawn from their component."""
+    rng = np.random.default_rng(seed)
+    w = rng.dirichlet(np.ones(n_comp))
+    mu = rng.uniform(0.1, 0.9, (n_comp, 3))
+    sig = rng.uniform(0.01, 0.05, n_comp)
+    comp = rng.choice(n_comp, size=n, p=w)
+    out = np.empty((n, 3), np.float64)
+    todo = np.arange(n)
+    while todo.size:
+        c = comp[todo]
+        x = mu[c] + sig[c, None] * rng.standard_normal((todo.size, 3))
+        ok = ((x >= 0) & (x <= 1)).all(1)
+        out[todo[ok]] = x[ok]
+        todo = todo[~ok]
+    return out.astype(F32)
+
+
+def kde_grid_axes(resolution: int = 256):
+    """f32 linspace axes of KDEEstimator._make_grid_3d over bounds (0,1)^3."""
+    a = np.linspace(0.0, 1.0, resolution, dtype=np.float32)
+    return a, a, a
+
+
+def kuhn_mesh(n: int = 128, jitter: float = 0.2, seed: int = 4):
+    """Unit cube split into n^3 hexes, each into the 6 Kuhn (Freudenthal)
+    tetrahedra around its (0,0,0)-(1,1,1) diagonal (a conforming mesh).
+    Interior nodes are moved by U(-jitter*h, jitter*h) per coordinate.
+    Returns (points ((n+1)^3,3) f32, connectivity (6 n^3, 4) int32)."""
+    m = n + 1
+    g = np.arange(m, dtype=np.float64) / n
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        I, J, K = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+        interior = ((I > 0) & (I < n) & (J > 0) & (J < n) & (K > 0) & (K < n)).ravel()
+        d = rng.uniform(-jitter / n, jitter / n, (int(interior.sum()), 3))
+        pts[interior] += d
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    base = ((i * m + j) * m + k).ravel()
+    step = np.array([m * m, m, 1])
+    tets = []
+    for perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        a = base + step[perm[0]]
+        b = a + step[perm[1]]
+        tets.append(np.stack([base, a, b, base + step.sum()], 1))
+    conn = np.stack(tets, 1).reshape(-1, 4).astype(np.int32)
+    return pts.astype(F32), conn
+
+
+def poiseuille_vortex(points, times):
+    """u = (1 - r^2, -0.2 s(t) (z-1/2)/(1/2), 0.2 s(t) (y-1/2)/(1/2)) with
+    r = |(y-1/2, z-1/2)|/(1/2), s(t) = 1 + 0.1 sin 2 pi t.  (T, N, 3) f32."""
+    p = np.asarray(points, np.float64)
+    y, z = (p[:, 1] - 0.5) / 0.5, (p[:, 2] - 0.5) / 0.5
+    r2 = y * y + z * z
+    out = []
+    for t in np.asarray(times, np.float64):
+        s = 1.0 + 0.1 * np.sin(2 * np.pi * t)
+        out.append(np.stack([1.0 - r2, -0.2 * s * z, 0.2 * s * y], 1))
+    return np.stack(out).astype(F32)

@@ -1,0 +1,129 @@
+"""GPU parity of the unstructured FEM path (jt_octree_sample, jt_track_octree)
+against the float32 oracle: octree descent, first-hit point location,
+P1 interpolation of time-blended nodal values and the fallback, bitwise;
+tracking with the continuous-inlet grid boundary, bitwise per step."""
+import numpy as np
+import pytest
+import torch
+
+import jaxtrace_amd as jt
+from jaxtrace_amd import workloads as W
+from oracle import boundary as OB
+from oracle import fields as OF
+from oracle import integrators as OI
+from oracle import octree as OO
+from oracle import tracker as OT
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _field(n=8, T=4, leaf=8, depth=12, seed=0):
+    pts, conn = W.kuhn_mesh(n, 0.2, seed)
+    times = (np.arange(T) / 14.0).astype(F32)
+    data = W.poiseuille_vortex(pts, times)
+    f = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn, max_elements_per_leaf=leaf, max_depth=depth)
+    return f, pts, conn, data, times
+
+
+def test_tree_matches_oracle(gpu):
+    f, pts, conn, _, _ = _field()
+    want = OO.build(pts, conn, 8, 12)
+    for k, v in want.items():
+        np.testing.assert_array_equal(f.octree[k], v.astype(f.octree[k].dtype), err_msg=k)
+
+
+@pytest.mark.parametrize("t", [0.0, 0.05, 0.1])
+def test_sample_bitwise(gpu, t):
+    f, pts, conn, data, times = _field()
+    rng = np.random.default_rng(1)
+    Q = np.concatenate([rng.uniform(0, 1, (600, 3)), rng.uniform(-0.2, 1.2, (200, 3)), pts[:50]]).astype(F32)
+    got, found = f.sample_at_positions(Q, t, return_found=True)
+    Fb = OF.sample_all_nodes_at_time(data, times, F32(t))
+    want, wfound = [], []
+    for q in Q:
+        v, e = OO.interpolate(f.octree, pts, conn, Fb, q, 12)
+        want.append(v)
+        wfound.append(e >= 0)
+    np.testing.assert_array_equal(found.astype(bool), np.array(wfound))
+    np.testing.assert_array_equal(got, np.stack(want))
+
+
+def test_internal_node_fallback_uses_last_element(gpu):
+    """A query whose octant child is missing stays on an internal node whose
+    element row is all -1, so the fallback reads element -1 = the last one."""
+    f, pts, conn, data, times = _field(n=4, leaf=2, depth=12)
+    tree = f.octree
+    qs = []
+    for node in np.nonzero(tree["nodes_is_leaf"] == 0)[0]:
+        for o in np.nonzero(tree["nodes_children"][node] < 0)[0]:
+            mn, mx = tree["nodes_min"][node], tree["nodes_max"][node]
+            c = (mn + mx) / F32(2)
+            q = np.where([(o >> a) & 1 for a in range(3)], (c + mx) / 2, (mn + c) / 2).astype(F32)
+            if OO.locate(tree, q, 12) == node:
+                qs.append(q)
+    if not qs:
+        pytest.skip("mesh has no internal node with an empty octant")
+    Q = np.stack(qs)
+    got, found = f.sample_at_positions(Q, 0.0, return_found=True)
+    Fb = OF.sample_all_nodes_at_time(data, times, F32(0.0))
+    last = conn[-1]
+    for q, g in zip(Q, got):
+        d = ((pts[last] - q) ** 2).sum(1)
+        np.testing.assert_array_equal(g, Fb[last[int(np.argmin(d))]])
+    assert not found.any()
+
+
+def test_track_inlet_per_step_bitwise(gpu):
+    f, pts, conn, data, times = _field(n=10, T=8)
+    bounds = [[0, 0, 0], [1, 1, 1]]
+    conc = {"x": 64, "y": 64, "z": 64}
+    bc = jt.continuous_inlet_boundary_factory(bounds, "x", "positive", "grid", conc)
+    x0 = W.uniform_particles(1500, 5)
+    traj = jt.create_tracker("rk4", f, bc).track_particles(x0, (0.0, 0.03), 31, dt=1e-3)
+    Fcache = {}
+
+    def fn(q, t):
+        key = float(t)
+        if key not in Fcache:
+            Fcache[key] = OF.sample_all_nodes_at_time(data, times, F32(t))
+        return np.stack([OO.interpolate(f.octree, pts, conn, Fcache[key], p, 12)[0] for p in q])
+
+    tg, dt = OT.time_grid((0.0, 0.03), 31, 1e-3)
+    for s in (0, 1, 17, 29):
+        nxt = OB.continuous_inlet(OI.rk4_step(traj.positions[s], tg[s], dt, fn), bounds, "x", "positive",
+                                  "grid", conc)
+        np.testing.assert_array_equal(traj.positions[s + 1], nxt)
+
+
+def test_full_size_mesh_properties(gpu):
+    """128^3 hexes x 6 = 12,582,912 jittered tets, 8 snapshots, 2^22 particles,
+    RK4 dt = 1e-3 with the continuous inlet: positions stay in the unit box
+    and 256 sampled particles match the oracle step bitwise."""
+    pts, conn = W.kuhn_mesh(128, 0.2, 4)
+    times = (np.arange(8) / 14.0).astype(F32)
+    data = W.poiseuille_vortex(pts, times)
+    f = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn)
+    bounds = [[0, 0, 0], [1, 1, 1]]
+    conc = {"x": 64, "y": 64, "z": 64}
+    bc = jt.continuous_inlet_boundary_factory(bounds, "x", "positive", "grid", conc)
+    x0 = W.uniform_particles(1 << 22, 5)
+    tr = jt.create_tracker("rk4", f, bc, return_device=True, record_every=5)
+    out = tr.track_particles(x0, (0.0, 0.01), 11, dt=1e-3).positions
+    assert bool(((out >= 0) & (out <= 1)).all())
+    idx = np.random.default_rng(6).choice(1 << 22, 256, replace=False)
+    cur = out[1].cpu().numpy()[idx]                  # state after step 5
+    Fcache = {}
+
+    def fn(q, t):
+        key = float(t)
+        if key not in Fcache:
+            Fcache[key] = OF.sample_all_nodes_at_time(data, times, F32(t))
+        return np.stack([OO.interpolate(f.octree, pts, conn, Fcache[key], p, 12)[0] for p in q])
+
+    tg, dt = OT.time_grid((0.0, 0.01), 11, 1e-3)
+    # the GPU state after 5 more steps from `cur` equals 5 oracle steps
+    x = cur
+    for s in range(5, 10):
+        x = OB.continuous_inlet(OI.rk4_step(x, tg[s], dt, fn), bounds, "x", "positive", "grid", conc)
+    np.testing.assert_array_equal(out[2].cpu().numpy()[idx], x)

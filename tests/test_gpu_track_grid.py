@@ -19,7 +19,8 @@ from oracle import tracker as OT
 pytestmark = pytest.mark.gpu
 F32 = np.float32
 
-STORAGES = [("float32", "packed"), ("float32", "padded"), ("bfloat16", "packed"), ("float16", "packed")]
+STORAGES = [("float32", "packed"), ("float32", "padded"), ("bfloat16", "packed"), ("float16", "packed"),
+            ("float32", "halo"), ("float32", "halo4"), ("bfloat16", "halo"), ("float16", "halo")]
 
 
 def _rounded(data, storage):
@@ -118,12 +119,14 @@ def _cfg1(n=10_000):
     return data, times, g, x0
 
 
-@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
-def test_cfg1_rk4_periodic_per_step_parity(gpu, storage):
+@pytest.mark.parametrize("storage,layout", [("float32", "packed"), ("bfloat16", "packed"),
+                                            ("float32", "halo"), ("bfloat16", "halo")])
+def test_cfg1_rk4_periodic_per_step_parity(gpu, storage, layout):
     """Config: ABC flow 32^3 x 4 snapshots, 10,000 particles (seed 42), RK4,
     dt = 0.01, 100 steps, periodic unit box."""
     data, times, g, x0 = _cfg1()
-    f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", storage=storage)
+    f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", storage=storage,
+                                     layout=layout)
     bc = jt.periodic_boundary([[0, 0, 0], [1, 1, 1]])
     traj = jt.create_tracker("rk4", f, bc).track_particles(x0, (0.0, 1.0), 101, dt=0.01)
     assert traj.positions.shape == (101, 10_000, 3)
@@ -202,3 +205,16 @@ def test_large_mixed_bc_properties(gpu):
         nxt = OB.per_axis(OI.rk4_step(cur, tgrid[s], dt, field_fn), [[0, 0, 0], [1, 1, 1]],
                           ["periodic", "reflective", "reflective"])
         np.testing.assert_array_equal(p[s + 1].cpu().numpy()[idx], nxt)
+
+
+@pytest.mark.parametrize("layout", ["packed", "halo"])
+def test_cell_sorting_is_invisible(gpu, layout):
+    """Re-ordering particles by cell every k steps (and writing trajectories
+    back through the permutation) leaves every recorded position bitwise
+    unchanged."""
+    data, times, g, x0 = _cfg1(5000)
+    f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", layout=layout)
+    bc = jt.mixed_boundary([[0, 0, 0], [1, 1, 1]], {"x": "periodic", "y": "reflective", "z": "reflective"})
+    a = jt.create_tracker("rk4", f, bc).track_particles(x0, (0, 1), 41)
+    b = jt.create_tracker("rk4", f, bc, sort_every=7, steps_per_launch=5).track_particles(x0, (0, 1), 41)
+    np.testing.assert_array_equal(a.positions, b.positions)

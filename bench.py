@@ -47,17 +47,21 @@ def parse():
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (known HBM traffic) for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     return ap.parse_args()
 
 
-def run_chunks(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events):
+def run_chunks(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events, sort_every=0):
     """Launch jt_track_grid over [0, n_steps) in chunks of spl steps; returns
     per-launch event durations (ms) when record_events."""
     durs = []
     evs = []
     o = L.TrackOpts()
     o.steps_per_launch = spl
-    o.sort_every = 0
+    o.sort_every = sort_every
     o.block = 0
     s0 = 0
     while s0 < n_steps:
@@ -81,7 +85,7 @@ def run_chunks(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch,
     return durs
 
 
-def measure(field, bc_obj, x0, args, torch, L, lib):
+def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
     dev = field.device
     n = x0.shape[0]
     total = args.warmup + args.steps
@@ -90,16 +94,19 @@ def measure(field, bc_obj, x0, args, torch, L, lib):
     desc = field.descriptor()
     bc = bc_obj.descriptor(dev)
     o = L.TrackOpts()
+    o.sort_every = sort_every
     ws_bytes = int(lib.jt_track_grid_workspace_size(n, total, C.byref(o)))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = L.stream_handle()
     x = x0.clone()
-    run_chunks(lib, L, desc, bc, x, step_times, args.warmup, args.steps_per_launch, ws, stream, torch, False)
+    run_chunks(lib, L, desc, bc, x, step_times, args.warmup, args.steps_per_launch, ws, stream, torch, False,
+               sort_every)
     torch.cuda.synchronize()
     x = x0.clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    durs = run_chunks(lib, L, desc, bc, x, step_times, args.steps, args.steps_per_launch, ws, stream, torch, True)
+    durs = run_chunks(lib, L, desc, bc, x, step_times, args.steps, args.steps_per_launch, ws, stream, torch, True,
+                      sort_every)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, durs, x
@@ -162,6 +169,50 @@ def step_error_vs_fp64(host_data_f32, host_data_rounded, times, grid, states, t)
     return float(rel.max())
 
 
+def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32):
+    """Adjoint workload: 128^3 x 8 random Fourier field (as above, t in [0,1]),
+    2^20 particles (seed 7), RK4 dt = 2e-3, 500 steps, periodic box,
+    L = mean |x_T - x*|^2 (x* seed 8), checkpoints every 32 steps.  Returns
+    forward and backward wall times."""
+    from jaxtrace_amd import adjoint as ADJ
+    data, times, g = W.random_fourier(n=128, n_snap=8, t_end=1.0, seed=2, device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    x0 = torch.rand((n_part, 3), generator=gen).to(dev)
+    tgt = torch.from_numpy(np.random.default_rng(8).random((n_part, 3)).astype(np.float32)).to(dev)
+    bc = jt.periodic_boundary([[0, 0, 0], [1, 1, 1]])
+    out = {}
+    for storage in ("float32", "bfloat16"):
+        res = {}
+        for rep in range(2):                      # first repetition warms up
+            U = data.clone().requires_grad_(True)
+            X = x0.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xT = ADJ.track_rk4(U, times, g, g, g, X, (0.0, steps * DT), steps + 1, dt=DT, boundary=bc,
+                               storage=storage, checkpoint_every=K)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss = ((xT - tgt) ** 2).sum(1).mean()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            res = {"fwd_s": t1 - t0, "bwd_s": t2 - t1, "ratio_bwd_over_fwd": (t2 - t1) / (t1 - t0)}
+        out[storage] = res
+    out["workload"] = (f"128^3 x 8 random Fourier, {n_part} particles, RK4 dt=2e-3, {steps} steps, "
+                       f"checkpoints every {K}, loss mean |x_T - x*|^2")
+    return out
+
+
+def copy_calibration(torch):
+    """A plain copy of 2^28 floats (1 GiB read + 1 GiB written), five times."""
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": a.numel() * 4, "copies": 5}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -170,6 +221,9 @@ def main():
                                    "one-GPU number is one shard at its real shape)"}))
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     import jaxtrace_amd as jt
     from jaxtrace_amd import _lib as L
     from jaxtrace_amd import workloads as W
@@ -265,7 +319,15 @@ def main():
                       f"(NumPy restatement of the reference algorithm: every node blended per "
                       f"stage, then trilinear), {el:.1f} s",
             "host_cpus": len(os.sched_getaffinity(0))}
-    result["adjoint_fwd_ratio"] = None
+    try:
+        if args.no_adjoint:
+            raise RuntimeError("skipped (--no-adjoint)")
+        adj = bench_adjoint(torch, jt, W, dev)
+        result["adjoint"] = adj
+        result["adjoint_fwd_ratio"] = adj["float32"]["ratio_bwd_over_fwd"]
+    except Exception as e:  # reported, never fatal
+        result["adjoint"] = {"error": str(e)}
+        result["adjoint_fwd_ratio"] = None
     result["setup_s"] = gen_s
     print(json.dumps(result), flush=True)
 

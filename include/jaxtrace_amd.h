@@ -267,6 +267,42 @@ int jt_track_octree(const jt_octree_field* field, const jt_boundary* bc, int32_t
                     int64_t n, const float* step_times, float dt, int32_t n_steps, int32_t record_every,
                     float* traj, const jt_track_opts* opts, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- nearest-node TimeSeriesField ------------------------------------------ */
+/* TimeSeriesField (time_series.py:82-473): snapshots (n_snap, n_nodes, 3) at
+ * fixed node positions, sampled at the nearest node (ties: lowest index). */
+typedef struct jt_nodes_field {
+    const float* node_pos;    /* (n_nodes, 3) f32 */
+    int64_t n_nodes;
+    const void*  data;        /* (n_snap, n_nodes, 3) f32 */
+    const float* times;       /* (n_snap,) f32 */
+    int32_t n_snap, time_interp, time_extrap;
+} jt_nodes_field;
+
+/* Replaces TimeSeriesField.sample_at_positions (time_series.py:391-398):
+ * _sample_all_nodes_at_time_jax then _nearest_neighbor_sample_jax (:249-259).
+ * nearest (nullable) receives the chosen node index. */
+int jt_nodes_sample(const jt_nodes_field* field, const float* Q, int64_t m, float t, float* out,
+                    int32_t* nearest, void* stream);
+size_t jt_track_nodes_workspace_size(int64_t n, int32_t n_steps);
+/* jt_track_grid for a nearest-node field (same arguments and recording). */
+int jt_track_nodes(const jt_nodes_field* field, const jt_boundary* bc, int32_t integrator, float* x,
+                   int64_t n, const float* step_times, float dt, int32_t n_steps, int32_t record_every,
+                   float* traj, const jt_track_opts* opts, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- adjoint RK4 (build-only: the reference has no differentiation) ------- */
+size_t jt_track_grid_adjoint_workspace_size(int64_t n, int32_t n_steps, int32_t ckpt_every);
+/* Reverse-mode gradient of n_steps RK4 steps of jt_track_grid (same field,
+ * boundary, step_times and dt).  ckpt holds the forward positions at steps
+ * 0, K, 2K, ... (K = ckpt_every; exactly jt_track_grid's trajectory with
+ * record_every = K), (n_steps/K + 1, n, 3).  Given grad_xT = dL/dx_T, writes
+ * grad_x0 = dL/dx_0 and ADDS dL/dU into grad_U, a float32
+ * (n_snap, nx, ny, nz, 3) array the caller zeroes.  Atomic accumulation:
+ * grad_U is reproducible to rounding, not bitwise. */
+int jt_track_grid_adjoint(const jt_grid_field* field, const jt_boundary* bc, const float* ckpt,
+                          int64_t n, const float* step_times, float dt, int32_t n_steps,
+                          int32_t ckpt_every, const float* grad_xT, float* grad_x0, float* grad_U,
+                          void* ws, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,7 +5,8 @@ Host code holds torch tensors on the ROCm device and calls hand-written HIP
 kernels through the C ABI declared in ``include/jaxtrace_amd.h``.
 """
 from ._lib import load as _load_library  # noqa: F401
-from .fields import StructuredGridSampler, StructuredTimeSeriesField, OctreeFEMTimeSeriesFieldOptimized
+from .fields import (StructuredGridSampler, StructuredTimeSeriesField, OctreeFEMTimeSeriesFieldOptimized,
+                     TimeSeriesField)
 from .boundary import (
     BoundaryCondition,
     periodic_boundary,
@@ -20,15 +21,16 @@ from .boundary import (
     absorbing_boundary,
 )
 from .tracking import TrackerOptions, Trajectory, ParticleTracker, create_tracker
+from .adjoint import track_rk4
 from .density import HashGridNeighbors, SPHDensityEstimator, KDEEstimator, scott_bandwidth, silverman_bandwidth
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "StructuredGridSampler", "StructuredTimeSeriesField", "OctreeFEMTimeSeriesFieldOptimized",
+    "TimeSeriesField", "StructuredGridSampler", "StructuredTimeSeriesField", "OctreeFEMTimeSeriesFieldOptimized",
     "BoundaryCondition", "periodic_boundary", "reflective_boundary", "clamping_boundary",
     "absorbing_boundary_factory", "mixed_boundary", "continuous_inlet_boundary_factory",
     "apply_periodic", "clamp_to_bounds", "reflect_boundary", "absorbing_boundary",
     "TrackerOptions", "Trajectory", "ParticleTracker", "create_tracker",
-    "HashGridNeighbors", "SPHDensityEstimator", "KDEEstimator", "scott_bandwidth", "silverman_bandwidth",
+    "track_rk4", "HashGridNeighbors", "SPHDensityEstimator", "KDEEstimator", "scott_bandwidth", "silverman_bandwidth",
 ]

@@ -14,7 +14,7 @@ import os
 import torch  # noqa: F401  (must precede loading _jt.so)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_jt.so")
+LIB_PATH = os.environ.get("JAXTRACE_AMD_LIB") or os.path.join(_HERE, "_jt.so")
 
 JT_OK, JT_EINVAL, JT_ENOMEM, JT_EHIP, JT_EUNSUPPORTED = 0, 1, 2, 3, 4
 
@@ -74,6 +74,13 @@ class OctreeField(C.Structure):
     ]
 
 
+class NodesField(C.Structure):
+    _fields_ = [
+        ("node_pos", C.c_void_p), ("n_nodes", C.c_int64), ("data", C.c_void_p), ("times", C.c_void_p),
+        ("n_snap", C.c_int32), ("time_interp", C.c_int32), ("time_extrap", C.c_int32),
+    ]
+
+
 class TrackOpts(C.Structure):
     _fields_ = [("steps_per_launch", C.c_int32), ("sort_every", C.c_int32), ("block", C.c_int32)]
 
@@ -119,6 +126,16 @@ _SIGS = {
     "jt_track_octree": (C.c_int, [C.POINTER(OctreeField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
                                   C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                   C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "jt_nodes_sample": (C.c_int, [C.POINTER(NodesField), C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    "jt_track_nodes_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "jt_track_nodes": (C.c_int, [C.POINTER(NodesField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
+                                 C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
+                                 C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "jt_track_grid_adjoint_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.c_int32]),
+    "jt_track_grid_adjoint": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_void_p, C.c_int64,
+                                        C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
 }
 
 

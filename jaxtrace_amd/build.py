@@ -32,8 +32,8 @@ COMMON = [
     "-Wno-unused-result",
 ]
 
-# per-file extra flags
-EXTRA = {}
+# per-file extra flags: hardware float atomics for the adjoint's scatter
+EXTRA = {"adjoint.hip": ["-munsafe-fp-atomics"]}
 
 
 def hipcc() -> str:

@@ -374,6 +374,47 @@ static __global__ void k_blend_table(GridDev g, const float* __restrict__ step_t
     tab[3 * s + 2] = time_blend(g.times, g.T, t2, g.interp, g.extrap);
 }
 
+// One integrator step (rk4.py:84-106, rk2.py:157-177, euler.py:150-157) for
+// any sampler f(blend, q, out); tab holds the step's three stage blends.
+template <int INTEG, class F>
+__device__ __forceinline__ void rk_step(F&& f, const Blend* __restrict__ tab, int s, float p[3], float dt,
+                                        float dth, float dt6) {
+    const Blend b0 = tab[3 * s];
+    if constexpr (INTEG == JT_RK4) {
+        // acc accumulates ((k1 + 2 k2) + 2 k3) + k4 in that order as each
+        // stage completes, so only one k is live at a time
+        const Blend b1 = tab[3 * s + 1];
+        const Blend b2 = tab[3 * s + 2];
+        float k[3], q[3], acc[3];
+        f(b0, p, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { acc[c] = k[c]; q[c] = p[c] + dth * k[c]; }
+        f(b1, q, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { acc[c] = acc[c] + 2.0f * k[c]; q[c] = p[c] + dth * k[c]; }
+        f(b1, q, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { acc[c] = acc[c] + 2.0f * k[c]; q[c] = p[c] + dt * k[c]; }
+        f(b2, q, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = p[c] + dt6 * (acc[c] + k[c]);   // rk4.py:44
+    } else if constexpr (INTEG == JT_RK2) {
+        const Blend b1 = tab[3 * s + 1];
+        float v1[3], v2[3], q[3];
+        f(b0, p, v1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * v1[c];
+        f(b1, q, v2);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v2[c];
+    } else {
+        float v[3];
+        f(b0, p, v);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
+    }
+}
+
 // Host-side translation of the public descriptors.
 int make_grid_dev(const jt_grid_field* f, GridDev& g, int& store_kind);
 int make_bc_dev(const jt_boundary* b, BcDev& d);

@@ -136,31 +136,9 @@ k_oct_track(OctDev o, BcDev bc, float* __restrict__ xs, int n, const Blend* __re
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
+    auto f = [&](const Blend& b, const float q[3], float out[3]) { oct_sample(o, b, q, out); };
     for (int s = s0; s < s1; ++s) {
-        const Blend b0 = tab[3 * s];
-        if constexpr (INTEG == JT_RK4) {
-            const Blend b1 = tab[3 * s + 1], b2 = tab[3 * s + 2];
-            float k1[3], k2[3], k3[3], k4[3], q[3];
-            oct_sample(o, b0, p, k1);
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k1[c];
-            oct_sample(o, b1, q, k2);
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k2[c];
-            oct_sample(o, b1, q, k3);
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dt * k3[c];
-            oct_sample(o, b2, q, k4);
-            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt6 * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
-        } else if constexpr (INTEG == JT_RK2) {
-            const Blend b1 = tab[3 * s + 1];
-            float v1[3], v2[3], q[3];
-            oct_sample(o, b0, p, v1);
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * v1[c];
-            oct_sample(o, b1, q, v2);
-            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v2[c];
-        } else {
-            float v[3];
-            oct_sample(o, b0, p, v);
-            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
-        }
+        rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);
         apply_bc(bc, p);
         if (traj != nullptr && (s + 1) % rec_every == 0) {
             float* out = traj + ((size_t)((s + 1) / rec_every) * (size_t)n + i) * 3;

@@ -97,8 +97,12 @@ int make_bc_dev(const jt_boundary* b, BcDev& d) {
 // ---- kernels -------------------------------------------------------------
 
 
+#ifndef JT_TRACK_WAVES_PER_EU
+#define JT_TRACK_WAVES_PER_EU 4
+#endif
+
 template <int INTEG, int ST, bool ZH>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, JT_TRACK_WAVES_PER_EU)
 k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
         const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
         int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {
@@ -108,40 +112,9 @@ k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
     if (i >= n) return;
     float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
     const size_t dst = ids ? (size_t)ids[i] : (size_t)i;
+    auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
     for (int s = s0; s < s1; ++s) {
-        const Blend b0 = tab[3 * s];
-        if constexpr (INTEG == JT_RK4) {               // rk4.py:84-106
-            const Blend b1 = tab[3 * s + 1];
-            const Blend b2 = tab[3 * s + 2];
-            float k1[3], k2[3], k3[3], k4[3], q[3];
-            sample<ST, ZH>(g, b0, p[0], p[1], p[2], k1);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k1[c];
-            sample<ST, ZH>(g, b1, q[0], q[1], q[2], k2);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * k2[c];
-            sample<ST, ZH>(g, b1, q[0], q[1], q[2], k3);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dt * k3[c];
-            sample<ST, ZH>(g, b2, q[0], q[1], q[2], k4);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)                 // rk4.py:44
-                p[c] = p[c] + dt6 * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
-        } else if constexpr (INTEG == JT_RK2) {        // rk2.py:157-177
-            const Blend b1 = tab[3 * s + 1];
-            float v1[3], v2[3], q[3];
-            sample<ST, ZH>(g, b0, p[0], p[1], p[2], v1);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) q[c] = p[c] + dth * v1[c];
-            sample<ST, ZH>(g, b1, q[0], q[1], q[2], v2);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v2[c];
-        } else {                                        // euler.py:150-157
-            float v[3];
-            sample<ST, ZH>(g, b0, p[0], p[1], p[2], v);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
-        }
+        rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);     // rk4.py:84-106 / rk2.py / euler.py
         apply_bc(bc, p);                                // tracker.py:238-239
         if (traj != nullptr && (s + 1) % rec_every == 0) {
             const size_t r = (size_t)((s + 1) / rec_every);

@@ -319,3 +319,86 @@ class OctreeFEMTimeSeriesFieldOptimized:
 
     def get_time_bounds(self):
         return (self.t_min, self.t_max)
+
+
+class TimeSeriesField:
+    """time_series.py:82-473: (T, N, 3) snapshots at fixed node positions,
+    blended in time and sampled at the nearest node (brute force over all
+    nodes, as the reference; ties go to the lowest node index)."""
+
+    kind = "nodes"
+
+    def __init__(self, data, times, positions, interpolation: str = "linear", extrapolation: str = "constant",
+                 _source_info: Optional[dict] = None, device=None):
+        self.device = torch.device(device) if device is not None else L.device()
+        d = np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data, F32)
+        t = np.asarray(times.cpu() if isinstance(times, torch.Tensor) else times, F32).reshape(-1)
+        p = np.asarray(positions.cpu() if isinstance(positions, torch.Tensor) else positions, F32)
+        if p.ndim == 2 and p.shape[1] == 2:
+            p = np.concatenate([p, np.zeros((p.shape[0], 1), F32)], 1)
+        if d.ndim != 3:                                                       # :120-128
+            raise ValueError(f"data must have 3 dims (T,N,3), got {d.shape}")
+        T, N, D = d.shape
+        if D != 3:
+            raise ValueError(f"data last dimension must be 3, got {D}")
+        if t.shape != (T,):
+            raise ValueError(f"times shape {t.shape} != (T,) with T={T}")
+        if p.shape != (N, 3):
+            raise ValueError(f"positions shape {p.shape} != (N,3) with N={N}")
+        interp, extra = interpolation.lower(), extrapolation.lower()
+        if interp not in ("nearest", "linear", "cubic"):
+            raise ValueError(f"Unsupported interpolation: {interpolation}")
+        if extra not in L.JT_EXTRAP:
+            raise ValueError(f"Unsupported extrapolation: {extrapolation}")
+        self.data, self.times, self.positions = d, t, p
+        self.interpolation, self.extrapolation = interp, extra
+        self.T, self.N = T, N
+        self.t_min, self.t_max = float(t[0]), float(t[-1])
+        self._source_info = _source_info or {}
+        self.storage = "float32"
+        self._dev = {"pos": torch.from_numpy(np.ascontiguousarray(p)).to(self.device),
+                     "data": torch.from_numpy(np.ascontiguousarray(d)).to(self.device),
+                     "times": torch.from_numpy(t.copy()).to(self.device)}
+        desc = L.NodesField()
+        desc.node_pos = self._dev["pos"].data_ptr()
+        desc.n_nodes = N
+        desc.data = self._dev["data"].data_ptr()
+        desc.times = self._dev["times"].data_ptr()
+        desc.n_snap = T
+        desc.time_interp = L.JT_TIME.get(interp, 0)
+        desc.time_extrap = L.JT_EXTRAP[extra]
+        self._desc = desc
+
+    def descriptor(self):
+        if self.interpolation == "cubic":
+            raise NotImplementedError("cubic interpolation not supported with JAX")
+        return self._desc
+
+    def sample_at_positions(self, query_positions, t: float):
+        lib = L.load()
+        x, is_torch = _ensure_positions(query_positions, self.device)
+        out = torch.empty_like(x)
+        L.check(lib.jt_nodes_sample(C.byref(self.descriptor()), L.ptr(x), x.shape[0], C.c_float(F32(t)),
+                                    L.ptr(out), None, L.stream_handle()), "jt_nodes_sample")
+        return out if is_torch else out.cpu().numpy()
+
+    _sample = sample_at_positions
+
+    def _parse(self, a, b):
+        if np.ndim(a) == 0 and np.ndim(b) != 0:
+            return float(a), b
+        if np.ndim(b) == 0 and np.ndim(a) != 0:
+            return float(b), a
+        raise TypeError("Cannot determine (t, positions) from arguments")
+
+    def __call__(self, a, b):
+        t, pos = self._parse(a, b)
+        return self.sample_at_positions(pos, t)
+
+    evaluate = __call__
+
+    def get_time_bounds(self):
+        return (self.t_min, self.t_max)
+
+    def get_spatial_bounds(self):
+        return self.positions.min(axis=0), self.positions.max(axis=0)

@@ -21,7 +21,7 @@ import torch
 
 from . import _lib as L
 from .boundary import BoundaryCondition
-from .fields import OctreeFEMTimeSeriesFieldOptimized, _GridSeries, _ensure_positions
+from .fields import OctreeFEMTimeSeriesFieldOptimized, TimeSeriesField, _GridSeries, _ensure_positions
 
 F32 = np.float32
 _INTEGRATORS = {"euler": L.JT_EULER, "rk2": L.JT_RK2, "rk4": L.JT_RK4}
@@ -157,20 +157,24 @@ class ParticleTracker:
         desc = fld.descriptor()
         bc = self.boundary.descriptor(dev)
         o = self._opts()
-        octree = getattr(fld, "kind", "grid") == "octree"
-        if octree:
+        kind = getattr(fld, "kind", "grid")
+        if kind == "octree":
             ws_bytes = int(lib.jt_track_octree_workspace_size(n, n_steps))
+            entry, name = lib.jt_track_octree, "jt_track_octree"
+        elif kind == "nodes":
+            ws_bytes = int(lib.jt_track_nodes_workspace_size(n, n_steps))
+            entry, name = lib.jt_track_nodes, "jt_track_nodes"
         else:
             ws_bytes = int(lib.jt_track_grid_workspace_size(n, n_steps, C.byref(o)))
+            entry, name = lib.jt_track_grid, "jt_track_grid"
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         traj = torch.empty((n_rec, n, 3), dtype=torch.float32, device=dev)
         state = x.clone()
         t0 = time.perf_counter()
-        entry = lib.jt_track_octree if octree else lib.jt_track_grid
         L.check(entry(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
                       L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
                       rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
-                      L.stream_handle()), "jt_track_octree" if octree else "jt_track_grid")
+                      L.stream_handle()), name)
         velocities = None
         if opt.record_velocities:
             velocities = torch.empty_like(traj)
@@ -193,9 +197,9 @@ def create_tracker(integrator_name: str, field, boundary_condition, **options) -
     """tracker.py:810-855."""
     if integrator_name not in _INTEGRATORS:
         raise ValueError(f"Unknown integrator: {integrator_name}. Available: {list(_INTEGRATORS)}")
-    if not isinstance(field, (_GridSeries, OctreeFEMTimeSeriesFieldOptimized)):
+    if not isinstance(field, (_GridSeries, OctreeFEMTimeSeriesFieldOptimized, TimeSeriesField)):
         raise TypeError("field must be a device-backed jaxtrace_amd field "
-                        "(StructuredGridSampler, StructuredTimeSeriesField, "
+                        "(TimeSeriesField, StructuredGridSampler, StructuredTimeSeriesField, "
                         f"OctreeFEMTimeSeriesFieldOptimized); got {type(field).__name__}")
     if not isinstance(boundary_condition, BoundaryCondition):
         if callable(boundary_condition):

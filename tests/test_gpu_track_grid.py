@@ -218,3 +218,25 @@ def test_cell_sorting_is_invisible(gpu, layout):
     a = jt.create_tracker("rk4", f, bc).track_particles(x0, (0, 1), 41)
     b = jt.create_tracker("rk4", f, bc, sort_every=7, steps_per_launch=5).track_particles(x0, (0, 1), 41)
     np.testing.assert_array_equal(a.positions, b.positions)
+
+
+def test_time_series_field_nearest_node(gpu):
+    """TimeSeriesField (time_series.py:82-473): blend every node, take the
+    nearest node's value; exhaustive argmin with ties to the lowest index."""
+    rng = np.random.default_rng(12)
+    nodes = rng.uniform(0, 1, (3000, 3)).astype(F32)
+    nodes[100] = nodes[7]                       # duplicate node: tie -> index 7
+    times = np.array([0.0, 0.4, 1.0], F32)
+    data = rng.standard_normal((3, 3000, 3)).astype(F32)
+    f = jt.TimeSeriesField(data, times, nodes)
+    Q = np.concatenate([rng.uniform(-0.1, 1.1, (700, 3)), nodes[[7, 100]]]).astype(F32)
+    for t in (-1.0, 0.0, 0.25, 1.0, 3.0):
+        want = OF.nearest_node_sample(data, times, nodes, Q, F32(t))
+        np.testing.assert_array_equal(f.sample_at_positions(Q, t), want)
+    # tracking through the nearest-node field, bitwise per step
+    bc = jt.clamping_boundary([[0, 0, 0], [1, 1, 1]])
+    x0 = W.uniform_particles(500, 13)
+    traj = jt.create_tracker("rk4", f, bc).track_particles(x0, (0.0, 0.5), 11)
+    fn = lambda p, t: OF.nearest_node_sample(data, times, nodes, p, t)
+    tt, pos = OT.track(x0, (0.0, 0.5), 11, fn, lambda p: OB.per_axis(p, [[0, 0, 0], [1, 1, 1]], ["clamping"] * 3))
+    np.testing.assert_array_equal(traj.positions, pos)

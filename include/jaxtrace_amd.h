@@ -115,6 +115,11 @@ typedef struct jt_track_opts {
     int32_t steps_per_launch; /* 0 = library default                           */
     int32_t sort_every;       /* re-sort particles by cell every k steps; 0 = never */
     int32_t block;            /* threads per block; 0 = default (256)          */
+    int32_t n_launch_events;  /* size of launch_events (may be 0)              */
+    void**  launch_events;    /* optional hipEvent_t array: events[2j] and
+                                 events[2j+1] are recorded on the stream right
+                                 before and after the j-th tracking-kernel
+                                 launch (profiling hook; NULL = none)          */
 } jt_track_opts;
 
 /* ---- library ------------------------------------------------------------- */

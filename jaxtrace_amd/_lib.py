@@ -82,7 +82,8 @@ class NodesField(C.Structure):
 
 
 class TrackOpts(C.Structure):
-    _fields_ = [("steps_per_launch", C.c_int32), ("sort_every", C.c_int32), ("block", C.c_int32)]
+    _fields_ = [("steps_per_launch", C.c_int32), ("sort_every", C.c_int32), ("block", C.c_int32),
+                ("n_launch_events", C.c_int32), ("launch_events", C.POINTER(C.c_void_p))]
 
 
 _lib = None

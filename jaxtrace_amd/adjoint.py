@@ -48,6 +48,7 @@ class _GridTrackRK4(torch.autograd.Function):
         ckpt = torch.empty((n_ck, n, 3), dtype=torch.float32, device=dev)
         state = x0.detach().to(torch.float32).contiguous().clone()
         o = L.TrackOpts()
+        o.sort_every = spec.sort_every if n >= 65536 else 0
         ws_bytes = int(lib.jt_track_grid_workspace_size(n, spec.n_steps, C.byref(o)))
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         L.check(lib.jt_track_grid(C.byref(desc), C.byref(bc), L.JT_RK4, L.ptr(state), n, L.ptr(spec.step_times),
@@ -80,8 +81,8 @@ class _GridTrackRK4(torch.autograd.Function):
 
 def track_rk4(U: torch.Tensor, times, grid_x, grid_y, grid_z, x0: torch.Tensor, time_span, n_timesteps: int,
               dt: Optional[float] = None, boundary: Optional[BoundaryCondition] = None,
-              bounds_handling: str = "periodic", storage: str = "float32", layout: str = "packed",
-              checkpoint_every: int = 32) -> torch.Tensor:
+              bounds_handling: str = "periodic", storage: str = "float32", layout: str = "halo",
+              checkpoint_every: int = 32, sort_every: int = 32) -> torch.Tensor:
     """Final positions after RK4 tracking, differentiable w.r.t. ``x0`` and ``U``.
 
     Time grid and dt follow track_particles (tracker.py:346-351)."""
@@ -102,5 +103,6 @@ def track_rk4(U: torch.Tensor, times, grid_x, grid_y, grid_z, x0: torch.Tensor, 
     spec.bounds = bounds_handling
     spec.storage, spec.layout = storage, layout
     spec.K = int(checkpoint_every)
+    spec.sort_every = int(sort_every)
     spec.bc = boundary if boundary is not None else BoundaryCondition("_open_bc", np.array([[0, 0, 0], [1, 1, 1]], F32))
     return _GridTrackRK4.apply(x0, U, spec)

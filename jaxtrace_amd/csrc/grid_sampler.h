@@ -26,6 +26,7 @@ struct GridDev {
     int nx, ny, nz;
     int nzs;                 // allocated z extent (nz, or nz+1 with a z halo)
     float o[3], d[3];        // origin, spacing
+    float inv[3];            // 1/spacing when that is exact (spacing a power of two), else 0
     float vhi[3];            // origin + f32(n-1)*d: inclusive validity bound
     float cmax[3];           // f32(max(n-1, 0)): clip bound of the grid coordinate
     int bmode;               // JT_BOUNDS_*
@@ -207,7 +208,10 @@ __device__ __forceinline__ void corner_pair(const void* b0, const void* b1, cons
 // structured.py:176-220 for one axis.
 __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, int n,
                                            int& i0, int& i1, float& w, bool& valid) {
-    float f = (x - g.o[ax]) / g.d[ax];
+    // true division (structured.py:176-178); when 1/d is exact (d a power of
+    // two) the product is the identical correctly rounded quotient
+    const float u = x - g.o[ax];
+    float f = (g.inv[ax] != 0.0f) ? u * g.inv[ax] : u / g.d[ax];
     if (g.bmode == JT_BOUNDS_PERIODIC) {
         f = jmodf(f, (float)n);
     } else {

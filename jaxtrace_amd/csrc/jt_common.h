@@ -46,7 +46,15 @@ __device__ __forceinline__ float f16_to_f32(uint16_t h) {
 }
 
 // jnp.mod on floats: fmod, then add the divisor when the signs differ.
+// For n > 0 the common cases are exact without fmod: a in [0, n) gives a;
+// a in (-n, 0) gives fl(a + n) (fmod returns a itself); a in [n, 2n) gives
+// a - n, which is exact (Sterbenz).  Other inputs take the general path.
 __device__ __forceinline__ float jmodf(float a, float n) {
+    if (n > 0.0f) {
+        if (a >= 0.0f && a < n) return a;
+        if (a < 0.0f && a > -n) return a + n;
+        if (a >= n && a < 2.0f * n) return a - n;
+    }
     float r = fmodf(a, n);
     if (r != 0.0f && ((r < 0.0f) != (n < 0.0f))) r = r + n;
     return r;

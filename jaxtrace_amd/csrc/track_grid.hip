@@ -16,6 +16,7 @@
 // XCD's L2 sees one contiguous (spatially compact, once sorted) particle range.
 #include "grid_sampler.h"
 #include <hipcub/hipcub.hpp>
+#include <cmath>
 
 namespace jt {
 
@@ -50,6 +51,11 @@ int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
     for (int a = 0; a < 3; ++a) {
         g.o[a] = f->origin[a];
         g.d[a] = f->spacing[a];
+        {   // power-of-two spacing: its reciprocal is exact
+            int e;
+            float m = frexpf(f->spacing[a], &e);
+            g.inv[a] = (m == 0.5f && std::isfinite(f->spacing[a])) ? 1.0f / f->spacing[a] : 0.0f;
+        }
         // structured.py:188-190: x_min + (N-1)*dx, int32*f32 -> f32
         float nm1 = (float)(n[a] - 1);
         float span = nm1 * f->spacing[a];
@@ -343,6 +349,8 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     while ((maxn - 1) >> shift >= 1024) ++shift;       // 10 bits per axis
     float* cur = x;
     const int* cur_ids = nullptr;
+    const int n_ev = (opts && opts->launch_events) ? opts->n_launch_events : 0;
+    int launch = 0;
     for (int s0 = 0; s0 < n_steps;) {
         int s1 = s0 + spl < n_steps ? s0 + spl : n_steps;
         if (sort_every > 0) {
@@ -360,6 +368,7 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
             }
         }
         const bool zh = g.nzs != g.nz;
+        if (2 * launch + 1 < n_ev) JT_HIP(hipEventRecord((hipEvent_t)opts->launch_events[2 * launch], s));
         switch (integrator) {
         case JT_RK4:
             if (zh) launch_track_st<JT_RK4, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
@@ -375,6 +384,8 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
             break;
         }
         JT_HIP(hipGetLastError());
+        if (2 * launch + 1 < n_ev) JT_HIP(hipEventRecord((hipEvent_t)opts->launch_events[2 * launch + 1], s));
+        ++launch;
         s0 = s1;
     }
     if (cur != x) {

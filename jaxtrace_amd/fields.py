@@ -9,9 +9,11 @@ errors) while holding torch tensors on the ROCm device:
   with StructuredGridSampler's trilinear interpolation (structured.py:167-250).
   The reference has no single class for it; this is that composition.
 
-Snapshots may be stored in float32 (zero-copy layout (T,Nx,Ny,Nz,3)) or in
-bfloat16/float16 (padded to 4 components, 8 bytes per node) to halve the
-gather traffic; all arithmetic stays float32.
+Snapshots may be stored in float32 or in bfloat16/float16 (padded to 4
+components, 8 bytes per node) to halve the gather traffic; all arithmetic
+stays float32.  The default 'halo' layout adds one z-plane so the two
+z-corners of a cell are one contiguous load; 'packed' keeps the caller's
+float32 (T,Nx,Ny,Nz,3) array without a copy.
 """
 from __future__ import annotations
 
@@ -160,7 +162,7 @@ class StructuredGridSampler(_GridSeries):
     whose temporal blend is an exact copy."""
 
     def __init__(self, velocity_data, grid_x, grid_y, grid_z, bounds_handling: str = "clamp",
-                 grid_meta=None, storage: str = "float32", layout: str = "packed", device=None):
+                 grid_meta=None, storage: str = "float32", layout: str = "halo", device=None):
         v = velocity_data if isinstance(velocity_data, torch.Tensor) else torch.from_numpy(
             np.ascontiguousarray(np.asarray(velocity_data, dtype=F32)))
         if v.ndim != 4:
@@ -184,7 +186,7 @@ class StructuredTimeSeriesField(_GridSeries):
 
     def __init__(self, data, times, grid_x, grid_y, grid_z, bounds_handling: str = "clamp",
                  interpolation: str = "linear", extrapolation: str = "constant",
-                 storage: str = "float32", layout: str = "packed", device=None,
+                 storage: str = "float32", layout: str = "halo", device=None,
                  _source_info: Optional[dict] = None):
         v = data if isinstance(data, torch.Tensor) else torch.from_numpy(
             np.ascontiguousarray(np.asarray(data, dtype=F32)))
@@ -211,7 +213,7 @@ class StructuredTimeSeriesField(_GridSeries):
     def get_time_bounds(self):
         return (self.t_min, self.t_max)
 
-    def with_storage(self, storage: str, layout: str = "packed") -> "StructuredTimeSeriesField":
+    def with_storage(self, storage: str, layout: str = "halo") -> "StructuredTimeSeriesField":
         """Same field, snapshots re-stored as ``storage`` (RNE from float32)."""
         src = self._snaps[:, :, :, :self.Nz, :3].to(torch.float32)
         return StructuredTimeSeriesField(src, self.times, self.grid_x, self.grid_y, self.grid_z,

@@ -38,7 +38,8 @@ class TrackerOptions:
       record_final_only  keep only x0 and the final positions
       return_device      Trajectory arrays stay torch tensors on the GPU
       steps_per_launch   steps fused into one kernel launch (0 = library default)
-      sort_every         re-sort particles by grid cell every k steps (0 = off)
+      sort_every         re-sort particles by grid cell every k steps (0 = off;
+                         None = 50 for >= 65536 particles on structured fields)
     """
     max_memory_gb: float = 8.0
     batch_size: Optional[int] = None
@@ -63,7 +64,7 @@ class TrackerOptions:
     record_final_only: bool = False
     return_device: bool = False
     steps_per_launch: int = 0
-    sort_every: int = 0
+    sort_every: Optional[int] = None
 
 
 class Trajectory:
@@ -125,10 +126,13 @@ class ParticleTracker:
     def integrator(self):
         return self.integrator_name
 
-    def _opts(self):
+    def _opts(self, n: int):
         o = L.TrackOpts()
         o.steps_per_launch = int(self.options.steps_per_launch)
-        o.sort_every = int(self.options.sort_every)
+        se = self.options.sort_every
+        if se is None:
+            se = 50 if (n >= 65536 and getattr(self.field, "kind", "grid") == "grid") else 0
+        o.sort_every = int(se)
         o.block = 0
         return o
 
@@ -156,7 +160,7 @@ class ParticleTracker:
         step_times = torch.from_numpy(times[:-1].copy() if n_steps > 0 else np.zeros(1, F32)).to(dev)
         desc = fld.descriptor()
         bc = self.boundary.descriptor(dev)
-        o = self._opts()
+        o = self._opts(n)
         kind = getattr(fld, "kind", "grid")
         if kind == "octree":
             ws_bytes = int(lib.jt_track_octree_workspace_size(n, n_steps))

@@ -51,14 +51,15 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
     return 1.0f;
 }
 
-// ---- block-level pre-reduction of the dL/dU scatter ----------------------
-// The threads of a block hold cell-sorted particles, so the corners they
-// scatter into cluster in a small box of nodes.  Per stage the block reduces
-// that box; if it holds at most TILE_NODES nodes (and does not wrap around a
-// periodic edge) contributions are summed in LDS with ds_add_f32 and each
-// touched node is flushed to HBM with one float atomic per component;
-// otherwise every contribution goes straight to HBM.
-#define TILE_NODES 2048
+// ---- block-level accumulation of the dL/dU scatter ------------------------
+// The threads of a block hold cell-sorted particles, so over a few steps the
+// corners they scatter into stay in a small box of nodes.  A reverse launch
+// sums its whole sub-segment's contributions for the box (two snapshot
+// slots) in LDS with ds_add_f32 and flushes each touched entry to HBM once
+// with a float atomic; corners outside the box (or in another snapshot) go
+// straight to HBM.
+#define TILE_NODES 3072
+#define ADJ_BLOCK 256
 
 __device__ __forceinline__ int wave_min_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -68,6 +69,12 @@ __device__ __forceinline__ int wave_max_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
     return v;
 }
+
+struct Tile {
+    int b[3], e[3];     // box origin (node indices) and extents
+    int smin;           // first snapshot slot
+    int vol;            // e0*e1*e2 (0: no box)
+};
 
 struct Stencil {
     int I[2], J[2], K[2];
@@ -108,170 +115,219 @@ __device__ __forceinline__ void vjp_local(const GridDev& g, const Blend& bl, con
             }
 }
 
-// Block-cooperative scatter of coef_snap * W_c * a into dL/dU.  Every thread
-// of the block must call it (active=false for threads without a particle).
-__device__ inline void block_scatter(const GridDev& g, const Blend& bl, const Stencil& st, const float a[3],
-                                     bool active, float* __restrict__ gU, float* tile, int* sred) {
-    float c0, c1;
-    blend_coefs(bl, c0, c1);
+// Add coef_snap * W_c * a for the 8 corners: into the LDS tile when the
+// corner and snapshot are covered, else straight to HBM.
+__device__ __forceinline__ void scatter(const GridDev& g, const Blend& bl, const Stencil& st, const float a[3],
+                                        const Tile& tl, float* tile, float* __restrict__ gU) {
+    float c[2];
+    blend_coefs(bl, c[0], c[1]);
+    const int snap[2] = {bl.i0, bl.i1};
     const size_t gsnap = (size_t)g.nx * g.ny * g.nz * 3;
-    float* g0 = gU + (size_t)bl.i0 * gsnap;
-    float* g1 = gU + (size_t)bl.i1 * gsnap;
-    // box of the block's corners
-    int lo[3], hi[3];
-    const int* ax[3] = {st.I, st.J, st.K};
-    for (int d = 0; d < 3; ++d) {
-        int l = min(ax[d][0], ax[d][1]), h = max(ax[d][0], ax[d][1]);
-        if (ax[d][1] < ax[d][0]) { l = 0; h = 1 << 20; }          // periodic wrap: no tile
-        lo[d] = active ? l : 0x7fffffff;
-        hi[d] = active ? h : -1;
+    const int tv = tl.vol * 3;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int ci = q >> 2, cj = (q >> 1) & 1, ck = q & 1;
+        const int I = st.I[ci], J = st.J[cj], K = st.K[ck];
+        const int li = I - tl.b[0], lj = J - tl.b[1], lk = K - tl.b[2];
+        const bool in = tl.vol > 0 && (unsigned)li < (unsigned)tl.e[0] && (unsigned)lj < (unsigned)tl.e[1] &&
+                        (unsigned)lk < (unsigned)tl.e[2];
+        const int loc = (li * tl.e[1] + lj) * tl.e[2] + lk;
+        const size_t node = ((size_t)(I * g.ny + J) * g.nz + K) * 3;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (c[h] == 0.0f) continue;
+            const float f = c[h] * st.W[q];
+            const int slot = snap[h] - tl.smin;
+            if (in && (slot == 0 || slot == 1)) {
+                float* tp = tile + slot * tv + loc * 3;
+                atomicAdd(tp, f * a[0]);
+                atomicAdd(tp + 1, f * a[1]);
+                atomicAdd(tp + 2, f * a[2]);
+            } else {
+                float* gp = gU + (size_t)snap[h] * gsnap + node;
+                atomicAdd(gp, f * a[0]);
+                atomicAdd(gp + 1, f * a[1]);
+                atomicAdd(gp + 2, f * a[2]);
+            }
+        }
+    }
+}
+
+// Forward recompute of one checkpoint segment [sa, sb): thread t takes
+// particle perm[t] (cell-sorted) and stores the state before each step at
+// seg[(s - sa) * n + pid] (particle order, so any later ordering can read it).
+template <int ST, bool ZH>
+__global__ void __launch_bounds__(256)
+k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const int* __restrict__ perm,
+                const Blend* __restrict__ tab, int sa, int sb, float dt, float dth, float dt6,
+                float* __restrict__ seg) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int pid = perm[t];
+    float x[3] = {x0s[3 * (size_t)pid], x0s[3 * (size_t)pid + 1], x0s[3 * (size_t)pid + 2]};
+    auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
+    for (int s = sa; s < sb; ++s) {
+        float* o = seg + ((size_t)(s - sa) * n + pid) * 3;
+        o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
+        rk_step<JT_RK4>(f, tab, s, x, dt, dth, dt6);
+        apply_bc(bc, x);
+    }
+}
+
+// Reverse steps s_hi-1 .. s_lo of the segment starting at sa.  Thread t takes
+// particle perm[t], sorted by its cell at step s_lo, so the block's corners
+// stay in a small box; its adjoint lives in lam_io[pid] between launches.
+template <int ST, bool ZH>
+__global__ void __launch_bounds__(ADJ_BLOCK)
+k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ perm, const Blend* __restrict__ tab,
+              int sa, int s_lo, int s_hi, float dt, float dth, float dt6, const float* __restrict__ seg,
+              float* __restrict__ lam_io, float* __restrict__ gU, float* __restrict__ gx0) {
+    __shared__ float tile[2 * TILE_NODES * 3];
+    __shared__ int sred[(ADJ_BLOCK / 64) * 6];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < n;                           // every thread stays: the block synchronises
+    const int pid = live ? perm[t] : perm[0];
+    // box: cells of this thread's states over the sub-segment, widened by the
+    // stencil reach (one cell below, two above), reduced over the block
+    int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+    if (live) {
+        const int nn[3] = {g.nx, g.ny, g.nz};
+        for (int s = s_lo; s < s_hi; ++s) {
+            const float* xi = seg + ((size_t)(s - sa) * n + pid) * 3;
+            for (int d = 0; d < 3; ++d) {
+                float f = floorf((xi[d] - g.o[d]) / g.d[d]);
+                int c = (f != f) ? 0 : (int)fminf(fmaxf(f, -4.0f), (float)(nn[d] + 4));
+                lo[d] = min(lo[d], c - 1);
+                hi[d] = max(hi[d], c + 2);
+            }
+        }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int d = 0; d < 3; ++d) {
         int l = wave_min_i(lo[d]), h = wave_max_i(hi[d]);
         if (lane == 0) { sred[wv * 6 + d] = l; sred[wv * 6 + 3 + d] = h; }
     }
+    for (int e = threadIdx.x; e < 2 * TILE_NODES * 3; e += blockDim.x) tile[e] = 0.0f;
     __syncthreads();
-    int bx[3], ex[3];
-    long long vol = 1;
-    for (int d = 0; d < 3; ++d) {
-        int l = sred[d], h = sred[3 + d];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { l = min(l, sred[w * 6 + d]); h = max(h, sred[w * 6 + 3 + d]); }
-        bx[d] = l;
-        ex[d] = h - l + 1;
-        vol *= (ex[d] > 0 ? ex[d] : 0);
-    }
-    __syncthreads();                                   // sred is reused by the next call
-    if (vol <= 0) return;                              // no active thread in the block
-    if (vol > TILE_NODES) {                            // direct atomics
-        if (!active) return;
-        for (int c = 0; c < 8; ++c) {
-            const int ci = c >> 2, cj = (c >> 1) & 1, ck = c & 1;
-            const size_t node = ((size_t)(st.I[ci] * g.ny + st.J[cj]) * g.nz + st.K[ck]) * 3;
-            if (c0 != 0.0f) { const float f = c0 * st.W[c]; for (int k = 0; k < 3; ++k) atomicAdd(g0 + node + k, f * a[k]); }
-            if (c1 != 0.0f) { const float f = c1 * st.W[c]; for (int k = 0; k < 3; ++k) atomicAdd(g1 + node + k, f * a[k]); }
+    Tile tl;
+    {
+        const int nn[3] = {g.nx, g.ny, g.nz};
+        long long vol = 1;
+        int ext[3];
+        for (int d = 0; d < 3; ++d) {
+            int l = sred[d], h = sred[3 + d];
+            for (int w = 1; w < ADJ_BLOCK / 64; ++w) { l = min(l, sred[w * 6 + d]); h = max(h, sred[w * 6 + 3 + d]); }
+            l = max(l, 0);
+            h = min(h, nn[d] - 1);
+            tl.b[d] = l;
+            ext[d] = h - l + 1;
+            vol *= (ext[d] > 0 ? ext[d] : 0);
         }
-        return;
-    }
-    const int tv = (int)vol * 3;
-    for (int e = threadIdx.x; e < 2 * tv; e += blockDim.x) tile[e] = 0.0f;
-    __syncthreads();
-    if (active) {
-        for (int c = 0; c < 8; ++c) {
-            const int ci = c >> 2, cj = (c >> 1) & 1, ck = c & 1;
-            const int li = ((st.I[ci] - bx[0]) * ex[1] + (st.J[cj] - bx[1])) * ex[2] + (st.K[ck] - bx[2]);
-            const float f0 = c0 * st.W[c], f1 = c1 * st.W[c];
-            for (int k = 0; k < 3; ++k) {
-                atomicAdd(&tile[li * 3 + k], f0 * a[k]);
-                atomicAdd(&tile[tv + li * 3 + k], f1 * a[k]);
+        if (vol > TILE_NODES) {                        // keep a central sub-box that fits
+            int cap = 1;
+            while ((cap + 1) * (cap + 1) * (cap + 1) <= TILE_NODES) ++cap;
+            vol = 1;
+            for (int d = 0; d < 3; ++d) {
+                if (ext[d] > cap) { tl.b[d] += (ext[d] - cap) / 2; ext[d] = cap; }
+                vol *= ext[d];
             }
         }
+        for (int d = 0; d < 3; ++d) tl.e[d] = ext[d];
+        tl.vol = vol > 0 ? (int)vol : 0;
+        int smin = tab[3 * s_lo].i0;                   // snapshot slots: the sub-segment's lowest index
+        for (int s = s_lo; s < s_hi; ++s)
+            for (int k = 0; k < 3; ++k) smin = min(smin, min(tab[3 * s + k].i0, tab[3 * s + k].i1));
+        tl.smin = smin;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 2 * tv; e += blockDim.x) {
-        const float v = tile[e];
-        if (v == 0.0f) continue;
-        const int snap = e >= tv;
-        const int r = e - snap * tv;
-        const int node = r / 3, comp = r - node * 3;
-        const int k = node % ex[2] + bx[2];
-        const int t2 = node / ex[2];
-        const int j = t2 % ex[1] + bx[1];
-        const int i = t2 / ex[1] + bx[0];
-        float* gs = snap ? g1 : g0;
-        atomicAdd(gs + ((size_t)(i * g.ny + j) * g.nz + k) * 3 + comp, v);
-    }
-    __syncthreads();                                   // tile is reused by the next call
-}
-
-template <int ST, bool ZH>
-__global__ void __launch_bounds__(256)
-k_adjoint(GridDev g, BcDev bc, const float* __restrict__ ckpt, int n, const int* __restrict__ perm,
-          const Blend* __restrict__ tab, int n_steps, int K, int c, float dt, float dth, float dt6,
-          float* __restrict__ lam_io, float* __restrict__ seg, float* __restrict__ gU, float* __restrict__ gx0) {
-    __shared__ float tile[2 * TILE_NODES * 3];
-    __shared__ int sred[4 * 6];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = t < n;                           // every thread stays: block_scatter synchronises
-    const int pid = live ? (perm ? perm[t] : t) : 0;
     float lam[3] = {lam_io[3 * (size_t)pid], lam_io[3 * (size_t)pid + 1], lam_io[3 * (size_t)pid + 2]};
     auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
-    const int sa = c * K;
-    const int sbnd = min(sa + K, n_steps);
-    const int slot = live ? t : 0;
-    float x[3] = {ckpt[((size_t)c * n + pid) * 3], ckpt[((size_t)c * n + pid) * 3 + 1],
-                  ckpt[((size_t)c * n + pid) * 3 + 2]};
-    for (int s = sa; s < sbnd; ++s) {                   // recompute the segment
-        if (live) {
-            float* o = seg + ((size_t)(s - sa) * n + slot) * 3;
-            o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
+    bool needs_xp = false;
+    for (int a = 0; a < 3; ++a) needs_xp = needs_xp || bc.kind[a] == JT_BC_REFLECTIVE || bc.kind[a] == JT_BC_CLAMP;
+    if (live) {
+        for (int s = s_hi - 1; s >= s_lo; --s) {
+            const float* xi = seg + ((size_t)(s - sa) * n + pid) * 3;
+            float p[3] = {xi[0], xi[1], xi[2]};
+            const Blend bl0 = tab[3 * s], bl1 = tab[3 * s + 1], bl2 = tab[3 * s + 2];
+            float k1[3], k2[3], k3[3], k4[3], x2[3], x3[3], x4[3], xp[3];
+            f(bl0, p, k1);
+            for (int a = 0; a < 3; ++a) x2[a] = p[a] + dth * k1[a];
+            f(bl1, x2, k2);
+            for (int a = 0; a < 3; ++a) x3[a] = p[a] + dth * k2[a];
+            f(bl1, x3, k3);
+            for (int a = 0; a < 3; ++a) x4[a] = p[a] + dt * k3[a];
+            if (needs_xp) {                            // the boundary's derivative depends on x'
+                f(bl2, x4, k4);
+                for (int a = 0; a < 3; ++a) xp[a] = p[a] + dt6 * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
+            } else {
+                xp[0] = xp[1] = xp[2] = 0.0f;
+            }
+            float a1[3], a2[3], a3[3], a4[3], l[3];
+            for (int a = 0; a < 3; ++a) {
+                lam[a] *= bc_deriv(bc.kind[a], xp[a], bc.lo[a], bc.hi[a]);
+                a4[a] = dt6 * lam[a];
+                a3[a] = 2.0f * dt6 * lam[a];
+                a2[a] = 2.0f * dt6 * lam[a];
+                a1[a] = dt6 * lam[a];
+            }
+            Stencil stc;
+            bool dep;
+            vjp_local<ST, ZH>(g, bl2, x4, a4, l, stc, dep);
+            if (dep) scatter(g, bl2, stc, a4, tl, tile, gU);
+            for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a3[a] += dt * l[a]; }
+            vjp_local<ST, ZH>(g, bl1, x3, a3, l, stc, dep);
+            if (dep) scatter(g, bl1, stc, a3, tl, tile, gU);
+            for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a2[a] += dth * l[a]; }
+            vjp_local<ST, ZH>(g, bl1, x2, a2, l, stc, dep);
+            if (dep) scatter(g, bl1, stc, a2, tl, tile, gU);
+            for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a1[a] += dth * l[a]; }
+            vjp_local<ST, ZH>(g, bl0, p, a1, l, stc, dep);
+            if (dep) scatter(g, bl0, stc, a1, tl, tile, gU);
+            for (int a = 0; a < 3; ++a) lam[a] += l[a];
         }
-        rk_step<JT_RK4>(f, tab, s, x, dt, dth, dt6);
-        apply_bc(bc, x);
+        lam_io[3 * (size_t)pid] = lam[0];
+        lam_io[3 * (size_t)pid + 1] = lam[1];
+        lam_io[3 * (size_t)pid + 2] = lam[2];
+        if (s_lo == 0 && gx0 != nullptr) {
+            gx0[3 * (size_t)pid] = lam[0];
+            gx0[3 * (size_t)pid + 1] = lam[1];
+            gx0[3 * (size_t)pid + 2] = lam[2];
+        }
     }
-    for (int s = sbnd - 1; s >= sa; --s) {              // reverse it
-        float p[3];
-        if (live) {
-            const float* xi = seg + ((size_t)(s - sa) * n + slot) * 3;
-            p[0] = xi[0]; p[1] = xi[1]; p[2] = xi[2];
-        } else {
-            p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
-        }
-        const Blend bl0 = tab[3 * s], bl1 = tab[3 * s + 1], bl2 = tab[3 * s + 2];
-        float k1[3], k2[3], k3[3], k4[3], x2[3], x3[3], x4[3], xp[3];
-        f(bl0, p, k1);
-        for (int a = 0; a < 3; ++a) x2[a] = p[a] + dth * k1[a];
-        f(bl1, x2, k2);
-        for (int a = 0; a < 3; ++a) x3[a] = p[a] + dth * k2[a];
-        f(bl1, x3, k3);
-        for (int a = 0; a < 3; ++a) x4[a] = p[a] + dt * k3[a];
-        f(bl2, x4, k4);
-        for (int a = 0; a < 3; ++a) xp[a] = p[a] + dt6 * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
-        float a1[3], a2[3], a3[3], a4[3], l[3];
-        for (int a = 0; a < 3; ++a) {
-            lam[a] *= bc_deriv(bc.kind[a], xp[a], bc.lo[a], bc.hi[a]);
-            a4[a] = dt6 * lam[a];
-            a3[a] = 2.0f * dt6 * lam[a];
-            a2[a] = 2.0f * dt6 * lam[a];
-            a1[a] = dt6 * lam[a];
-        }
-        Stencil stc;
-        bool dep;
-        vjp_local<ST, ZH>(g, bl2, x4, a4, l, stc, dep);
-        block_scatter(g, bl2, stc, a4, live && dep, gU, tile, sred);
-        for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a3[a] += dt * l[a]; }
-        vjp_local<ST, ZH>(g, bl1, x3, a3, l, stc, dep);
-        block_scatter(g, bl1, stc, a3, live && dep, gU, tile, sred);
-        for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a2[a] += dth * l[a]; }
-        vjp_local<ST, ZH>(g, bl1, x2, a2, l, stc, dep);
-        block_scatter(g, bl1, stc, a2, live && dep, gU, tile, sred);
-        for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a1[a] += dth * l[a]; }
-        vjp_local<ST, ZH>(g, bl0, p, a1, l, stc, dep);
-        block_scatter(g, bl0, stc, a1, live && dep, gU, tile, sred);
-        for (int a = 0; a < 3; ++a) lam[a] += l[a];
-    }
-    if (!live) return;
-    lam_io[3 * (size_t)pid] = lam[0];
-    lam_io[3 * (size_t)pid + 1] = lam[1];
-    lam_io[3 * (size_t)pid + 2] = lam[2];
-    if (c == 0 && gx0 != nullptr) {
-        gx0[3 * (size_t)pid] = lam[0];
-        gx0[3 * (size_t)pid + 1] = lam[1];
-        gx0[3 * (size_t)pid + 2] = lam[2];
+    __syncthreads();
+    const int tv = tl.vol * 3;
+    const size_t gsnap = (size_t)g.nx * g.ny * g.nz * 3;
+    for (int e = threadIdx.x; e < 2 * tv; e += blockDim.x) {        // flush the box
+        const float v = tile[e];
+        if (v == 0.0f) continue;
+        const int slot = e >= tv;
+        const int r = e - slot * tv;
+        const int node = r / 3, comp = r - node * 3;
+        const int k = node % tl.e[2] + tl.b[2];
+        const int t2 = node / tl.e[2];
+        const int j = t2 % tl.e[1] + tl.b[1];
+        const int i = t2 / tl.e[1] + tl.b[0];
+        atomicAdd(gU + (size_t)(tl.smin + slot) * gsnap + ((size_t)(i * g.ny + j) * g.nz + k) * 3 + comp, v);
     }
 }
 
-template <bool ZH>
-static void launch_adj(int st, dim3 gr, dim3 bl, hipStream_t s, const GridDev& g, const BcDev& bc,
-                       const float* ckpt, int n, const int* perm, const Blend* tab, int n_steps, int K, int c,
-                       float dt, float dth, float dt6, float* lam, float* seg, float* gU, float* gx0) {
-    switch (st) {
-    case ST_F32x3: hipLaunchKernelGGL((k_adjoint<ST_F32x3, ZH>), gr, bl, 0, s, g, bc, ckpt, n, perm, tab, n_steps, K, c, dt, dth, dt6, lam, seg, gU, gx0); break;
-    case ST_F32x4: hipLaunchKernelGGL((k_adjoint<ST_F32x4, ZH>), gr, bl, 0, s, g, bc, ckpt, n, perm, tab, n_steps, K, c, dt, dth, dt6, lam, seg, gU, gx0); break;
-    case ST_BF16x4: hipLaunchKernelGGL((k_adjoint<ST_BF16x4, ZH>), gr, bl, 0, s, g, bc, ckpt, n, perm, tab, n_steps, K, c, dt, dth, dt6, lam, seg, gU, gx0); break;
-    default: hipLaunchKernelGGL((k_adjoint<ST_F16x4, ZH>), gr, bl, 0, s, g, bc, ckpt, n, perm, tab, n_steps, K, c, dt, dth, dt6, lam, seg, gU, gx0); break;
-    }
-}
+#define JT_ADJ_DISPATCH(KERNEL, GRID, BLOCK, ...)                                                        \
+    do {                                                                                                  \
+        if (zh) {                                                                                         \
+            switch (st) {                                                                                 \
+            case ST_F32x3: hipLaunchKernelGGL((KERNEL<ST_F32x3, true>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+            case ST_F32x4: hipLaunchKernelGGL((KERNEL<ST_F32x4, true>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+            case ST_BF16x4: hipLaunchKernelGGL((KERNEL<ST_BF16x4, true>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<ST_F16x4, true>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+            }                                                                                             \
+        } else {                                                                                          \
+            switch (st) {                                                                                 \
+            case ST_F32x3: hipLaunchKernelGGL((KERNEL<ST_F32x3, false>), GRID, BLOCK, 0, s, __VA_ARGS__); break;  \
+            case ST_F32x4: hipLaunchKernelGGL((KERNEL<ST_F32x4, false>), GRID, BLOCK, 0, s, __VA_ARGS__); break;  \
+            case ST_BF16x4: hipLaunchKernelGGL((KERNEL<ST_BF16x4, false>), GRID, BLOCK, 0, s, __VA_ARGS__); break;\
+            default: hipLaunchKernelGGL((KERNEL<ST_F16x4, false>), GRID, BLOCK, 0, s, __VA_ARGS__); break;        \
+            }                                                                                             \
+        }                                                                                                 \
+    } while (0)
 
 // Morton key of each particle's cell (10 bits per axis) for the backward sort.
 __device__ __forceinline__ unsigned adj_spread3(unsigned v) {
@@ -364,19 +420,35 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     JT_HIP(hipGetLastError());
     const float dth = 0.5f * dt, dt6 = dt / 6.0f;
     const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
+    const int R = 8;                                   // re-sort interval inside a segment
     int maxn = g.nx > g.ny ? g.nx : g.ny;
     maxn = maxn > g.nz ? maxn : g.nz;
     int shift = 0;
     while ((maxn - 1) >> shift >= 1024) ++shift;
-    dim3 block(256), grid((unsigned)((n + 255) / 256));
+    dim3 b256(256), g256((unsigned)((n + 255) / 256));
+    dim3 badj(ADJ_BLOCK), gadj((unsigned)((n + ADJ_BLOCK - 1) / ADJ_BLOCK));
     const bool zh = g.nzs != g.nz;
     for (int c = n_seg - 1; c >= 0; --c) {
-        // order this segment's particles by the cell of their checkpoint
-        hipLaunchKernelGGL(k_adj_keys, grid, block, 0, s, g, ckpt + (size_t)c * n * 3, (int)n, shift, keys, idx);
+        const int sa = c * ckpt_every;
+        const int sb = sa + ckpt_every < n_steps ? sa + ckpt_every : n_steps;
+        const float* xc = ckpt + (size_t)c * n * 3;
+        hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, xc, (int)n, shift, keys, idx);
         JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, 30, s));
-        if (zh) launch_adj<true>(st, grid, block, s, g, d, ckpt, (int)n, perm, tab, n_steps, ckpt_every, c, dt, dth, dt6, lam, seg, grad_U, grad_x0);
-        else launch_adj<false>(st, grid, block, s, g, d, ckpt, (int)n, perm, tab, n_steps, ckpt_every, c, dt, dth, dt6, lam, seg, grad_U, grad_x0);
+        JT_ADJ_DISPATCH(k_adj_recompute, g256, b256, g, d, xc, (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6,
+                        seg);
         JT_HIP(hipGetLastError());
+        const int nsub = (sb - sa + R - 1) / R;
+        for (int j = nsub - 1; j >= 0; --j) {
+            const int s_lo = sa + j * R;
+            const int s_hi = s_lo + R < sb ? s_lo + R : sb;
+            // order particles by their cell at the start of this sub-segment
+            hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift,
+                               keys, idx);
+            JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, 30, s));
+            JT_ADJ_DISPATCH(k_adj_reverse, gadj, badj, g, d, (int)n, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth,
+                            dt6, (const float*)seg, lam, grad_U, grad_x0);
+            JT_HIP(hipGetLastError());
+        }
     }
     return JT_OK;
 }

@@ -4,13 +4,22 @@
 // the forward path of track_grid.hip: given dL/dx_T it returns dL/dx_0 and
 // dL/dU for the snapshots U, with positions checkpointed every K steps by the
 // forward run (its trajectory recorded every K steps) and recomputed in
-// between.  One thread owns one particle; its adjoint state stays in registers
-// across all segments of a launch.  Per step the forward stages are recomputed
-// with the forward kernel's exact arithmetic, then reversed:
+// between.  Per step the forward stages are recomputed with the forward
+// kernel's exact arithmetic, then reversed:
 //   x' = BC(x + dt/6 (k1 + 2k2 + 2k3 + k4)),  k_i = u(x_i, t_i)
 // using the trilinear Jacobian (corner weights' derivatives times the blended
-// corner values) and scattering  coef_snap * W_corner * abar  into dL/dU with
-// hardware float atomics (global_atomic_add_f32).
+// corner values) and adding  coef_snap * W_corner * abar  into dL/dU.
+//
+// The dL/dU scatter is reduced in three levels, because float atomics are the
+// cost that dominates (about 200 G lane-ops/s for ds_add_f32 on the whole
+// chip, about 21 G/s for scattered global_atomic_add_f32):
+//   1. registers: a thread sums its contributions for one cell (the 8 corner
+//      nodes) and one snapshot pair while its stage points stay in that cell,
+//      and flushes the 48 sums only when the cell changes;
+//   2. LDS: a block of cell-sorted particles flushes into a box of nodes
+//      (two snapshot slots) around its particles' start cells;
+//   3. HBM: the box is added to dL/dU with coalesced atomics at the end of
+//      the launch; flushes outside the box go straight to HBM.
 #include "grid_sampler.h"
 #include <hipcub/hipcub.hpp>
 
@@ -51,15 +60,17 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
     return 1.0f;
 }
 
-// ---- block-level accumulation of the dL/dU scatter ------------------------
-// The threads of a block hold cell-sorted particles, so over a few steps the
-// corners they scatter into stay in a small box of nodes.  A reverse launch
-// sums its whole sub-segment's contributions for the box (two snapshot
-// slots) in LDS with ds_add_f32 and flushes each touched entry to HBM once
-// with a float atomic; corners outside the box (or in another snapshot) go
-// straight to HBM.
+// ---- LDS box -----------------------------------------------------------------
+#ifndef TILE_NODES
 #define TILE_NODES 3072
+#endif
 #define ADJ_BLOCK 256
+#ifndef ADJ_RESORT
+#define ADJ_RESORT 8
+#endif
+#ifndef ADJ_MARGIN
+#define ADJ_MARGIN 1
+#endif
 
 __device__ __forceinline__ int wave_min_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -72,26 +83,28 @@ __device__ __forceinline__ int wave_max_i(int v) {
 
 struct Tile {
     int b[3], e[3];     // box origin (node indices) and extents
-    int smin;           // first snapshot slot
+    int smin;           // snapshot of slot 0 (slot 1 is smin + 1)
     int vol;            // e0*e1*e2 (0: no box)
 };
 
 struct Stencil {
     int I[2], J[2], K[2];
-    float W[8];
+    float wx[2], wy[2], wz[2];   // corner weight of (ci, cj, ck) = wx[ci] * wy[cj] * wz[ck]
 };
 
-// J^T a and the stencil of u at q (no scatter).
+// J^T a and the stencil of u at q.
 template <int ST, bool ZH>
 __device__ __forceinline__ void vjp_local(const GridDev& g, const Blend& bl, const float q[3], const float a[3],
                                           float lam[3], Stencil& st, bool& depends) {
-    float wx[2], wy[2], wz[2], dx[2], dy[2], dz[2];
+    float dx[2], dy[2], dz[2];
     bool valid = true;
-    axis_w(g, 0, q[0], g.nx, st.I[0], st.I[1], wx, dx, valid);
-    axis_w(g, 1, q[1], g.ny, st.J[0], st.J[1], wy, dy, valid);
-    axis_w(g, 2, q[2], g.nz, st.K[0], st.K[1], wz, dz, valid);
+    axis_w(g, 0, q[0], g.nx, st.I[0], st.I[1], st.wx, dx, valid);
+    axis_w(g, 1, q[1], g.ny, st.J[0], st.J[1], st.wy, dy, valid);
+    axis_w(g, 2, q[2], g.nz, st.K[0], st.K[1], st.wz, dz, valid);
     lam[0] = lam[1] = lam[2] = 0.0f;
     depends = !(!valid && g.bmode == JT_BOUNDS_ZERO);
+    if (!depends) return;
+    const float *wx = st.wx, *wy = st.wy, *wz = st.wz;
     const char* base = (const char*)g.data;
     const size_t sb = (size_t)g.snap_stride * ElemBytes<ST>::v;
     const void* b0 = base + (size_t)bl.i0 * sb;
@@ -100,54 +113,92 @@ __device__ __forceinline__ void vjp_local(const GridDev& g, const Blend& bl, con
 #pragma unroll
     for (int ci = 0; ci < 2; ++ci)
 #pragma unroll
-        for (int cj = 0; cj < 2; ++cj)
+        for (int cj = 0; cj < 2; ++cj) {
+            float v[2][3];
+            const int row = (st.I[ci] * g.ny + st.J[cj]) * g.nzs;
+            if constexpr (ZH) {
+                corner_pair<ST>(b0, b1, bl, oma, row + st.K[0], v[0], v[1]);
+            } else {
+                corner<ST>(b0, b1, bl, oma, row + st.K[0], v[0]);
+                corner<ST>(b0, b1, bl, oma, row + st.K[1], v[1]);
+            }
 #pragma unroll
             for (int ck = 0; ck < 2; ++ck) {
-                const int c = ci * 4 + cj * 2 + ck;
-                st.W[c] = depends ? wx[ci] * wy[cj] * wz[ck] : 0.0f;
-                if (!depends) continue;
-                float v[3];
-                corner<ST>(b0, b1, bl, oma, (st.I[ci] * g.ny + st.J[cj]) * g.nzs + st.K[ck], v);
-                const float s = a[0] * v[0] + a[1] * v[1] + a[2] * v[2];
+                const float s = a[0] * v[ck][0] + a[1] * v[ck][1] + a[2] * v[ck][2];
                 lam[0] += s * dx[ci] * wy[cj] * wz[ck];
                 lam[1] += s * wx[ci] * dy[cj] * wz[ck];
                 lam[2] += s * wx[ci] * wy[cj] * dz[ck];
             }
+        }
 }
 
-// Add coef_snap * W_c * a for the 8 corners: into the LDS tile when the
-// corner and snapshot are covered, else straight to HBM.
-__device__ __forceinline__ void scatter(const GridDev& g, const Blend& bl, const Stencil& st, const float a[3],
-                                        const Tile& tl, float* tile, float* __restrict__ gU) {
+// Level-1 accumulator: sums of c_h * W_q * a over the stage points of one
+// run (same stencil, same snapshot pair) of one thread.
+struct RunAcc {
+    int I[2], J[2], K[2];
+    int snap[2];
+    bool live;
+    float v[2][8][3];
+};
+
+__device__ __forceinline__ void run_reset(RunAcc& r, const Stencil& st, const Blend& bl) {
+    r.I[0] = st.I[0]; r.I[1] = st.I[1];
+    r.J[0] = st.J[0]; r.J[1] = st.J[1];
+    r.K[0] = st.K[0]; r.K[1] = st.K[1];
+    r.snap[0] = bl.i0; r.snap[1] = bl.i1;
+    r.live = true;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r.v[h][q][0] = r.v[h][q][1] = r.v[h][q][2] = 0.0f;
+}
+
+__device__ __forceinline__ void run_add(RunAcc& r, const Stencil& st, const Blend& bl, const float a[3]) {
     float c[2];
     blend_coefs(bl, c[0], c[1]);
-    const int snap[2] = {bl.i0, bl.i1};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float W = st.wx[q >> 2] * st.wy[(q >> 1) & 1] * st.wz[q & 1];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float f = c[h] * W;
+            r.v[h][q][0] += f * a[0];
+            r.v[h][q][1] += f * a[1];
+            r.v[h][q][2] += f * a[2];
+        }
+    }
+}
+
+// Add a run's sums into the LDS box (ds_add_f32) when its node and snapshot
+// are covered, else into dL/dU (global_atomic_add_f32).  The two atomics
+// carry different scopes so the compiler keeps them apart instead of merging
+// them into one flat atomic.  Zero sums are skipped.
+__device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, const Tile& tl, float* tile,
+                                          float* __restrict__ gU) {
     const size_t gsnap = (size_t)g.nx * g.ny * g.nz * 3;
     const int tv = tl.vol * 3;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int ci = q >> 2, cj = (q >> 1) & 1, ck = q & 1;
-        const int I = st.I[ci], J = st.J[cj], K = st.K[ck];
+        const int I = r.I[q >> 2], J = r.J[(q >> 1) & 1], K = r.K[q & 1];
         const int li = I - tl.b[0], lj = J - tl.b[1], lk = K - tl.b[2];
-        const bool in = tl.vol > 0 && (unsigned)li < (unsigned)tl.e[0] && (unsigned)lj < (unsigned)tl.e[1] &&
+        const bool in = (unsigned)li < (unsigned)tl.e[0] && (unsigned)lj < (unsigned)tl.e[1] &&
                         (unsigned)lk < (unsigned)tl.e[2];
-        const int loc = (li * tl.e[1] + lj) * tl.e[2] + lk;
-        const size_t node = ((size_t)(I * g.ny + J) * g.nz + K) * 3;
+        const int loc = ((li * tl.e[1] + lj) * tl.e[2] + lk) * 3;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (c[h] == 0.0f) continue;
-            const float f = c[h] * st.W[q];
-            const int slot = snap[h] - tl.smin;
+            const float* v = r.v[h][q];
+            if (v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f) continue;
+            const int slot = r.snap[h] - tl.smin;
             if (in && (slot == 0 || slot == 1)) {
-                float* tp = tile + slot * tv + loc * 3;
-                atomicAdd(tp, f * a[0]);
-                atomicAdd(tp + 1, f * a[1]);
-                atomicAdd(tp + 2, f * a[2]);
+                const int e = slot * tv + loc;
+                __hip_atomic_fetch_add(&tile[e], v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&tile[e + 1], v[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&tile[e + 2], v[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             } else {
-                float* gp = gU + (size_t)snap[h] * gsnap + node;
-                atomicAdd(gp, f * a[0]);
-                atomicAdd(gp + 1, f * a[1]);
-                atomicAdd(gp + 2, f * a[2]);
+                float* gp = gU + (size_t)r.snap[h] * gsnap + ((size_t)(I * g.ny + J) * g.nz + K) * 3;
+                __hip_atomic_fetch_add(gp, v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gp + 1, v[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gp + 2, v[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
@@ -155,19 +206,22 @@ __device__ __forceinline__ void scatter(const GridDev& g, const Blend& bl, const
 
 // Forward recompute of one checkpoint segment [sa, sb): thread t takes
 // particle perm[t] (cell-sorted) and stores the state before each step at
-// seg[(s - sa) * n + pid] (particle order, so any later ordering can read it).
+// seg[(s - sa) * n + t] (thread order: coalesced stores).
 template <int ST, bool ZH>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
 k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const int* __restrict__ perm,
                 const Blend* __restrict__ tab, int sa, int sb, float dt, float dth, float dt6,
                 float* __restrict__ seg) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = gridDim.x >> 3;                    // gridDim.x is a multiple of 8: consecutive
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);   // tiles share an XCD's L2
+    const int t = lb * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int pid = perm[t];
     float x[3] = {x0s[3 * (size_t)pid], x0s[3 * (size_t)pid + 1], x0s[3 * (size_t)pid + 2]};
     auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
+#pragma unroll 1
     for (int s = sa; s < sb; ++s) {
-        float* o = seg + ((size_t)(s - sa) * n + pid) * 3;
+        float* o = seg + ((size_t)(s - sa) * n + t) * 3;
         o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
         rk_step<JT_RK4>(f, tab, s, x, dt, dth, dt6);
         apply_bc(bc, x);
@@ -175,60 +229,70 @@ k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const
 }
 
 // Reverse steps s_hi-1 .. s_lo of the segment starting at sa.  Thread t takes
-// particle perm[t], sorted by its cell at step s_lo, so the block's corners
-// stay in a small box; its adjoint lives in lam_io[pid] between launches.
+// segment slot slot = order[t] (slots sorted by their cell at step s_lo, so a
+// block's particles start in a small region), particle pid = seg_pid[slot];
+// its adjoint lives in lam_io[pid] between launches.
 template <int ST, bool ZH>
 __global__ void __launch_bounds__(ADJ_BLOCK)
-k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ perm, const Blend* __restrict__ tab,
-              int sa, int s_lo, int s_hi, float dt, float dth, float dt6, const float* __restrict__ seg,
-              float* __restrict__ lam_io, float* __restrict__ gU, float* __restrict__ gx0) {
+k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const int* __restrict__ seg_pid,
+              const Blend* __restrict__ tab, int sa, int s_lo, int s_hi, float dt, float dth, float dt6,
+              const float* __restrict__ seg, float* __restrict__ lam_io, float* __restrict__ gU,
+              float* __restrict__ gx0) {
     __shared__ float tile[2 * TILE_NODES * 3];
-    __shared__ int sred[(ADJ_BLOCK / 64) * 6];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = t < n;                           // every thread stays: the block synchronises
-    const int pid = live ? perm[t] : perm[0];
-    // box: cells of this thread's states over the sub-segment, widened by the
-    // stencil reach (one cell below, two above), reduced over the block
-    int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
-    if (live) {
-        const int nn[3] = {g.nx, g.ny, g.nz};
-        for (int s = s_lo; s < s_hi; ++s) {
-            const float* xi = seg + ((size_t)(s - sa) * n + pid) * 3;
-            for (int d = 0; d < 3; ++d) {
-                float f = floorf((xi[d] - g.o[d]) / g.d[d]);
-                int c = (f != f) ? 0 : (int)fminf(fmaxf(f, -4.0f), (float)(nn[d] + 4));
-                lo[d] = min(lo[d], c - 1);
-                hi[d] = max(hi[d], c + 2);
-            }
+    __shared__ int sred[(ADJ_BLOCK / 64) * 6 + 3];
+    const int per = gridDim.x >> 3;                    // gridDim.x is a multiple of 8: consecutive
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);   // tiles share an XCD's L2
+    if (lb * ADJ_BLOCK >= n) return;                   // whole block past the end (block-uniform)
+    const int t = lb * ADJ_BLOCK + threadIdx.x;
+    const bool live = t < n;                           // every other thread stays: the block synchronises
+    const int slot = live ? order[t] : order[0];
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    // box: the block's start cells widened by the stencil (one node above)
+    // and ADJ_MARGIN cells of motion each way
+    int c0[3];
+    {
+        const float* xi = seg + ((size_t)(s_lo - sa) * n + slot) * 3;
+        for (int d = 0; d < 3; ++d) {
+            float f = floorf((xi[d] - g.o[d]) / g.d[d]);
+            c0[d] = (f != f) ? 0 : (int)fminf(fmaxf(f, 0.0f), (float)(nn[d] - 1));
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int d = 0; d < 3; ++d) {
-        int l = wave_min_i(lo[d]), h = wave_max_i(hi[d]);
+        int l = wave_min_i(live ? c0[d] : 0x7fffffff), h = wave_max_i(live ? c0[d] : -1);
         if (lane == 0) { sred[wv * 6 + d] = l; sred[wv * 6 + 3 + d] = h; }
     }
+    const int mid = (min(n - 1, lb * ADJ_BLOCK + ADJ_BLOCK / 2) & (ADJ_BLOCK - 1));
+    if ((int)threadIdx.x == mid)
+        for (int d = 0; d < 3; ++d) sred[(ADJ_BLOCK / 64) * 6 + d] = c0[d];
     for (int e = threadIdx.x; e < 2 * TILE_NODES * 3; e += blockDim.x) tile[e] = 0.0f;
     __syncthreads();
     Tile tl;
     {
-        const int nn[3] = {g.nx, g.ny, g.nz};
         long long vol = 1;
         int ext[3];
         for (int d = 0; d < 3; ++d) {
             int l = sred[d], h = sred[3 + d];
             for (int w = 1; w < ADJ_BLOCK / 64; ++w) { l = min(l, sred[w * 6 + d]); h = max(h, sred[w * 6 + 3 + d]); }
-            l = max(l, 0);
-            h = min(h, nn[d] - 1);
+            l = max(l - ADJ_MARGIN, 0);
+            h = min(h + 1 + ADJ_MARGIN, nn[d] - 1);
             tl.b[d] = l;
             ext[d] = h - l + 1;
             vol *= (ext[d] > 0 ? ext[d] : 0);
         }
-        if (vol > TILE_NODES) {                        // keep a central sub-box that fits
-            int cap = 1;
+        if (vol > TILE_NODES) {                        // a cube of the largest fitting side around the
+            int cap = 1;                               // block's middle particle
             while ((cap + 1) * (cap + 1) * (cap + 1) <= TILE_NODES) ++cap;
             vol = 1;
             for (int d = 0; d < 3; ++d) {
-                if (ext[d] > cap) { tl.b[d] += (ext[d] - cap) / 2; ext[d] = cap; }
+                if (ext[d] > cap) {
+                    const int m = sred[(ADJ_BLOCK / 64) * 6 + d];
+                    int b0 = m - cap / 2;
+                    b0 = max(b0, tl.b[d]);
+                    b0 = min(b0, tl.b[d] + ext[d] - cap);
+                    tl.b[d] = b0;
+                    ext[d] = cap;
+                }
                 vol *= ext[d];
             }
         }
@@ -238,52 +302,77 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ perm, const Bl
         for (int s = s_lo; s < s_hi; ++s)
             for (int k = 0; k < 3; ++k) smin = min(smin, min(tab[3 * s + k].i0, tab[3 * s + k].i1));
         tl.smin = smin;
+        if (tl.vol == 0) { tl.e[0] = tl.e[1] = tl.e[2] = 0; }
     }
-    float lam[3] = {lam_io[3 * (size_t)pid], lam_io[3 * (size_t)pid + 1], lam_io[3 * (size_t)pid + 2]};
+    const int pid = live ? seg_pid[slot] : 0;
+    float lam[3] = {0.0f, 0.0f, 0.0f};
+    if (live) { lam[0] = lam_io[3 * (size_t)pid]; lam[1] = lam_io[3 * (size_t)pid + 1]; lam[2] = lam_io[3 * (size_t)pid + 2]; }
     auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
     bool needs_xp = false;
     for (int a = 0; a < 3; ++a) needs_xp = needs_xp || bc.kind[a] == JT_BC_REFLECTIVE || bc.kind[a] == JT_BC_CLAMP;
+    RunAcc run;
+    run.live = false;
     if (live) {
         for (int s = s_hi - 1; s >= s_lo; --s) {
-            const float* xi = seg + ((size_t)(s - sa) * n + pid) * 3;
-            float p[3] = {xi[0], xi[1], xi[2]};
+            const float* xi = seg + ((size_t)(s - sa) * n + slot) * 3;
+            const float p[3] = {xi[0], xi[1], xi[2]};
             const Blend bl0 = tab[3 * s], bl1 = tab[3 * s + 1], bl2 = tab[3 * s + 2];
-            float k1[3], k2[3], k3[3], k4[3], x2[3], x3[3], x4[3], xp[3];
-            f(bl0, p, k1);
-            for (int a = 0; a < 3; ++a) x2[a] = p[a] + dth * k1[a];
-            f(bl1, x2, k2);
-            for (int a = 0; a < 3; ++a) x3[a] = p[a] + dth * k2[a];
-            f(bl1, x3, k3);
-            for (int a = 0; a < 3; ++a) x4[a] = p[a] + dt * k3[a];
-            if (needs_xp) {                            // the boundary's derivative depends on x'
-                f(bl2, x4, k4);
-                for (int a = 0; a < 3; ++a) xp[a] = p[a] + dt6 * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
-            } else {
-                xp[0] = xp[1] = xp[2] = 0.0f;
+            // forward stages with the forward kernel's arithmetic (rk_step):
+            // leaves the stage positions x2, x3, x4 and, when the boundary's
+            // derivative needs it, the stepped position xp
+            float q[3] = {p[0], p[1], p[2]}, acc[3], x2[3], x3[3], x4[3], xp[3] = {0.0f, 0.0f, 0.0f};
+            const int nst = needs_xp ? 4 : 3;
+#pragma unroll 1
+            for (int r = 0; r < nst; ++r) {
+                const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
+                float k[3];
+                f(b, q, k);
+                if (r == 3) {
+                    for (int c = 0; c < 3; ++c) xp[c] = p[c] + dt6 * (acc[c] + k[c]);
+                } else {
+                    const float h = r == 2 ? dt : dth;
+                    for (int c = 0; c < 3; ++c) {
+                        acc[c] = r == 0 ? k[c] : acc[c] + 2.0f * k[c];
+                        q[c] = p[c] + h * k[c];
+                        x2[c] = x3[c]; x3[c] = x4[c]; x4[c] = q[c];
+                    }
+                }
             }
-            float a1[3], a2[3], a3[3], a4[3], l[3];
-            for (int a = 0; a < 3; ++a) {
-                lam[a] *= bc_deriv(bc.kind[a], xp[a], bc.lo[a], bc.hi[a]);
-                a4[a] = dt6 * lam[a];
-                a3[a] = 2.0f * dt6 * lam[a];
-                a2[a] = 2.0f * dt6 * lam[a];
-                a1[a] = dt6 * lam[a];
+            float L[3];                                // dL/dx' pulled back through the boundary
+            for (int c = 0; c < 3; ++c) {
+                lam[c] *= bc_deriv(bc.kind[c], xp[c], bc.lo[c], bc.hi[c]);
+                L[c] = lam[c];
             }
-            Stencil stc;
-            bool dep;
-            vjp_local<ST, ZH>(g, bl2, x4, a4, l, stc, dep);
-            if (dep) scatter(g, bl2, stc, a4, tl, tile, gU);
-            for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a3[a] += dt * l[a]; }
-            vjp_local<ST, ZH>(g, bl1, x3, a3, l, stc, dep);
-            if (dep) scatter(g, bl1, stc, a3, tl, tile, gU);
-            for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a2[a] += dth * l[a]; }
-            vjp_local<ST, ZH>(g, bl1, x2, a2, l, stc, dep);
-            if (dep) scatter(g, bl1, stc, a2, tl, tile, gU);
-            for (int a = 0; a < 3; ++a) { lam[a] += l[a]; a1[a] += dth * l[a]; }
-            vjp_local<ST, ZH>(g, bl0, p, a1, l, stc, dep);
-            if (dep) scatter(g, bl0, stc, a1, tl, tile, gU);
-            for (int a = 0; a < 3; ++a) lam[a] += l[a];
+            // reverse stages 4, 3, 2, 1: a_r = w_r L + h_r l_(r+1), lam += l_r
+            float lp[3] = {0.0f, 0.0f, 0.0f}, cur[3] = {x4[0], x4[1], x4[2]};
+#pragma unroll 1
+            for (int r = 3; r >= 0; --r) {
+                const float w = (r == 0 || r == 3) ? dt6 : 2.0f * dt6;
+                const float h = r == 3 ? 0.0f : (r == 2 ? dt : dth);
+                const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
+                float av[3], l[3];
+                for (int c = 0; c < 3; ++c) av[c] = w * L[c] + h * lp[c];
+                Stencil stc;
+                bool dep;
+                vjp_local<ST, ZH>(g, b, cur, av, l, stc, dep);
+                if (dep) {
+                    const bool same = run.live && stc.I[0] == run.I[0] && stc.J[0] == run.J[0] &&
+                                      stc.K[0] == run.K[0] && stc.I[1] == run.I[1] && stc.J[1] == run.J[1] &&
+                                      stc.K[1] == run.K[1] && b.i0 == run.snap[0] && b.i1 == run.snap[1];
+                    if (!same) {
+                        if (run.live) run_flush(g, run, tl, tile, gU);
+                        run_reset(run, stc, b);
+                    }
+                    run_add(run, stc, b, av);
+                }
+                for (int c = 0; c < 3; ++c) {
+                    lam[c] += l[c];
+                    lp[c] = l[c];
+                    cur[c] = x3[c]; x3[c] = x2[c]; x2[c] = p[c];
+                }
+            }
         }
+        if (run.live) run_flush(g, run, tl, tile, gU);
         lam_io[3 * (size_t)pid] = lam[0];
         lam_io[3 * (size_t)pid + 1] = lam[1];
         lam_io[3 * (size_t)pid + 2] = lam[2];
@@ -299,14 +388,14 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ perm, const Bl
     for (int e = threadIdx.x; e < 2 * tv; e += blockDim.x) {        // flush the box
         const float v = tile[e];
         if (v == 0.0f) continue;
-        const int slot = e >= tv;
-        const int r = e - slot * tv;
+        const int sl = e >= tv;
+        const int r = e - sl * tv;
         const int node = r / 3, comp = r - node * 3;
         const int k = node % tl.e[2] + tl.b[2];
         const int t2 = node / tl.e[2];
         const int j = t2 % tl.e[1] + tl.b[1];
         const int i = t2 / tl.e[1] + tl.b[0];
-        atomicAdd(gU + (size_t)(tl.smin + slot) * gsnap + ((size_t)(i * g.ny + j) * g.nz + k) * 3 + comp, v);
+        atomicAdd(gU + (size_t)(tl.smin + sl) * gsnap + ((size_t)(i * g.ny + j) * g.nz + k) * 3 + comp, v);
     }
 }
 
@@ -365,7 +454,7 @@ extern "C" size_t jt_track_grid_adjoint_workspace_size(int64_t n, int32_t n_step
     a.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
     a.take<float>(3 * nn);                                          // adjoint state
     a.take<float>(3 * nn * (ckpt_every > 0 ? ckpt_every : 1));      // segment states
-    a.take<unsigned>(nn); a.take<unsigned>(nn); a.take<int>(nn); a.take<int>(nn);   // cell sort
+    a.take<unsigned>(nn); a.take<unsigned>(nn); a.take<int>(nn); a.take<int>(nn); a.take<int>(nn);   // cell sorts
     size_t temp = 0;
     hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
                                        (const int*)nullptr, (int*)nullptr, (int)nn, 0, 30);
@@ -402,12 +491,13 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     unsigned* keys = arena.take<unsigned>(n);
     unsigned* skeys = arena.take<unsigned>(n);
     int* idx = arena.take<int>(n);
-    int* perm = arena.take<int>(n);
+    int* perm = arena.take<int>(n);                   // segment slot -> particle
+    int* order = arena.take<int>(n);                  // reverse thread -> segment slot
     size_t temp = 0;
     hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
                                        (const int*)nullptr, (int*)nullptr, (int)n, 0, 30);
     char* tmp = arena.take<char>(temp);
-    if (!tab || !lam || !seg || !tmp || !arena.ok())
+    if (!tab || !lam || !seg || !order || !tmp || !arena.ok())
         return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu",
                     jt_track_grid_adjoint_workspace_size(n, n_steps, ckpt_every), ws_bytes);
     JT_HIP(hipMemcpyAsync(lam, grad_xT, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -420,13 +510,14 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     JT_HIP(hipGetLastError());
     const float dth = 0.5f * dt, dt6 = dt / 6.0f;
     const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
-    const int R = 8;                                   // re-sort interval inside a segment
+    const int R = ADJ_RESORT;                          // re-sort interval inside a segment
     int maxn = g.nx > g.ny ? g.nx : g.ny;
     maxn = maxn > g.nz ? maxn : g.nz;
     int shift = 0;
     while ((maxn - 1) >> shift >= 1024) ++shift;
     dim3 b256(256), g256((unsigned)((n + 255) / 256));
-    dim3 badj(ADJ_BLOCK), gadj((unsigned)((n + ADJ_BLOCK - 1) / ADJ_BLOCK));
+    dim3 g256x((unsigned)((((n + 255) / 256) + 7) / 8 * 8));   // multiple of 8 for the XCD remap
+    dim3 badj(ADJ_BLOCK), gadj((unsigned)((((n + ADJ_BLOCK - 1) / ADJ_BLOCK) + 7) / 8 * 8));
     const bool zh = g.nzs != g.nz;
     for (int c = n_seg - 1; c >= 0; --c) {
         const int sa = c * ckpt_every;
@@ -434,19 +525,19 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
         const float* xc = ckpt + (size_t)c * n * 3;
         hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, xc, (int)n, shift, keys, idx);
         JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, 30, s));
-        JT_ADJ_DISPATCH(k_adj_recompute, g256, b256, g, d, xc, (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6,
+        JT_ADJ_DISPATCH(k_adj_recompute, g256x, b256, g, d, xc, (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6,
                         seg);
         JT_HIP(hipGetLastError());
         const int nsub = (sb - sa + R - 1) / R;
         for (int j = nsub - 1; j >= 0; --j) {
             const int s_lo = sa + j * R;
             const int s_hi = s_lo + R < sb ? s_lo + R : sb;
-            // order particles by their cell at the start of this sub-segment
+            // order the segment slots by their cell at the start of this sub-segment
             hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift,
                                keys, idx);
-            JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, 30, s));
-            JT_ADJ_DISPATCH(k_adj_reverse, gadj, badj, g, d, (int)n, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth,
-                            dt6, (const float*)seg, lam, grad_U, grad_x0);
+            JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, order, (int)n, 0, 30, s));
+            JT_ADJ_DISPATCH(k_adj_reverse, gadj, badj, g, d, (int)n, (const int*)order, (const int*)perm, tab, sa,
+                            s_lo, s_hi, dt, dth, dt6, (const float*)seg, lam, grad_U, grad_x0);
             JT_HIP(hipGetLastError());
         }
     }

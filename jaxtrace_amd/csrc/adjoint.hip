@@ -10,18 +10,21 @@
 // using the trilinear Jacobian (corner weights' derivatives times the blended
 // corner values) and adding  coef_snap * W_corner * abar  into dL/dU.
 //
-// The dL/dU scatter is reduced in three levels, because float atomics are the
-// cost that dominates (about 200 G lane-ops/s for ds_add_f32 on the whole
-// chip, about 21 G/s for scattered global_atomic_add_f32):
+// The dL/dU scatter is reduced in three levels, because atomics are the cost
+// that dominates (measured on the whole chip: about 4 T lane-ops/s for LDS
+// ds_add_f64 but only 0.2 T/s for ds_add_f32, and about 21 G/s for scattered
+// global_atomic_add_f32):
 //   1. registers: a thread sums its contributions for one cell (the 8 corner
 //      nodes) and one snapshot pair while its stage points stay in that cell,
 //      and flushes the 48 sums only when the cell changes;
-//   2. LDS: a block of cell-sorted particles flushes into a box of nodes
-//      (two snapshot slots) around its particles' start cells;
-//   3. HBM: the box is added to dL/dU with coalesced atomics at the end of
-//      the launch; flushes outside the box go straight to HBM.
+//   2. LDS: a block of Hilbert-sorted particles flushes, in float64, into a
+//      box of nodes (two snapshot slots) around its particles' start cells;
+//   3. HBM: the box is added to dL/dU with coalesced float atomics at the end
+//      of the launch; flushes outside the box go straight to HBM.
 #include "grid_sampler.h"
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <vector>
 
 namespace jt {
 
@@ -34,18 +37,6 @@ __device__ __forceinline__ void blend_coefs(const Blend& b, float& c0, float& c1
     case BL_RIGHT:  c0 = -b.a; c1 = 1.0f + b.a; break;
     default:        c0 = 0.0f; c1 = 0.0f; break;
     }
-}
-
-// Per-axis weights (1-w, w) and their x-derivatives.
-__device__ __forceinline__ void axis_w(const GridDev& g, int ax, float x, int n, int& i0, int& i1,
-                                       float w[2], float dw[2], bool& valid) {
-    float raw = (x - g.o[ax]) / g.d[ax];
-    float ww;
-    axis_coord(g, ax, x, n, i0, i1, ww, valid);
-    float gd = 1.0f / g.d[ax];
-    if (g.bmode != JT_BOUNDS_PERIODIC && !(raw >= 0.0f && raw <= g.cmax[ax])) gd = 0.0f;   // clipped
-    w[0] = 1.0f - ww; w[1] = ww;
-    dw[0] = -gd; dw[1] = gd;
 }
 
 // d BC(x)/dx per axis (periodic 1, reflective +-1, clamp 1 inside / 0 outside).
@@ -62,14 +53,14 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
 
 // ---- LDS box -----------------------------------------------------------------
 #ifndef TILE_NODES
-#define TILE_NODES 3072
+#define TILE_NODES 1660            // 2 slots x 3 float64 per node + a spare: < 80 KiB, two blocks per CU
 #endif
 #define ADJ_BLOCK 256
 #ifndef ADJ_RESORT
 #define ADJ_RESORT 8
 #endif
 #ifndef ADJ_MARGIN
-#define ADJ_MARGIN 1
+#define ADJ_MARGIN 0
 #endif
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -92,44 +83,56 @@ struct Stencil {
     float wx[2], wy[2], wz[2];   // corner weight of (ci, cj, ck) = wx[ci] * wy[cj] * wz[ck]
 };
 
-// J^T a and the stencil of u at q.
+// u(q) with the forward sampler's exact arithmetic, plus its Jacobian
+// J[c*3 + ax] = du_c/dx_ax from the same eight blended corner values: the
+// trilinear weights' derivatives are +-1/d per axis (0 where a non-periodic
+// axis is clipped).  depends = false where u is the constant 0 of the 'zero'
+// bounds mode.
 template <int ST, bool ZH>
-__device__ __forceinline__ void vjp_local(const GridDev& g, const Blend& bl, const float q[3], const float a[3],
-                                          float lam[3], Stencil& st, bool& depends) {
-    float dx[2], dy[2], dz[2];
-    bool valid = true;
-    axis_w(g, 0, q[0], g.nx, st.I[0], st.I[1], st.wx, dx, valid);
-    axis_w(g, 1, q[1], g.ny, st.J[0], st.J[1], st.wy, dy, valid);
-    axis_w(g, 2, q[2], g.nz, st.K[0], st.K[1], st.wz, dz, valid);
-    lam[0] = lam[1] = lam[2] = 0.0f;
-    depends = !(!valid && g.bmode == JT_BOUNDS_ZERO);
-    if (!depends) return;
-    const float *wx = st.wx, *wy = st.wy, *wz = st.wz;
-    const char* base = (const char*)g.data;
-    const size_t sb = (size_t)g.snap_stride * ElemBytes<ST>::v;
-    const void* b0 = base + (size_t)bl.i0 * sb;
-    const void* b1 = base + (size_t)bl.i1 * sb;
-    const float oma = 1.0f - bl.a;
+__device__ __forceinline__ void sample_jac(const GridDev& g, const Blend& bl, const float q[3], float u[3],
+                                           float J[9], bool& depends) {
+    Corners C;
+    gather_corners<ST, ZH>(g, bl, q[0], q[1], q[2], C);
+    lerp_corners(g, C, u);
+    depends = !(!C.valid && g.bmode == JT_BOUNDS_ZERO);
+    float gd[3];
 #pragma unroll
-    for (int ci = 0; ci < 2; ++ci)
+    for (int ax = 0; ax < 3; ++ax) {
+        const float raw = (q[ax] - g.o[ax]) / g.d[ax];
+        gd[ax] = 1.0f / g.d[ax];
+        if (g.bmode != JT_BOUNDS_PERIODIC && !(raw >= 0.0f && raw <= g.cmax[ax])) gd[ax] = 0.0f;
+        if (!depends) gd[ax] = 0.0f;
+    }
+    const float wx = C.wx, wy = C.wy, wz = C.wz;
+    const float ax_ = 1.0f - wx, ay = 1.0f - wy, az = 1.0f - wz;
 #pragma unroll
-        for (int cj = 0; cj < 2; ++cj) {
-            float v[2][3];
-            const int row = (st.I[ci] * g.ny + st.J[cj]) * g.nzs;
-            if constexpr (ZH) {
-                corner_pair<ST>(b0, b1, bl, oma, row + st.K[0], v[0], v[1]);
-            } else {
-                corner<ST>(b0, b1, bl, oma, row + st.K[0], v[0]);
-                corner<ST>(b0, b1, bl, oma, row + st.K[1], v[1]);
+    for (int c = 0; c < 3; ++c) {
+        float Z[2][2], dZ[2][2];
+#pragma unroll
+        for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+            for (int cj = 0; cj < 2; ++cj) {
+                const float z0 = C.v[ci * 4 + cj * 2][c], z1 = C.v[ci * 4 + cj * 2 + 1][c];
+                Z[ci][cj] = fmaf(z1, wz, z0 * az);
+                dZ[ci][cj] = z1 - z0;
             }
+        float Y[2], dYy[2], dYz[2];
 #pragma unroll
-            for (int ck = 0; ck < 2; ++ck) {
-                const float s = a[0] * v[ck][0] + a[1] * v[ck][1] + a[2] * v[ck][2];
-                lam[0] += s * dx[ci] * wy[cj] * wz[ck];
-                lam[1] += s * wx[ci] * dy[cj] * wz[ck];
-                lam[2] += s * wx[ci] * wy[cj] * dz[ck];
-            }
+        for (int ci = 0; ci < 2; ++ci) {
+            Y[ci] = fmaf(Z[ci][1], wy, Z[ci][0] * ay);
+            dYy[ci] = Z[ci][1] - Z[ci][0];
+            dYz[ci] = fmaf(dZ[ci][1], wy, dZ[ci][0] * ay);
         }
+        J[c * 3 + 0] = (Y[1] - Y[0]) * gd[0];
+        J[c * 3 + 1] = fmaf(dYy[1], wx, dYy[0] * ax_) * gd[1];
+        J[c * 3 + 2] = fmaf(dYz[1], wx, dYz[0] * ax_) * gd[2];
+    }
+}
+
+// l = J^T a
+__device__ __forceinline__ void jac_t(const float J[9], const float a[3], float l[3]) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) l[ax] = fmaf(J[6 + ax], a[2], fmaf(J[3 + ax], a[1], J[ax] * a[0]));
 }
 
 // Level-1 accumulator: sums of c_h * W_q * a over the stage points of one
@@ -162,39 +165,51 @@ __device__ __forceinline__ void run_add(RunAcc& r, const Stencil& st, const Blen
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const float f = c[h] * W;
-            r.v[h][q][0] += f * a[0];
-            r.v[h][q][1] += f * a[1];
-            r.v[h][q][2] += f * a[2];
+            r.v[h][q][0] = fmaf(f, a[0], r.v[h][q][0]);
+            r.v[h][q][1] = fmaf(f, a[1], r.v[h][q][1]);
+            r.v[h][q][2] = fmaf(f, a[2], r.v[h][q][2]);
         }
     }
 }
 
-// Add a run's sums into the LDS box (ds_add_f32) when its node and snapshot
-// are covered, else into dL/dU (global_atomic_add_f32).  The two atomics
-// carry different scopes so the compiler keeps them apart instead of merging
-// them into one flat atomic.  Zero sums are skipped.
-__device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, const Tile& tl, float* tile,
+// Add a run's sums into the LDS box (ds_add_f64) where its node and snapshot
+// are covered, else into dL/dU (global_atomic_add_f32).  The LDS adds are
+// unconditional (a missing corner adds into a spare node past the box, never
+// read back) so the wave takes no branch per corner; only lanes with a miss
+// run the second, global loop.  The two atomics carry different scopes so the
+// compiler keeps them apart instead of merging them into one flat atomic.
+__device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, const Tile& tl, double* tile,
                                           float* __restrict__ gU) {
-    const size_t gsnap = (size_t)g.nx * g.ny * g.nz * 3;
     const int tv = tl.vol * 3;
+    const int spare = 2 * TILE_NODES * 3;
+    unsigned miss = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int I = r.I[q >> 2], J = r.J[(q >> 1) & 1], K = r.K[q & 1];
-        const int li = I - tl.b[0], lj = J - tl.b[1], lk = K - tl.b[2];
+        const int li = r.I[q >> 2] - tl.b[0], lj = r.J[(q >> 1) & 1] - tl.b[1], lk = r.K[q & 1] - tl.b[2];
         const bool in = (unsigned)li < (unsigned)tl.e[0] && (unsigned)lj < (unsigned)tl.e[1] &&
                         (unsigned)lk < (unsigned)tl.e[2];
         const int loc = ((li * tl.e[1] + lj) * tl.e[2] + lk) * 3;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const float* v = r.v[h][q];
-            if (v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f) continue;
             const int slot = r.snap[h] - tl.smin;
-            if (in && (slot == 0 || slot == 1)) {
-                const int e = slot * tv + loc;
-                __hip_atomic_fetch_add(&tile[e], v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(&tile[e + 1], v[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(&tile[e + 2], v[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
+            const bool ok = in && (slot == 0 || slot == 1);
+            const int e = ok ? slot * tv + loc : spare;
+            const float* v = r.v[h][q];
+            __hip_atomic_fetch_add(&tile[e], (double)v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&tile[e + 1], (double)v[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&tile[e + 2], (double)v[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!ok && (v[0] != 0.0f || v[1] != 0.0f || v[2] != 0.0f)) miss |= 1u << (q * 2 + h);
+        }
+    }
+    if (miss) {
+        const size_t gsnap = (size_t)g.nx * g.ny * g.nz * 3;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int I = r.I[q >> 2], J = r.J[(q >> 1) & 1], K = r.K[q & 1];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!((miss >> (q * 2 + h)) & 1u)) continue;
+                const float* v = r.v[h][q];
                 float* gp = gU + (size_t)r.snap[h] * gsnap + ((size_t)(I * g.ny + J) * g.nz + K) * 3;
                 __hip_atomic_fetch_add(gp, v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_fetch_add(gp + 1, v[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -238,7 +253,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
               const Blend* __restrict__ tab, int sa, int s_lo, int s_hi, float dt, float dth, float dt6,
               const float* __restrict__ seg, float* __restrict__ lam_io, float* __restrict__ gU,
               float* __restrict__ gx0) {
-    __shared__ float tile[2 * TILE_NODES * 3];
+    __shared__ double tile[2 * TILE_NODES * 3 + 3];   // + the spare node
     __shared__ int sred[(ADJ_BLOCK / 64) * 6 + 3];
     const int per = gridDim.x >> 3;                    // gridDim.x is a multiple of 8: consecutive
     const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);   // tiles share an XCD's L2
@@ -265,7 +280,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     const int mid = (min(n - 1, lb * ADJ_BLOCK + ADJ_BLOCK / 2) & (ADJ_BLOCK - 1));
     if ((int)threadIdx.x == mid)
         for (int d = 0; d < 3; ++d) sred[(ADJ_BLOCK / 64) * 6 + d] = c0[d];
-    for (int e = threadIdx.x; e < 2 * TILE_NODES * 3; e += blockDim.x) tile[e] = 0.0f;
+    for (int e = threadIdx.x; e < 2 * TILE_NODES * 3 + 3; e += blockDim.x) tile[e] = 0.0;
     __syncthreads();
     Tile tl;
     {
@@ -307,9 +322,6 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     const int pid = live ? seg_pid[slot] : 0;
     float lam[3] = {0.0f, 0.0f, 0.0f};
     if (live) { lam[0] = lam_io[3 * (size_t)pid]; lam[1] = lam_io[3 * (size_t)pid + 1]; lam[2] = lam_io[3 * (size_t)pid + 2]; }
-    auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
-    bool needs_xp = false;
-    for (int a = 0; a < 3; ++a) needs_xp = needs_xp || bc.kind[a] == JT_BC_REFLECTIVE || bc.kind[a] == JT_BC_CLAMP;
     RunAcc run;
     run.live = false;
     if (live) {
@@ -317,45 +329,70 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
             const float* xi = seg + ((size_t)(s - sa) * n + slot) * 3;
             const float p[3] = {xi[0], xi[1], xi[2]};
             const Blend bl0 = tab[3 * s], bl1 = tab[3 * s + 1], bl2 = tab[3 * s + 2];
-            // forward stages with the forward kernel's arithmetic (rk_step):
-            // leaves the stage positions x2, x3, x4 and, when the boundary's
-            // derivative needs it, the stepped position xp
-            float q[3] = {p[0], p[1], p[2]}, acc[3], x2[3], x3[3], x4[3], xp[3] = {0.0f, 0.0f, 0.0f};
-            const int nst = needs_xp ? 4 : 3;
+            // forward stages with the forward kernel's arithmetic (rk_step),
+            // keeping the Jacobians of stages 1-3 (Ja, Jb, Jc) and the stage
+            // points x2, x3, x4; stage 4's Jacobian is used at once
+            float q[3] = {p[0], p[1], p[2]}, acc[3], x2[3], x3[3], x4[3];
+            float Ja[9], Jb[9], Jc[9];
+            bool da = true, db = true, dc = true, d4 = true;
+            float L[3], A3[3], l[3];
 #pragma unroll 1
-            for (int r = 0; r < nst; ++r) {
+            for (int r = 0; r < 4; ++r) {
                 const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
-                float k[3];
-                f(b, q, k);
+                float k[3], Jn[9];
+                bool dn;
+                sample_jac<ST, ZH>(g, b, q, k, Jn, dn);
                 if (r == 3) {
-                    for (int c = 0; c < 3; ++c) xp[c] = p[c] + dt6 * (acc[c] + k[c]);
+                    float xp[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        xp[c] = p[c] + dt6 * (acc[c] + k[c]);
+                        L[c] = lam[c] * bc_deriv(bc.kind[c], xp[c], bc.lo[c], bc.hi[c]);
+                        A3[c] = dt6 * L[c];
+                    }
+                    jac_t(Jn, A3, l);
+                    d4 = dn;
                 } else {
                     const float h = r == 2 ? dt : dth;
+#pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         acc[c] = r == 0 ? k[c] : acc[c] + 2.0f * k[c];
                         q[c] = p[c] + h * k[c];
                         x2[c] = x3[c]; x3[c] = x4[c]; x4[c] = q[c];
                     }
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) { Ja[e] = Jb[e]; Jb[e] = Jc[e]; Jc[e] = Jn[e]; }
+                    da = db; db = dc; dc = dn;
                 }
             }
-            float L[3];                                // dL/dx' pulled back through the boundary
-            for (int c = 0; c < 3; ++c) {
-                lam[c] *= bc_deriv(bc.kind[c], xp[c], bc.lo[c], bc.hi[c]);
-                L[c] = lam[c];
-            }
-            // reverse stages 4, 3, 2, 1: a_r = w_r L + h_r l_(r+1), lam += l_r
-            float lp[3] = {0.0f, 0.0f, 0.0f}, cur[3] = {x4[0], x4[1], x4[2]};
+            // reverse: a_r = w_r L + h_r l_(r+1), l_r = J_r^T a_r, lam = L + sum l_r
+            float A[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { lam[c] = L[c] + l[c]; A[2][c] = fmaf(dt, l[c], 2.0f * dt6 * L[c]); }
+            jac_t(Jc, A[2], l);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { lam[c] += l[c]; A[1][c] = fmaf(dth, l[c], 2.0f * dt6 * L[c]); }
+            jac_t(Jb, A[1], l);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { lam[c] += l[c]; A[0][c] = fmaf(dth, l[c], dt6 * L[c]); }
+            jac_t(Ja, A[0], l);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) lam[c] += l[c];
+            // dL/dU: stage points 4, 3, 2, 1 through the run accumulator
+            float cur[3] = {x4[0], x4[1], x4[2]}, av[3] = {A3[0], A3[1], A3[2]};
+            float y2[3] = {x3[0], x3[1], x3[2]}, y1[3] = {x2[0], x2[1], x2[2]};
+            float b2[3] = {A[2][0], A[2][1], A[2][2]}, b1[3] = {A[1][0], A[1][1], A[1][2]};
+            bool dcur = d4, d2 = dc, d1 = db;
 #pragma unroll 1
             for (int r = 3; r >= 0; --r) {
-                const float w = (r == 0 || r == 3) ? dt6 : 2.0f * dt6;
-                const float h = r == 3 ? 0.0f : (r == 2 ? dt : dth);
-                const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
-                float av[3], l[3];
-                for (int c = 0; c < 3; ++c) av[c] = w * L[c] + h * lp[c];
-                Stencil stc;
-                bool dep;
-                vjp_local<ST, ZH>(g, b, cur, av, l, stc, dep);
-                if (dep) {
+                if (dcur) {
+                    const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
+                    Stencil stc;
+                    bool valid = true;
+                    float w;
+                    axis_coord(g, 0, cur[0], g.nx, stc.I[0], stc.I[1], w, valid); stc.wx[0] = 1.0f - w; stc.wx[1] = w;
+                    axis_coord(g, 1, cur[1], g.ny, stc.J[0], stc.J[1], w, valid); stc.wy[0] = 1.0f - w; stc.wy[1] = w;
+                    axis_coord(g, 2, cur[2], g.nz, stc.K[0], stc.K[1], w, valid); stc.wz[0] = 1.0f - w; stc.wz[1] = w;
                     const bool same = run.live && stc.I[0] == run.I[0] && stc.J[0] == run.J[0] &&
                                       stc.K[0] == run.K[0] && stc.I[1] == run.I[1] && stc.J[1] == run.J[1] &&
                                       stc.K[1] == run.K[1] && b.i0 == run.snap[0] && b.i1 == run.snap[1];
@@ -365,11 +402,12 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
                     }
                     run_add(run, stc, b, av);
                 }
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    lam[c] += l[c];
-                    lp[c] = l[c];
-                    cur[c] = x3[c]; x3[c] = x2[c]; x2[c] = p[c];
+                    cur[c] = y2[c]; y2[c] = y1[c]; y1[c] = p[c];
+                    av[c] = b2[c]; b2[c] = b1[c]; b1[c] = A[0][c];
                 }
+                dcur = d2; d2 = d1; d1 = da;
             }
         }
         if (run.live) run_flush(g, run, tl, tile, gU);
@@ -386,7 +424,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     const int tv = tl.vol * 3;
     const size_t gsnap = (size_t)g.nx * g.ny * g.nz * 3;
     for (int e = threadIdx.x; e < 2 * tv; e += blockDim.x) {        // flush the box
-        const float v = tile[e];
+        const float v = (float)tile[e];
         if (v == 0.0f) continue;
         const int sl = e >= tv;
         const int r = e - sl * tv;
@@ -418,7 +456,13 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
         }                                                                                                 \
     } while (0)
 
-// Morton key of each particle's cell (10 bits per axis) for the backward sort.
+// Hilbert index of each particle's cell for the backward sort, by Skilling's
+// transpose algorithm (J. Skilling, "Programming the Hilbert curve", AIP Conf.
+// Proc. 707, 381 (2004)): undo the excess work per bit plane, Gray-encode, and
+// interleave the transposed coordinates (x most significant).  Consecutive
+// Hilbert cells are face neighbours, so a block of consecutive particles spans
+// a compact box even where it crosses between aligned sub-cubes (a Z-order
+// range can jump across two or three axes there).
 __device__ __forceinline__ unsigned adj_spread3(unsigned v) {
     v &= 0x3ffu;
     v = (v | (v << 16)) & 0x030000ffu;
@@ -428,8 +472,33 @@ __device__ __forceinline__ unsigned adj_spread3(unsigned v) {
     return v;
 }
 
+__device__ __forceinline__ unsigned hilbert3(unsigned x0, unsigned x1, unsigned x2, int bits) {
+    unsigned X[3] = {x0, x1, x2};
+    const unsigned M = 1u << (bits - 1);
+    for (unsigned Q = M; Q > 1; Q >>= 1) {
+        const unsigned P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;                             // invert
+            } else {
+                const unsigned t = (X[0] ^ X[i]) & P;  // exchange
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];                                      // Gray encode
+    X[2] ^= X[1];
+    unsigned t = 0;
+    for (unsigned Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t; X[1] ^= t; X[2] ^= t;
+    return (adj_spread3(X[0]) << 2) | (adj_spread3(X[1]) << 1) | adj_spread3(X[2]);
+}
+
 __global__ void __launch_bounds__(256)
-k_adj_keys(GridDev g, const float* __restrict__ xs, int n, int shift, unsigned* __restrict__ keys,
+k_adj_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int bits, unsigned* __restrict__ keys,
            int* __restrict__ idx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -440,7 +509,7 @@ k_adj_keys(GridDev g, const float* __restrict__ xs, int n, int shift, unsigned* 
         int ci = (f != f) ? 0 : (int)fminf(fmaxf(floorf(f), 0.0f), (float)(nn[a] - 1));
         c[a] = (unsigned)ci >> shift;
     }
-    keys[i] = (adj_spread3(c[0]) << 2) | (adj_spread3(c[1]) << 1) | adj_spread3(c[2]);
+    keys[i] = hilbert3(c[0], c[1], c[2], bits);
     idx[i] = i;
 }
 
@@ -510,11 +579,19 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     JT_HIP(hipGetLastError());
     const float dth = 0.5f * dt, dt6 = dt / 6.0f;
     const int n_seg = (n_steps + ckpt_every - 1) / ckpt_every;
-    const int R = ADJ_RESORT;                          // re-sort interval inside a segment
+    const int R = ADJ_RESORT;                          // longest sub-segment (re-sort interval)
     int maxn = g.nx > g.ny ? g.nx : g.ny;
     maxn = maxn > g.nz ? maxn : g.nz;
     int shift = 0;
     while ((maxn - 1) >> shift >= 1024) ++shift;
+    int bits = 1;                                      // bits per axis of the shifted cell index
+    while (((maxn - 1) >> shift) >> bits) ++bits;
+    const int end_bit = 3 * bits;
+    // the blend table on the host, to cut sub-segments where the snapshot
+    // pair changes (the LDS box holds two snapshot slots)
+    std::vector<Blend> htab((size_t)3 * n_steps);
+    JT_HIP(hipMemcpyAsync(htab.data(), tab, htab.size() * sizeof(Blend), hipMemcpyDeviceToHost, s));
+    JT_HIP(hipStreamSynchronize(s));
     dim3 b256(256), g256((unsigned)((n + 255) / 256));
     dim3 g256x((unsigned)((((n + 255) / 256) + 7) / 8 * 8));   // multiple of 8 for the XCD remap
     dim3 badj(ADJ_BLOCK), gadj((unsigned)((((n + ADJ_BLOCK - 1) / ADJ_BLOCK) + 7) / 8 * 8));
@@ -523,22 +600,37 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
         const int sa = c * ckpt_every;
         const int sb = sa + ckpt_every < n_steps ? sa + ckpt_every : n_steps;
         const float* xc = ckpt + (size_t)c * n * 3;
-        hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, xc, (int)n, shift, keys, idx);
-        JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, 30, s));
+        hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, xc, (int)n, shift, bits, keys, idx);
+        JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, end_bit, s));
         JT_ADJ_DISPATCH(k_adj_recompute, g256x, b256, g, d, xc, (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6,
                         seg);
         JT_HIP(hipGetLastError());
-        const int nsub = (sb - sa + R - 1) / R;
-        for (int j = nsub - 1; j >= 0; --j) {
-            const int s_lo = sa + j * R;
-            const int s_hi = s_lo + R < sb ? s_lo + R : sb;
+        // sub-segments of at most R steps, cut where a third snapshot would enter
+        int s_hi = sb;
+        while (s_hi > sa) {
+            int s_lo = s_hi - 1;
+            int u0 = htab[3 * (size_t)s_lo].i0, u1 = u0;
+            auto fits = [&](int st) {
+                int a0 = u0, a1 = u1;
+                for (int k = 0; k < 3; ++k) {
+                    const Blend& b = htab[3 * (size_t)st + k];
+                    a0 = std::min(a0, std::min(b.i0, b.i1));
+                    a1 = std::max(a1, std::max(b.i0, b.i1));
+                }
+                if (a1 - a0 > 1) return false;
+                u0 = a0; u1 = a1;
+                return true;
+            };
+            fits(s_lo);
+            while (s_lo > sa && s_hi - s_lo < R && fits(s_lo - 1)) --s_lo;
             // order the segment slots by their cell at the start of this sub-segment
             hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift,
-                               keys, idx);
-            JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, order, (int)n, 0, 30, s));
+                               bits, keys, idx);
+            JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, order, (int)n, 0, end_bit, s));
             JT_ADJ_DISPATCH(k_adj_reverse, gadj, badj, g, d, (int)n, (const int*)order, (const int*)perm, tab, sa,
                             s_lo, s_hi, dt, dth, dt6, (const float*)seg, lam, grad_U, grad_x0);
             JT_HIP(hipGetLastError());
+            s_hi = s_lo;
         }
     }
     return JT_OK;

@@ -226,17 +226,23 @@ __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, in
     i1 = min(max(jj, 0), n - 1);
 }
 
-// Blended trilinear sample u(x, t) (structured.py:167-250 after the blend).
+// The eight blended corner values around a point and its trilinear weights.
+struct Corners {
+    float v[8][3];      // corner (ci, cj, ck) at index ci*4 + cj*2 + ck
+    float wx, wy, wz;
+    bool valid;
+};
+
+// Gather half of the trilinear sample (structured.py:176-232 after the blend).
 // ZH: z-halo layout, the (k0, k0+1) corners are one contiguous pair.
 template <int ST, bool ZH = false>
-__device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
-                                       float px, float py, float pz, float out[3]) {
+__device__ __forceinline__ void gather_corners(const GridDev& g, const Blend& bl,
+                                               float px, float py, float pz, Corners& C) {
     int i0, i1, j0, j1, k0, k1;
-    float wx, wy, wz;
-    bool valid = true;
-    axis_coord(g, 0, px, g.nx, i0, i1, wx, valid);
-    axis_coord(g, 1, py, g.ny, j0, j1, wy, valid);
-    axis_coord(g, 2, pz, g.nz, k0, k1, wz, valid);
+    C.valid = true;
+    axis_coord(g, 0, px, g.nx, i0, i1, C.wx, C.valid);
+    axis_coord(g, 1, py, g.ny, j0, j1, C.wy, C.valid);
+    axis_coord(g, 2, pz, g.nz, k0, k1, C.wz, C.valid);
     const char* base = (const char*)g.data;
     const size_t sb = (size_t)g.snap_stride * ElemBytes<ST>::v;
     const void* b0 = base + (size_t)bl.i0 * sb;
@@ -245,37 +251,50 @@ __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
     const int r0 = i0 * g.ny, r1 = i1 * g.ny;
     const int n00 = (r0 + j0) * g.nzs, n10 = (r1 + j0) * g.nzs;
     const int n01 = (r0 + j1) * g.nzs, n11 = (r1 + j1) * g.nzs;
-    float v000[3], v100[3], v010[3], v110[3], v001[3], v101[3], v011[3], v111[3];
     if constexpr (ZH) {
-        corner_pair<ST>(b0, b1, bl, oma, n00 + k0, v000, v001);
-        corner_pair<ST>(b0, b1, bl, oma, n10 + k0, v100, v101);
-        corner_pair<ST>(b0, b1, bl, oma, n01 + k0, v010, v011);
-        corner_pair<ST>(b0, b1, bl, oma, n11 + k0, v110, v111);
+        corner_pair<ST>(b0, b1, bl, oma, n00 + k0, C.v[0], C.v[1]);
+        corner_pair<ST>(b0, b1, bl, oma, n10 + k0, C.v[4], C.v[5]);
+        corner_pair<ST>(b0, b1, bl, oma, n01 + k0, C.v[2], C.v[3]);
+        corner_pair<ST>(b0, b1, bl, oma, n11 + k0, C.v[6], C.v[7]);
     } else {
-        corner<ST>(b0, b1, bl, oma, n00 + k0, v000);
-        corner<ST>(b0, b1, bl, oma, n10 + k0, v100);
-        corner<ST>(b0, b1, bl, oma, n01 + k0, v010);
-        corner<ST>(b0, b1, bl, oma, n11 + k0, v110);
-        corner<ST>(b0, b1, bl, oma, n00 + k1, v001);
-        corner<ST>(b0, b1, bl, oma, n10 + k1, v101);
-        corner<ST>(b0, b1, bl, oma, n01 + k1, v011);
-        corner<ST>(b0, b1, bl, oma, n11 + k1, v111);
+        corner<ST>(b0, b1, bl, oma, n00 + k0, C.v[0]);
+        corner<ST>(b0, b1, bl, oma, n10 + k0, C.v[4]);
+        corner<ST>(b0, b1, bl, oma, n01 + k0, C.v[2]);
+        corner<ST>(b0, b1, bl, oma, n11 + k0, C.v[6]);
+        corner<ST>(b0, b1, bl, oma, n00 + k1, C.v[1]);
+        corner<ST>(b0, b1, bl, oma, n10 + k1, C.v[5]);
+        corner<ST>(b0, b1, bl, oma, n01 + k1, C.v[3]);
+        corner<ST>(b0, b1, bl, oma, n11 + k1, C.v[7]);
     }
+}
+
+// Interpolation half: lerp along x, then y, then z (structured.py:234-247).
+__device__ __forceinline__ void lerp_corners(const GridDev& g, const Corners& C, float out[3]) {
+    const float wx = C.wx, wy = C.wy, wz = C.wz;
     const float ax = 1.0f - wx, ay = 1.0f - wy, az = 1.0f - wz;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float c00 = v000[c] * ax + v100[c] * wx;       // :234-237
-        float c10 = v010[c] * ax + v110[c] * wx;
-        float c01 = v001[c] * ax + v101[c] * wx;
-        float c11 = v011[c] * ax + v111[c] * wx;
+        float c00 = C.v[0][c] * ax + C.v[4][c] * wx;   // :234-237
+        float c10 = C.v[2][c] * ax + C.v[6][c] * wx;
+        float c01 = C.v[1][c] * ax + C.v[5][c] * wx;
+        float c11 = C.v[3][c] * ax + C.v[7][c] * wx;
         float c0 = c00 * ay + c10 * wy;                 // :239-240
         float c1 = c01 * ay + c11 * wy;
         out[c] = c0 * az + c1 * wz;                     // :242
     }
-    if (!valid) {                                       // :244-247
+    if (!C.valid) {                                     // :244-247
         if (g.bmode == JT_BOUNDS_ZERO) { out[0] = out[1] = out[2] = 0.0f; }
         else if (g.bmode == JT_BOUNDS_NAN) { out[0] = out[1] = out[2] = __int_as_float(0x7fc00000); }
     }
+}
+
+// Blended trilinear sample u(x, t) (structured.py:167-250 after the blend).
+template <int ST, bool ZH = false>
+__device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
+                                       float px, float py, float pz, float out[3]) {
+    Corners C;
+    gather_corners<ST, ZH>(g, bl, px, py, pz, C);
+    lerp_corners(g, C, out);
 }
 
 // ---- boundary epilogue (boundary.py) ----------------------------------------

@@ -233,8 +233,89 @@ struct Corners {
     bool valid;
 };
 
-// Gather half of the trilinear sample (structured.py:176-232 after the blend).
+// The eight blended corners from node arrays b0/b1 (the two snapshots) with
+// y-row stride ny_ and z-row stride nzs_: the snapshots themselves, or a tile
+// of them staged in LDS (local indices and the tile's strides).
 // ZH: z-halo layout, the (k0, k0+1) corners are one contiguous pair.
+// Blend n node values of the two snapshots with the same per-value arithmetic
+// as corner(): (1-a) V0 + a V1 ('interp'), V0 + a (V1 - V0) / V1 + a (V1 - V0)
+// (linear extrapolation left / right of the series).  COPY, ZERO and NAN are
+// handled by the caller before V1 is loaded.
+template <int N>
+__device__ __forceinline__ void blend_n(const Blend& bl, const float (*A)[3], const float (*B)[3],
+                                        float (*V)[3]) {
+    const float al = bl.a, oma = 1.0f - bl.a;
+    if (bl.mode == BL_INTERP) {
+#pragma unroll
+        for (int q = 0; q < N; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) V[q][c] = oma * A[q][c] + al * B[q][c];
+    } else if (bl.mode == BL_LEFT) {
+#pragma unroll
+        for (int q = 0; q < N; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) V[q][c] = A[q][c] + al * (B[q][c] - A[q][c]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < N; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) V[q][c] = B[q][c] + al * (B[q][c] - A[q][c]);
+    }
+}
+
+// The eight stored corner values of one snapshot; corner (ci, cj, ck) at
+// index ci*4 + cj*2 + ck.  ZH: the (k0, k0+1) corners are one contiguous pair.
+template <int ST, bool ZH>
+__device__ __forceinline__ void load_corners(const void* base, const int row[4], int k0, int k1, float (*A)[3]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if constexpr (ZH) {
+            float a[6];
+            load_pair<ST>(base, row[r] + k0, a);
+            A[2 * r][0] = a[0]; A[2 * r][1] = a[1]; A[2 * r][2] = a[2];
+            A[2 * r + 1][0] = a[3]; A[2 * r + 1][1] = a[4]; A[2 * r + 1][2] = a[5];
+        } else {
+            load_node<ST>(base, row[r] + k0, A[2 * r][0], A[2 * r][1], A[2 * r][2]);
+            load_node<ST>(base, row[r] + k1, A[2 * r + 1][0], A[2 * r + 1][1], A[2 * r + 1][2]);
+        }
+    }
+}
+
+// The eight blended corners from node arrays b0/b1 (the two snapshots) with
+// y-row stride ny_ and z-row stride nzs_: the snapshots themselves, or a tile
+// of them staged in LDS (local indices and the tile's strides).  The blend
+// mode is dispatched once, so all corner loads of a snapshot issue together.
+template <int ST, bool ZH>
+__device__ __forceinline__ void gather_from(const void* b0, const void* b1, const Blend& bl, int i0, int i1,
+                                            int j0, int j1, int k0, int k1, int ny_, int nzs_, Corners& C) {
+    if (bl.mode == BL_ZERO || bl.mode == BL_NAN) {
+        const float z = bl.mode == BL_ZERO ? 0.0f : __int_as_float(0x7fc00000);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) C.v[q][0] = C.v[q][1] = C.v[q][2] = z;
+        return;
+    }
+    const int r0 = i0 * ny_, r1 = i1 * ny_;
+    // rows in corner order (ci, cj): (0,0) (0,1) (1,0) (1,1)
+    const int row[4] = {(r0 + j0) * nzs_, (r0 + j1) * nzs_, (r1 + j0) * nzs_, (r1 + j1) * nzs_};
+    // both snapshots' loads issue before any wait ('copy' reads V0 twice)
+    float A[8][3], B[8][3];
+    load_corners<ST, ZH>(b0, row, k0, k1, A);
+    load_corners<ST, ZH>(bl.mode == BL_COPY ? b0 : b1, row, k0, k1, B);
+    __asm__ volatile("" ::: "memory");                 // keep the loads ahead of the mode branch
+    if (bl.mode == BL_COPY) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { C.v[q][0] = A[q][0]; C.v[q][1] = A[q][1]; C.v[q][2] = A[q][2]; }
+        return;
+    }
+    blend_n<8>(bl, A, B, C.v);
+}
+
+template <int ST>
+__device__ __forceinline__ const void* snap_base(const GridDev& g, int snap) {
+    return (const char*)g.data + (size_t)snap * (size_t)g.snap_stride * ElemBytes<ST>::v;
+}
+
+// Gather half of the trilinear sample (structured.py:176-232 after the blend).
 template <int ST, bool ZH = false>
 __device__ __forceinline__ void gather_corners(const GridDev& g, const Blend& bl,
                                                float px, float py, float pz, Corners& C) {
@@ -243,29 +324,8 @@ __device__ __forceinline__ void gather_corners(const GridDev& g, const Blend& bl
     axis_coord(g, 0, px, g.nx, i0, i1, C.wx, C.valid);
     axis_coord(g, 1, py, g.ny, j0, j1, C.wy, C.valid);
     axis_coord(g, 2, pz, g.nz, k0, k1, C.wz, C.valid);
-    const char* base = (const char*)g.data;
-    const size_t sb = (size_t)g.snap_stride * ElemBytes<ST>::v;
-    const void* b0 = base + (size_t)bl.i0 * sb;
-    const void* b1 = base + (size_t)bl.i1 * sb;
-    const float oma = 1.0f - bl.a;
-    const int r0 = i0 * g.ny, r1 = i1 * g.ny;
-    const int n00 = (r0 + j0) * g.nzs, n10 = (r1 + j0) * g.nzs;
-    const int n01 = (r0 + j1) * g.nzs, n11 = (r1 + j1) * g.nzs;
-    if constexpr (ZH) {
-        corner_pair<ST>(b0, b1, bl, oma, n00 + k0, C.v[0], C.v[1]);
-        corner_pair<ST>(b0, b1, bl, oma, n10 + k0, C.v[4], C.v[5]);
-        corner_pair<ST>(b0, b1, bl, oma, n01 + k0, C.v[2], C.v[3]);
-        corner_pair<ST>(b0, b1, bl, oma, n11 + k0, C.v[6], C.v[7]);
-    } else {
-        corner<ST>(b0, b1, bl, oma, n00 + k0, C.v[0]);
-        corner<ST>(b0, b1, bl, oma, n10 + k0, C.v[4]);
-        corner<ST>(b0, b1, bl, oma, n01 + k0, C.v[2]);
-        corner<ST>(b0, b1, bl, oma, n11 + k0, C.v[6]);
-        corner<ST>(b0, b1, bl, oma, n00 + k1, C.v[1]);
-        corner<ST>(b0, b1, bl, oma, n10 + k1, C.v[5]);
-        corner<ST>(b0, b1, bl, oma, n01 + k1, C.v[3]);
-        corner<ST>(b0, b1, bl, oma, n11 + k1, C.v[7]);
-    }
+    gather_from<ST, ZH>(snap_base<ST>(g, bl.i0), snap_base<ST>(g, bl.i1), bl, i0, i1, j0, j1, k0, k1, g.ny,
+                        g.nzs, C);
 }
 
 // Interpolation half: lerp along x, then y, then z (structured.py:234-247).

@@ -40,7 +40,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--particles", type=int, default=N_PARTICLES)
     ap.add_argument("--steps-per-launch", type=int, default=50)
     ap.add_argument("--sort-every", type=int, default=50)
@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known HBM traffic) for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -210,6 +211,158 @@ def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32):
     return out
 
 
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9      # non-packed FP32 VALU: 256 CUs x 4 SIMD16 x 2.4 GHz
+
+
+def _events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_octree(torch, jt, W, L, lib, dev, steps=20):
+    """Unstructured FEM workload: the unit box split into 128^3 hexes x 6 tets
+    (interior nodes jittered by up to 0.2 h, seed 4: 12,582,912 tets on 129^3
+    nodes), 8 snapshots of the Poiseuille-plus-vortex field, 2^22 uniform
+    particles (seed 5), RK4 dt = 1e-3 with the continuous inlet at x = 0
+    (64 x 64 grid replacement), particles Hilbert-sorted every 10 steps.
+    Timed: `steps` steps after a warm-up run.  Roofline: the point-location
+    byte model 24 + 4 [L 29 + c 28 + a 64 + 96] B per particle-step, with L,
+    c, a the mean levels descended, leaf candidates scanned and box hits
+    counted by jt_octree_locate_stats at the initial positions."""
+    pts, conn = W.kuhn_mesh(128, 0.2, 4)
+    times = (np.arange(8) / 14.0).astype(np.float32)
+    data = W.poiseuille_vortex(pts, times)
+    t0 = time.perf_counter()
+    f = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    bc = jt.continuous_inlet_boundary_factory([[0, 0, 0], [1, 1, 1]], "x", "positive", "grid",
+                                              {"x": 64, "y": 64, "z": 64})
+    n = 1 << 22
+    x0 = torch.from_numpy(W.uniform_particles(n, 5)).to(dev)
+    tr = jt.create_tracker("rk4", f, bc, return_device=True, record_final_only=True)
+    span = (0.0, steps * 1e-3)
+    tr.track_particles(x0, span, steps + 1, dt=1e-3)
+    e0, e1 = _events(torch)
+    torch.cuda.synchronize()
+    e0.record()
+    tr.track_particles(x0, span, steps + 1, dt=1e-3)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    rate = n * steps / (ms * 1e-3)
+    stats = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    L.check(lib.jt_octree_locate_stats(C.byref(f.descriptor()), L.ptr(x0), n, C.c_float(0.0), L.ptr(stats),
+                                       L.stream_handle()), "jt_octree_locate_stats")
+    Lm, cm, am = (float(v) for v in stats.double().mean(0).cpu())
+    bpps = 24 + 4 * (Lm * 29 + cm * 28 + am * 64 + 96)
+    achieved = bpps * rate / 1e9
+    return {"value": rate, "unit": "particle-steps/s", "ms": ms, "steps": steps, "particles": n,
+            "octree_build_s": build_s,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_particle_step": bpps,
+                         "levels": Lm, "candidates_scanned": cm, "box_hits": am},
+            "workload": "128^3 x 6 jittered tets, 8 snapshots, 2^22 particles, RK4 dt=1e-3, continuous inlet"}
+
+
+def bench_density(torch, jt, W, L, lib, dev, kde_res=128, sample=1 << 16):
+    """Density workload: 2^24 particles from the 64-component Gaussian mixture
+    (seed 6).
+    SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
+    hash grid of cell h with 64 slots per cell, 512 neighbours, x32 keys (the
+    reference as shipped); timed: the fused query + kernel-sum launch
+    (jt_sph_eval) and the hash-grid build.  Roofline: 27 x 8 + C_q x 16 + 4 B
+    per query, C_q (candidates read) from jt_hashgrid_query_stats on a
+    sample of queries.
+    KDE: Scott bandwidth, evaluate_3d grid of kde_res^3 points, cut-off 5.3 h
+    (each dropped term < 1e-6 of the kernel's peak); timed: the
+    jt_kde_grid_cutoff launch.  Rate: pairs within the cut-off per second
+    (counted by jt_hashgrid_query_stats on a sample), against a VALU bound of
+    16 lane-op slots per pair (12 VALU + exp at 4)."""
+    F = np.float32
+    P = W.gaussian_mixture(1 << 24, seed=6)
+    Pd = torch.from_numpy(P).to(dev)
+    h = 2.0 / 256
+    a = np.linspace(0.0, 1.0, 256, dtype=F)
+    X, Y, Z = np.meshgrid(a, a, a, indexing="xy")
+    Qd = torch.from_numpy(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(F)).to(dev)
+    del X, Y, Z
+    M = int(Qd.shape[0])
+    out = {}
+    e0, e1 = _events(torch)
+    torch.cuda.synchronize()
+    e0.record()
+    est = jt.SPHDensityEstimator(Pd, dimensions="3d", smoothing_length=h, kernel_type="cubic_spline",
+                                 per_cell_cap=64, max_neighbors=512, normalize=True, key_mode="x32")
+    e1.record()
+    torch.cuda.synchronize()
+    build_ms = e0.elapsed_time(e1)
+    rho = torch.empty(M, dtype=torch.float32, device=dev)
+    radius = F(2.0 * est._mean_h)
+
+    def sph():
+        L.check(lib.jt_sph_eval(C.byref(est._hash._desc), L.ptr(Qd), M, C.c_float(radius), 64, 512,
+                                L.ptr(est._m_dev), L.ptr(est._h_dev), C.c_float(F(est._mean_h)),
+                                L.JT_SPH["cubic_spline"], L.ptr(rho), L.stream_handle()), "jt_sph_eval")
+    sph()
+    torch.cuda.synchronize()
+    e0.record()
+    sph()
+    e1.record()
+    torch.cuda.synchronize()
+    sph_ms = e0.elapsed_time(e1)
+    idx = torch.from_numpy(np.random.default_rng(12).choice(M, sample, replace=False)).to(dev)
+    Qs = Qd[idx].contiguous()
+    st = torch.empty((sample, 2), dtype=torch.int32, device=dev)
+    L.check(lib.jt_hashgrid_query_stats(C.byref(est._hash._desc), L.ptr(Qs), sample, C.c_float(radius), 64,
+                                        L.ptr(st), L.stream_handle()), "jt_hashgrid_query_stats")
+    cq = float(st[:, 0].double().mean())
+    bpq = 27 * 8 + cq * 16 + 4
+    ach = bpq * M / (sph_ms * 1e-3) / 1e9
+    out["sph"] = {"value": M / (sph_ms * 1e-3), "unit": "queries/s", "ms": sph_ms, "queries": M,
+                  "hashgrid_build_ms": build_ms,
+                  "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": ach / HBM_PEAK_GBS, "bytes_per_query": bpq, "mean_candidates": cq,
+                               "mean_within_radius": float(st[:, 1].double().mean())},
+                  "workload": "2^24 Gaussian-mixture particles, 256^3 query grid, cubic spline h=2/256, x32 keys"}
+    del est, rho
+    kde = jt.KDEEstimator(Pd, bandwidth_rule="scott", normalize=True, resolution=kde_res,
+                          bounds=((0, 1), (0, 1), (0, 1)), cutoff_sigmas=5.3)
+    x, y, z = kde._axes_3d()
+    R = F(5.3 * kde.h)
+    hg = jt.HashGridNeighbors(Pd, float(R), 1, "x64", dev)
+    gx, gy, gz = (torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (x, y, z))
+    D = torch.empty((len(y), len(x), len(z)), dtype=torch.float32, device=dev)
+
+    def kd():
+        L.check(lib.jt_kde_grid_cutoff(C.byref(hg._desc), kde.N, L.ptr(gx), len(x), L.ptr(gy), len(y), L.ptr(gz),
+                                       len(z), C.c_float(F(kde.h)), C.c_float(R), L.ptr(D), L.stream_handle()),
+                "jt_kde_grid_cutoff")
+    kd()
+    torch.cuda.synchronize()
+    e0.record()
+    kd()
+    e1.record()
+    torch.cuda.synchronize()
+    kde_ms = e0.elapsed_time(e1)
+    Xk, Yk, Zk = np.meshgrid(x, y, z, indexing="xy")
+    Qk = np.stack([Xk.ravel(), Yk.ravel(), Zk.ravel()], 1).astype(F)
+    mk = len(Qk)
+    sel = np.random.default_rng(13).choice(mk, min(sample, mk), replace=False)
+    Qks = torch.from_numpy(Qk[sel]).to(dev)
+    stk = torch.empty((len(sel), 2), dtype=torch.int32, device=dev)
+    L.check(lib.jt_hashgrid_query_stats(C.byref(hg._desc), L.ptr(Qks), len(sel), C.c_float(R), 1 << 30,
+                                        L.ptr(stk), L.stream_handle()), "jt_hashgrid_query_stats")
+    pairs = float(stk[:, 1].double().mean()) * mk
+    rate = pairs / (kde_ms * 1e-3)
+    peak = VALU_LANE_OPS / 16.0
+    out["kde"] = {"value": rate, "unit": "pairs/s", "ms": kde_ms, "grid": [kde_res] * 3, "pairs_within_cutoff": pairs,
+                  "bandwidth": float(kde.h), "cutoff": float(R),
+                  "roofline": {"bound": "valu", "achieved": rate, "peak": peak, "unit": "pairs/s",
+                               "frac": rate / peak},
+                  "workload": f"2^24 Gaussian-mixture particles, {kde_res}^3 grid, Scott bandwidth, cut-off 5.3 h"}
+    return out
+
+
 def copy_calibration(torch):
     """A plain copy of 2^28 floats (1 GiB read + 1 GiB written), five times."""
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -339,6 +492,12 @@ def main():
     except Exception as e:  # reported, never fatal
         result["adjoint"] = {"error": str(e)}
         result["adjoint_fwd_ratio"] = None
+    if not args.no_extras:
+        for key, fn in (("octree", bench_octree), ("density", bench_density)):
+            try:
+                result[key] = fn(torch, jt, W, L, lib, dev)
+            except Exception as e:  # reported, never fatal
+                result[key] = {"error": str(e)}
     result["setup_s"] = gen_s
     print(json.dumps(result), flush=True)
 

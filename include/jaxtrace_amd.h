@@ -201,6 +201,13 @@ int jt_hashgrid_build(const jt_hashgrid* g, const float* P, void* ws, size_t ws_
 int jt_hashgrid_query(const jt_hashgrid* g, const float* Q, int64_t m, float radius,
                       int32_t per_cell_cap, int32_t max_neighbors, int32_t* out_idx,
                       float* out_dist, void* stream);
+/* No reference counterpart: per query, stats[2*i] = candidates read (sum over
+ * the probed cells of min(count, per_cell_cap)) and stats[2*i+1] = those within
+ * radius; the input-determined counts of the SPH byte model and KDE pair
+ * count, for the benchmark's roofline. */
+int jt_hashgrid_query_stats(const jt_hashgrid* h, const float* Q, int64_t m, float radius, int32_t per_cell_cap,
+                            int32_t* stats, void* stream);
+
 /* Replaces SPHDensityEstimator._evaluate_density (sph.py:269-279): the sum of
  * m_j W(d_j, h_j) over the query's first max_neighbors neighbours within
  * radius.  masses/smoothing may be NULL (1 and h_mean). */
@@ -266,7 +273,13 @@ typedef struct jt_octree_field {
  * where a containing element was found, 0 where the fallback was used. */
 int jt_octree_sample(const jt_octree_field* field, const float* x, int64_t n, float t, float* out,
                      int32_t* found, void* stream);
-size_t jt_track_octree_workspace_size(int64_t n, int32_t n_steps);
+/* No reference counterpart: per query point, the three input-determined
+ * counts of the point-location byte model (stats[3*i+0..2] = octree levels
+ * descended, leaf candidates scanned up to the first hit, candidates whose
+ * box contains the point), for the benchmark's roofline. */
+int jt_octree_locate_stats(const jt_octree_field* field, const float* x, int64_t n, float t, int32_t* stats,
+                           void* stream);
+size_t jt_track_octree_workspace_size(int64_t n, int32_t n_steps, const jt_track_opts* opts);
 /* jt_track_grid for an octree field (same arguments and recording). */
 int jt_track_octree(const jt_octree_field* field, const jt_boundary* bc, int32_t integrator, float* x,
                     int64_t n, const float* step_times, float dt, int32_t n_steps, int32_t record_every,

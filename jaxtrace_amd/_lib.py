@@ -123,7 +123,9 @@ _SIGS = {
     "jt_octree_free": (None, [C.c_void_p]),
     "jt_octree_sample": (C.c_int, [C.POINTER(OctreeField), C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),
-    "jt_track_octree_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "jt_track_octree_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.POINTER(TrackOpts)]),
+    "jt_hashgrid_query_stats": (C.c_int, [C.POINTER(HashGridDesc), C.c_void_p, C.c_int64, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
+    "jt_octree_locate_stats": (C.c_int, [C.POINTER(OctreeField), C.c_void_p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
     "jt_track_octree": (C.c_int, [C.POINTER(OctreeField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
                                   C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                   C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),

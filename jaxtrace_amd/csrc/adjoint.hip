@@ -22,7 +22,7 @@
 //   3. HBM: the box is added to dL/dU with coalesced float atomics at the end
 //      of the launch; flushes outside the box go straight to HBM.
 #include "grid_sampler.h"
-#include <hipcub/hipcub.hpp>
+#include "cell_sort.h"
 #include <algorithm>
 #include <vector>
 
@@ -456,47 +456,8 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
         }                                                                                                 \
     } while (0)
 
-// Hilbert index of each particle's cell for the backward sort, by Skilling's
-// transpose algorithm (J. Skilling, "Programming the Hilbert curve", AIP Conf.
-// Proc. 707, 381 (2004)): undo the excess work per bit plane, Gray-encode, and
-// interleave the transposed coordinates (x most significant).  Consecutive
-// Hilbert cells are face neighbours, so a block of consecutive particles spans
-// a compact box even where it crosses between aligned sub-cubes (a Z-order
-// range can jump across two or three axes there).
-__device__ __forceinline__ unsigned adj_spread3(unsigned v) {
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
-__device__ __forceinline__ unsigned hilbert3(unsigned x0, unsigned x1, unsigned x2, int bits) {
-    unsigned X[3] = {x0, x1, x2};
-    const unsigned M = 1u << (bits - 1);
-    for (unsigned Q = M; Q > 1; Q >>= 1) {
-        const unsigned P = Q - 1;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            if (X[i] & Q) {
-                X[0] ^= P;                             // invert
-            } else {
-                const unsigned t = (X[0] ^ X[i]) & P;  // exchange
-                X[0] ^= t;
-                X[i] ^= t;
-            }
-        }
-    }
-    X[1] ^= X[0];                                      // Gray encode
-    X[2] ^= X[1];
-    unsigned t = 0;
-    for (unsigned Q = M; Q > 1; Q >>= 1)
-        if (X[2] & Q) t ^= Q - 1;
-    X[0] ^= t; X[1] ^= t; X[2] ^= t;
-    return (adj_spread3(X[0]) << 2) | (adj_spread3(X[1]) << 1) | adj_spread3(X[2]);
-}
-
+// Hilbert index of each particle's cell for the backward sort: a block of
+// consecutive particles then starts in a compact box.
 __global__ void __launch_bounds__(256)
 k_adj_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int bits, unsigned* __restrict__ keys,
            int* __restrict__ idx) {

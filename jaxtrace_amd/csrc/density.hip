@@ -427,6 +427,56 @@ extern "C" int jt_hashgrid_query(const jt_hashgrid* h, const float* Q, int64_t m
     return JT_OK;
 }
 
+// Per query: C_q = sum over the probed cells of min(count, cap) (candidates
+// read), and the number of those within r.  The SPH/KDE byte and pair models.
+__global__ void __launch_bounds__(256)
+k_query_stats(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int* __restrict__ out) {
+    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= m) return;
+    const int D = g.dim;
+    float q[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
+    int cq[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) cq[a] = cell_coord(q[a], g.origin[a], g.inv);
+    const int probes = (D == 3) ? 27 : 9;
+    int cand = 0, within = 0;
+    for (int p = 0; p < probes; ++p) {
+        int di = (D == 3) ? p / 9 - 1 : p / 3 - 1;
+        int dj = (D == 3) ? (p / 3) % 3 - 1 : p % 3 - 1;
+        int dk = (D == 3) ? p % 3 - 1 : 0;
+        int cell = (D == 3) ? probe_cell(g, cq[0] + di, cq[1] + dj, cq[2] + dk)
+                            : probe_cell(g, cq[0] + di, cq[1] + dj, 0);
+        if (cell < 0) continue;
+        const int st = g.cstart[cell];
+        const int cnt = min(g.cend[cell] - st, cap);
+        cand += cnt;
+        for (int s = 0; s < cnt; ++s) {
+            const float* pp = g.spos + (size_t)(st + s) * D;
+            float dx = pp[0] - q[0], dy = pp[1] - q[1];
+            float d2 = dx * dx + dy * dy;
+            if (D == 3) { float dz = pp[2] - q[2]; d2 = d2 + dz * dz; }
+            within += (sqrtf(d2) <= r) ? 1 : 0;
+        }
+    }
+    out[2 * (size_t)qi] = cand;
+    out[2 * (size_t)qi + 1] = within;
+}
+
+extern "C" int jt_hashgrid_query_stats(const jt_hashgrid* h, const float* Q, int64_t m, float radius,
+                                       int32_t per_cell_cap, int32_t* stats, void* stream) {
+    HGDev g;
+    int rc = make_hg(h, g);
+    if (rc) return rc;
+    JT_REQUIRE(m >= 0 && m < (1LL << 31), "m out of range");
+    JT_REQUIRE(per_cell_cap >= 1, "per_cell_cap must be >= 1");
+    if (m == 0) return JT_OK;
+    JT_REQUIRE(Q && stats, "NULL pointer");
+    hipLaunchKernelGGL(k_query_stats, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g, Q,
+                       (int)m, radius, per_cell_cap, stats);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
 extern "C" int jt_sph_eval(const jt_hashgrid* h, const float* Q, int64_t m, float radius, int32_t per_cell_cap,
                            int32_t max_neighbors, const float* masses, const float* smoothing, float h_mean,
                            int32_t kernel, float* out_rho, void* stream) {

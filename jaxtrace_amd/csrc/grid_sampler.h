@@ -263,6 +263,32 @@ __device__ __forceinline__ void blend_n(const Blend& bl, const float (*A)[3], co
     }
 }
 
+// Blended values of N arbitrary nodes of float32 (N, 3) snapshots, all loads
+// issued before the blend-mode branch (per-value arithmetic of corner()).
+template <int N>
+__device__ __forceinline__ void blend_nodes_f32(const float* b0, const float* b1, const Blend& bl,
+                                                const int* nodes, float (*V)[3]) {
+    if (bl.mode == BL_ZERO || bl.mode == BL_NAN) {
+        const float z = bl.mode == BL_ZERO ? 0.0f : __int_as_float(0x7fc00000);
+#pragma unroll
+        for (int q = 0; q < N; ++q) V[q][0] = V[q][1] = V[q][2] = z;
+        return;
+    }
+    const float* bb = bl.mode == BL_COPY ? b0 : b1;
+    float A[N][3], B[N][3];
+#pragma unroll
+    for (int q = 0; q < N; ++q) load_node<ST_F32x3>(b0, nodes[q], A[q][0], A[q][1], A[q][2]);
+#pragma unroll
+    for (int q = 0; q < N; ++q) load_node<ST_F32x3>(bb, nodes[q], B[q][0], B[q][1], B[q][2]);
+    __asm__ volatile("" ::: "memory");                 // keep the loads ahead of the mode branch
+    if (bl.mode == BL_COPY) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) { V[q][0] = A[q][0]; V[q][1] = A[q][1]; V[q][2] = A[q][2]; }
+        return;
+    }
+    blend_n<N>(bl, A, B, V);
+}
+
 // The eight stored corner values of one snapshot; corner (ci, cj, ck) at
 // index ci*4 + cj*2 + ck.  ZH: the (k0, k0+1) corners are one contiguous pair.
 template <int ST, bool ZH>

@@ -11,6 +11,7 @@
 // same).  Fallback: nearest of the 4 nodes of leaf slot 0 (slot value -1
 // indexes the last element, as JAX negative indexing does).
 #include "grid_sampler.h"
+#include "cell_sort.h"
 
 namespace jt {
 
@@ -53,7 +54,11 @@ __device__ __forceinline__ void blended_node(const OctDev& o, const Blend& bl, f
 }
 
 // Returns 1 if an element was found (value in out), else 0 (fallback value in out).
-__device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q[3], float out[3]) {
+// STATS: st[0..2] += levels descended, candidates scanned, candidates whose
+// box contains q (the inputs of the point-location byte model).
+template <bool STATS = false>
+__device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q[3], float out[3],
+                                 int* st = nullptr) {
     const float oma = 1.0f - bl.a;
     int node = 0;
     for (int it = 0; it < o.max_depth; ++it) {             // :278-302
@@ -64,14 +69,17 @@ __device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q
         int ch = o.nchild[node * 8 + oc];
         if (ch < 0) break;                                  // stays here for the remaining levels
         node = ch;
+        if constexpr (STATS) ++st[0];
     }
     const int* cand = o.nelem + (size_t)node * o.max_elems;
     const int cnt = o.ncount[node];
     for (int k = 0; k < cnt; ++k) {                         // :309-360, first hit wins
         const int e = cand[k];
         const float* eb = o.ebounds + (size_t)e * 6;
+        if constexpr (STATS) ++st[1];
         if (!(q[0] >= eb[0] && q[1] >= eb[1] && q[2] >= eb[2] &&
               q[0] <= eb[3] && q[1] <= eb[4] && q[2] <= eb[5])) continue;
+        if constexpr (STATS) ++st[2];
         const int* ni = o.conn + (size_t)e * 4;
         float v[4][3];
         for (int j = 0; j < 4; ++j)
@@ -92,7 +100,9 @@ __device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q
         const float lo = -1e-6f, hi = 0.999999f;           // :246-248
         if (b0 >= lo && b1 >= lo && b2 >= lo && b3 >= lo && b0 <= hi && b1 <= hi && b2 <= hi && b3 <= hi) {
             float F[4][3];
-            for (int j = 0; j < 4; ++j) blended_node(o, bl, oma, ni[j], F[j]);
+            const int nodes[4] = {ni[0], ni[1], ni[2], ni[3]};
+            blend_nodes_f32<4>(o.data + (size_t)bl.i0 * o.snap_stride, o.data + (size_t)bl.i1 * o.snap_stride, bl,
+                               nodes, F);
             for (int a = 0; a < 3; ++a)
                 out[a] = ((b0 * F[0][a] + b1 * F[1][a]) + b2 * F[2][a]) + b3 * F[3][a];
             return 1;
@@ -115,6 +125,20 @@ __device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q
 }
 
 __global__ void __launch_bounds__(256)
+k_oct_stats(OctDev o, const float* __restrict__ x, int n, float t, int* __restrict__ stats) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Blend b = time_blend(o.times, o.T, t, o.interp, o.extrap);
+    float q[3] = {x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2]};
+    float v[3];
+    int st[3] = {0, 0, 0};
+    oct_sample<true>(o, b, q, v, st);
+    stats[3 * (size_t)i] = st[0];
+    stats[3 * (size_t)i + 1] = st[1];
+    stats[3 * (size_t)i + 2] = st[2];
+}
+
+__global__ void __launch_bounds__(256)
 k_oct_sample(OctDev o, const float* __restrict__ x, int n, float t, float* __restrict__ out,
              int* __restrict__ found) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -129,25 +153,47 @@ k_oct_sample(OctDev o, const float* __restrict__ x, int n, float t, float* __res
     if (found) found[i] = f;
 }
 
+// Fused multi-step tracking through the octree field; particles may be
+// cell-sorted (ids maps a slot to the caller's particle for trajectory rows).
+// Blocks map XCD-contiguously: block b takes chunk (b % 8) * (grid / 8) + b / 8.
 template <int INTEG>
 __global__ void __launch_bounds__(256)
 k_oct_track(OctDev o, BcDev bc, float* __restrict__ xs, int n, const Blend* __restrict__ tab, int s0, int s1,
-            float dt, float dth, float dt6, int rec_every, float* __restrict__ traj) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+            float dt, float dth, float dt6, int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {
+    const int per = gridDim.x >> 3;                     // gridDim.x is a multiple of 8
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int i = lb * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const size_t dst = ids ? (size_t)ids[i] : (size_t)i;
     float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
     auto f = [&](const Blend& b, const float q[3], float out[3]) { oct_sample(o, b, q, out); };
     for (int s = s0; s < s1; ++s) {
         rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);
         apply_bc(bc, p);
         if (traj != nullptr && (s + 1) % rec_every == 0) {
-            float* out = traj + ((size_t)((s + 1) / rec_every) * (size_t)n + i) * 3;
+            float* out = traj + ((size_t)((s + 1) / rec_every) * (size_t)n + dst) * 3;
             out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
         }
     }
     xs[3 * (size_t)i] = p[0];
     xs[3 * (size_t)i + 1] = p[1];
     xs[3 * (size_t)i + 2] = p[2];
+}
+
+// Hilbert key of each particle's cell on a 1024^3 lattice over the root box.
+__global__ void __launch_bounds__(256)
+k_oct_keys(OctDev o, const float* __restrict__ xs, int n, unsigned* __restrict__ keys, int* __restrict__ perm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned c[3];
+    for (int a = 0; a < 3; ++a) {
+        const float lo = o.nmin[a], w = o.nmax[a] - o.nmin[a];
+        float f = (w > 0.0f) ? (xs[3 * (size_t)i + a] - lo) / w * 1024.0f : 0.0f;
+        f = (f != f) ? 0.0f : fminf(fmaxf(floorf(f), 0.0f), 1023.0f);
+        c[a] = (unsigned)f;
+    }
+    keys[i] = hilbert3(c[0], c[1], c[2], 10);
+    perm[i] = i;
 }
 
 static int make_oct(const jt_octree_field* f, OctDev& o) {
@@ -186,10 +232,29 @@ extern "C" int jt_octree_sample(const jt_octree_field* field, const float* x, in
     return JT_OK;
 }
 
-extern "C" size_t jt_track_octree_workspace_size(int64_t n, int32_t n_steps) {
-    (void)n;
+extern "C" int jt_octree_locate_stats(const jt_octree_field* field, const float* x, int64_t n, float t,
+                                      int32_t* stats, void* stream) {
+    OctDev o;
+    int rc = make_oct(field, o);
+    if (rc) return rc;
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(x && stats, "NULL x/stats");
+    hipLaunchKernelGGL(k_oct_stats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, o, x,
+                       (int)n, t, stats);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+extern "C" size_t jt_track_octree_workspace_size(int64_t n, int32_t n_steps, const jt_track_opts* opts) {
     Arena a(nullptr, 0);
     a.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
+    if (opts && opts->sort_every > 0 && n > 0) {
+        a.take<unsigned>(n); a.take<unsigned>(n);
+        a.take<int>(n); a.take<int>(n); a.take<int>(n); a.take<int>(n);
+        a.take<float>((size_t)3 * n); a.take<float>((size_t)3 * n);
+        a.take<char>(sort_temp_bytes((int)n));
+    }
     return a.used + 256;
 }
 
@@ -211,9 +276,26 @@ extern "C" int jt_track_octree(const jt_octree_field* field, const jt_boundary* 
     if (traj != nullptr && n > 0)
         JT_HIP(hipMemcpyAsync(traj, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
     if (n == 0 || n_steps == 0) return JT_OK;
+    JT_REQUIRE(opts == nullptr || (opts->sort_every >= 0 && opts->steps_per_launch >= 0), "bad opts");
+    const int sort_every = opts ? opts->sort_every : 0;
     Arena arena(ws, ws_bytes);
     Blend* tab = arena.take<Blend>((size_t)3 * n_steps);
-    if (!tab || !arena.ok()) return fail(JT_ENOMEM, "workspace too small");
+    unsigned *keys = nullptr, *skeys = nullptr;
+    int *perm = nullptr, *sperm = nullptr, *ids = nullptr, *ids2 = nullptr;
+    float *xa = nullptr, *xb = nullptr;
+    char* tmp = nullptr;
+    size_t temp = 0;
+    if (sort_every > 0) {
+        keys = arena.take<unsigned>(n); skeys = arena.take<unsigned>(n);
+        perm = arena.take<int>(n); sperm = arena.take<int>(n);
+        ids = arena.take<int>(n); ids2 = arena.take<int>(n);
+        xa = arena.take<float>((size_t)3 * n); xb = arena.take<float>((size_t)3 * n);
+        temp = sort_temp_bytes((int)n);
+        tmp = arena.take<char>(temp);
+    }
+    if (!tab || !arena.ok() || (sort_every > 0 && tmp == nullptr))
+        return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu",
+                    jt_track_octree_workspace_size(n, n_steps, opts), ws_bytes);
     GridDev g{};
     g.times = o.times; g.T = o.T; g.interp = o.interp; g.extrap = o.extrap;
     hipLaunchKernelGGL(k_blend_table, dim3((n_steps + 127) / 128), dim3(128), 0, s, g, step_times, n_steps, dt,
@@ -221,15 +303,37 @@ extern "C" int jt_track_octree(const jt_octree_field* field, const jt_boundary* 
     JT_HIP(hipGetLastError());
     int spl = (opts && opts->steps_per_launch > 0) ? opts->steps_per_launch : (n >= (1 << 20) ? 10 : 500);
     const float dth = 0.5f * dt, dt6 = dt / 6.0f;
-    dim3 block(256), grid((unsigned)((n + 255) / 256));
-    for (int s0 = 0; s0 < n_steps; s0 += spl) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    dim3 block(256), grid((nb + 7) / 8 * 8), eb(256), eg(nb);
+    float* cur = x;
+    const int* cur_ids = nullptr;
+    for (int s0 = 0; s0 < n_steps;) {
         int s1 = s0 + spl < n_steps ? s0 + spl : n_steps;
+        if (sort_every > 0) {
+            const int next_sort = (s0 / sort_every + 1) * sort_every;
+            if (s1 > next_sort) s1 = next_sort;
+            if (s0 % sort_every == 0) {
+                hipLaunchKernelGGL(k_oct_keys, eg, eb, 0, s, o, cur, (int)n, keys, perm);
+                JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0, 30, s));
+                float* nx = (cur == xa) ? xb : xa;
+                int* nid = (cur_ids == ids) ? ids2 : ids;
+                hipLaunchKernelGGL(k_permute, eg, eb, 0, s, cur, cur_ids, sperm, (int)n, nx, nid);
+                JT_HIP(hipGetLastError());
+                cur = nx;
+                cur_ids = nid;
+            }
+        }
         if (integrator == JT_RK4)
-            hipLaunchKernelGGL(k_oct_track<JT_RK4>, grid, block, 0, s, o, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            hipLaunchKernelGGL(k_oct_track<JT_RK4>, grid, block, 0, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
         else if (integrator == JT_RK2)
-            hipLaunchKernelGGL(k_oct_track<JT_RK2>, grid, block, 0, s, o, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            hipLaunchKernelGGL(k_oct_track<JT_RK2>, grid, block, 0, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
         else
-            hipLaunchKernelGGL(k_oct_track<JT_EULER>, grid, block, 0, s, o, d, x, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            hipLaunchKernelGGL(k_oct_track<JT_EULER>, grid, block, 0, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+        JT_HIP(hipGetLastError());
+        s0 = s1;
+    }
+    if (cur != x) {
+        hipLaunchKernelGGL(k_unpermute, eg, eb, 0, s, cur, cur_ids, (int)n, x);
         JT_HIP(hipGetLastError());
     }
     return JT_OK;

@@ -15,7 +15,7 @@
 // the 8 XCDs, so block b takes chunk (b % 8) * (grid / 8) + b / 8 and each
 // XCD's L2 sees one contiguous (spatially compact, once sorted) particle range.
 #include "grid_sampler.h"
-#include <hipcub/hipcub.hpp>
+#include "cell_sort.h"
 #include <cmath>
 
 namespace jt {
@@ -157,15 +157,6 @@ __global__ void __launch_bounds__(256) k_boundary(BcDev bc, float* __restrict__ 
 }
 
 // ---- particle re-ordering by grid cell ------------------------------------
-__device__ __forceinline__ unsigned spread3(unsigned v) {     // 10 bits -> every third bit
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 __device__ __forceinline__ unsigned cell_bits(float x, float o, float d, int n, int shift) {
     float f = (x - o) / d;
     int c = (f != f) ? 0 : (int)fminf(fmaxf(floorf(f), 0.0f), (float)(n - 1));
@@ -182,28 +173,6 @@ k_morton(GridDev g, const float* __restrict__ xs, int n, int shift, unsigned* __
     unsigned cz = cell_bits(xs[3 * (size_t)i + 2], g.o[2], g.d[2], g.nz, shift);
     keys[i] = (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
     perm[i] = i;
-}
-
-__global__ void __launch_bounds__(256)
-k_permute(const float* __restrict__ xin, const int* __restrict__ idin, const int* __restrict__ perm, int n,
-          float* __restrict__ xout, int* __restrict__ idout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int j = perm[i];
-    xout[3 * (size_t)i] = xin[3 * (size_t)j];
-    xout[3 * (size_t)i + 1] = xin[3 * (size_t)j + 1];
-    xout[3 * (size_t)i + 2] = xin[3 * (size_t)j + 2];
-    idout[i] = idin ? idin[j] : j;
-}
-
-__global__ void __launch_bounds__(256)
-k_unpermute(const float* __restrict__ xs, const int* __restrict__ ids, int n, float* __restrict__ x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t j = (size_t)ids[i];
-    x[3 * j] = xs[3 * (size_t)i];
-    x[3 * j + 1] = xs[3 * (size_t)i + 1];
-    x[3 * j + 2] = xs[3 * (size_t)i + 2];
 }
 
 template <int INTEG, bool ZH>
@@ -229,12 +198,6 @@ static void launch_sample(int st, dim3 grid, dim3 block, hipStream_t s, const Gr
     }
 }
 
-static size_t sort_temp_bytes(int n) {
-    size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (const int*)nullptr, (int*)nullptr, n, 0, 30);
-    return temp;
-}
 
 }  // namespace jt
 

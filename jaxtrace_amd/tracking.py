@@ -39,7 +39,8 @@ class TrackerOptions:
       return_device      Trajectory arrays stay torch tensors on the GPU
       steps_per_launch   steps fused into one kernel launch (0 = library default)
       sort_every         re-sort particles by grid cell every k steps (0 = off;
-                         None = 50 for >= 65536 particles on structured fields)
+                         None = from 65536 particles, every 50 steps on
+                         structured fields and every 10 on octree fields)
     """
     max_memory_gb: float = 8.0
     batch_size: Optional[int] = None
@@ -131,7 +132,8 @@ class ParticleTracker:
         o.steps_per_launch = int(self.options.steps_per_launch)
         se = self.options.sort_every
         if se is None:
-            se = 50 if (n >= 65536 and getattr(self.field, "kind", "grid") == "grid") else 0
+            kind = getattr(self.field, "kind", "grid")
+            se = 0 if n < 65536 else {"grid": 50, "octree": 10}.get(kind, 0)
         o.sort_every = int(se)
         o.block = 0
         return o
@@ -163,7 +165,7 @@ class ParticleTracker:
         o = self._opts(n)
         kind = getattr(fld, "kind", "grid")
         if kind == "octree":
-            ws_bytes = int(lib.jt_track_octree_workspace_size(n, n_steps))
+            ws_bytes = int(lib.jt_track_octree_workspace_size(n, n_steps, C.byref(o)))
             entry, name = lib.jt_track_octree, "jt_track_octree"
         elif kind == "nodes":
             ws_bytes = int(lib.jt_track_nodes_workspace_size(n, n_steps))

@@ -96,6 +96,21 @@ def test_track_inlet_per_step_bitwise(gpu):
         np.testing.assert_array_equal(traj.positions[s + 1], nxt)
 
 
+def test_cell_sorting_is_invisible(gpu):
+    """Re-ordering particles along the Hilbert curve of their cell every k
+    steps (trajectories written back through the permutation) leaves every
+    recorded position bitwise unchanged, with and without the inlet."""
+    f, pts, conn, data, times = _field(n=10, T=8)
+    bounds = [[0, 0, 0], [1, 1, 1]]
+    x0 = W.uniform_particles(3000, 7)
+    for bc in (jt.continuous_inlet_boundary_factory(bounds, "x", "positive", "grid", {"x": 64, "y": 64, "z": 64}),
+               jt.reflective_boundary(bounds)):
+        a = jt.create_tracker("rk4", f, bc, sort_every=0).track_particles(x0, (0.0, 0.04), 41, dt=1e-3)
+        b = jt.create_tracker("rk4", f, bc, sort_every=3, steps_per_launch=7).track_particles(x0, (0.0, 0.04), 41,
+                                                                                            dt=1e-3)
+        np.testing.assert_array_equal(a.positions, b.positions)
+
+
 def test_full_size_mesh_properties(gpu):
     """128^3 hexes x 6 = 12,582,912 jittered tets, 8 snapshots, 2^22 particles,
     RK4 dt = 1e-3 with the continuous inlet: positions stay in the unit box

@@ -347,7 +347,7 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=128, sample=1 << 16):
     Xk, Yk, Zk = np.meshgrid(x, y, z, indexing="xy")
     Qk = np.stack([Xk.ravel(), Yk.ravel(), Zk.ravel()], 1).astype(F)
     mk = len(Qk)
-    sel = np.random.default_rng(13).choice(mk, min(sample, mk), replace=False)
+    sel = np.random.default_rng(13).choice(mk, min(2048, mk), replace=False)
     Qks = torch.from_numpy(Qk[sel]).to(dev)
     stk = torch.empty((len(sel), 2), dtype=torch.int32, device=dev)
     L.check(lib.jt_hashgrid_query_stats(C.byref(hg._desc), L.ptr(Qks), len(sel), C.c_float(R), 1 << 30,

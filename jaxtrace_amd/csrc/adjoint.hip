@@ -223,7 +223,7 @@ __device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, con
 // particle perm[t] (cell-sorted) and stores the state before each step at
 // seg[(s - sa) * n + t] (thread order: coalesced stores).
 template <int ST, bool ZH>
-__global__ void __launch_bounds__(256, 4)
+__global__ void __launch_bounds__(256, TrackWaves<ST>::v)
 k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const int* __restrict__ perm,
                 const Blend* __restrict__ tab, int sa, int sb, float dt, float dth, float dt6,
                 float* __restrict__ seg) {

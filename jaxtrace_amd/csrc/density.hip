@@ -429,10 +429,12 @@ extern "C" int jt_hashgrid_query(const jt_hashgrid* h, const float* Q, int64_t m
 
 // Per query: C_q = sum over the probed cells of min(count, cap) (candidates
 // read), and the number of those within r.  The SPH/KDE byte and pair models.
+// One wavefront per query, lanes striding over each cell's candidates.
 __global__ void __launch_bounds__(256)
 k_query_stats(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int* __restrict__ out) {
-    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= m) return;
+    const int lane = threadIdx.x & 63;
+    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (qi >= m) return;                       // whole wave exits together
     const int D = g.dim;
     float q[3] = {0.f, 0.f, 0.f};
     for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
@@ -450,7 +452,7 @@ k_query_stats(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int
         const int st = g.cstart[cell];
         const int cnt = min(g.cend[cell] - st, cap);
         cand += cnt;
-        for (int s = 0; s < cnt; ++s) {
+        for (int s = lane; s < cnt; s += 64) {
             const float* pp = g.spos + (size_t)(st + s) * D;
             float dx = pp[0] - q[0], dy = pp[1] - q[1];
             float d2 = dx * dx + dy * dy;
@@ -458,8 +460,11 @@ k_query_stats(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int
             within += (sqrtf(d2) <= r) ? 1 : 0;
         }
     }
-    out[2 * (size_t)qi] = cand;
-    out[2 * (size_t)qi + 1] = within;
+    for (int o = 32; o > 0; o >>= 1) within += __shfl_xor(within, o);
+    if (lane == 0) {
+        out[2 * (size_t)qi] = cand;
+        out[2 * (size_t)qi + 1] = within;
+    }
 }
 
 extern "C" int jt_hashgrid_query_stats(const jt_hashgrid* h, const float* Q, int64_t m, float radius,
@@ -471,7 +476,7 @@ extern "C" int jt_hashgrid_query_stats(const jt_hashgrid* h, const float* Q, int
     JT_REQUIRE(per_cell_cap >= 1, "per_cell_cap must be >= 1");
     if (m == 0) return JT_OK;
     JT_REQUIRE(Q && stats, "NULL pointer");
-    hipLaunchKernelGGL(k_query_stats, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g, Q,
+    hipLaunchKernelGGL(k_query_stats, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, (hipStream_t)stream, g, Q,
                        (int)m, radius, per_cell_cap, stats);
     JT_HIP(hipGetLastError());
     return JT_OK;

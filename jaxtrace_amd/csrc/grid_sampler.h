@@ -383,6 +383,12 @@ __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
     lerp_corners(g, C, out);
 }
 
+// Waves per SIMD the tracking kernels' register budget is sized for: float32
+// snapshots keep all 16 corner loads of a stage in flight without spilling at
+// 3 (168 VGPRs); 16-bit snapshots need half the registers per load and run
+// best at 4.
+template <int ST> struct TrackWaves { static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? 3 : 4; };
+
 // ---- boundary epilogue (boundary.py) ----------------------------------------
 struct BcDev {
     int kind[3];

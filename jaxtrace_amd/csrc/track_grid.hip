@@ -103,12 +103,10 @@ int make_bc_dev(const jt_boundary* b, BcDev& d) {
 // ---- kernels -------------------------------------------------------------
 
 
-#ifndef JT_TRACK_WAVES_PER_EU
-#define JT_TRACK_WAVES_PER_EU 4
-#endif
+
 
 template <int INTEG, int ST, bool ZH>
-__global__ void __launch_bounds__(256, JT_TRACK_WAVES_PER_EU)
+__global__ void __launch_bounds__(256, TrackWaves<ST>::v)
 k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
         const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
         int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {

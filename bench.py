@@ -329,7 +329,7 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=128, sample=1 << 16):
                           bounds=((0, 1), (0, 1), (0, 1)), cutoff_sigmas=5.3)
     x, y, z = kde._axes_3d()
     R = F(5.3 * kde.h)
-    hg = jt.HashGridNeighbors(Pd, float(R), 1, "x64", dev)
+    hg = jt.HashGridNeighbors(Pd, float(R) / 2, 1, "x64", dev)
     gx, gy, gz = (torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (x, y, z))
     D = torch.empty((len(y), len(x), len(z)), dtype=torch.float32, device=dev)
 
@@ -350,7 +350,8 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=128, sample=1 << 16):
     sel = np.random.default_rng(13).choice(mk, min(2048, mk), replace=False)
     Qks = torch.from_numpy(Qk[sel]).to(dev)
     stk = torch.empty((len(sel), 2), dtype=torch.int32, device=dev)
-    L.check(lib.jt_hashgrid_query_stats(C.byref(hg._desc), L.ptr(Qks), len(sel), C.c_float(R), 1 << 30,
+    hgs = jt.HashGridNeighbors(Pd, float(R), 1, "x64", dev)        # cells of R: the 27 probes cover R
+    L.check(lib.jt_hashgrid_query_stats(C.byref(hgs._desc), L.ptr(Qks), len(sel), C.c_float(R), 1 << 30,
                                         L.ptr(stk), L.stream_handle()), "jt_hashgrid_query_stats")
     pairs = float(stk[:, 1].double().mean()) * mk
     rate = pairs / (kde_ms * 1e-3)

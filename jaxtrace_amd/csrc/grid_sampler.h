@@ -220,7 +220,12 @@ __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, in
     }
     float fl = floorf(f);
     int ii = (fl != fl) ? 0 : (int)fl;                 // NaN -> 0
-    int jj = (g.bmode == JT_BOUNDS_PERIODIC) ? (ii + 1) % n : min(max(ii + 1, 0), n - 1);
+    // periodic: (ii + 1) mod n; ii is in [0, n] here (jmodf can round up to n),
+    // so one conditional subtract gives the remainder, and the clamp below
+    // covers n = 1
+    int jj = ii + 1;
+    if (g.bmode == JT_BOUNDS_PERIODIC) jj = (jj >= n) ? jj - n : jj;
+    else jj = min(max(jj, 0), n - 1);
     w = f - (float)ii;
     i0 = min(max(ii, 0), n - 1);                       // gather index clamp
     i1 = min(max(jj, 0), n - 1);
@@ -289,20 +294,28 @@ __device__ __forceinline__ void blend_nodes_f32(const float* b0, const float* b1
     blend_n<N>(bl, A, B, V);
 }
 
+// Bytes of one stored node.
+template <int ST> struct NodeBytes { static constexpr unsigned v = (ST == ST_F32x3) ? 12u : (ST == ST_F32x4 ? 16u : 8u); };
+
 // The eight stored corner values of one snapshot; corner (ci, cj, ck) at
-// index ci*4 + cj*2 + ck.  ZH: the (k0, k0+1) corners are one contiguous pair.
+// index ci*4 + cj*2 + ck.  Addresses are a uniform snapshot base plus 32-bit
+// byte offsets (row[] of the (i, j) rows, k0b/k1b of the z nodes), so each
+// load takes the scalar-base form with no 64-bit address arithmetic.
+// ZH: the (k0, k0+1) corners are one contiguous pair.
 template <int ST, bool ZH>
-__device__ __forceinline__ void load_corners(const void* base, const int row[4], int k0, int k1, float (*A)[3]) {
+__device__ __forceinline__ void load_corners(const void* base, const unsigned row[4], unsigned k0b, unsigned k1b,
+                                             float (*A)[3]) {
+    const char* b = (const char*)base;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         if constexpr (ZH) {
             float a[6];
-            load_pair<ST>(base, row[r] + k0, a);
+            load_pair<ST>(b + (row[r] + k0b), 0, a);
             A[2 * r][0] = a[0]; A[2 * r][1] = a[1]; A[2 * r][2] = a[2];
             A[2 * r + 1][0] = a[3]; A[2 * r + 1][1] = a[4]; A[2 * r + 1][2] = a[5];
         } else {
-            load_node<ST>(base, row[r] + k0, A[2 * r][0], A[2 * r][1], A[2 * r][2]);
-            load_node<ST>(base, row[r] + k1, A[2 * r + 1][0], A[2 * r + 1][1], A[2 * r + 1][2]);
+            load_node<ST>(b + (row[r] + k0b), 0, A[2 * r][0], A[2 * r][1], A[2 * r][2]);
+            load_node<ST>(b + (row[r] + k1b), 0, A[2 * r + 1][0], A[2 * r + 1][1], A[2 * r + 1][2]);
         }
     }
 }
@@ -320,13 +333,18 @@ __device__ __forceinline__ void gather_from(const void* b0, const void* b1, cons
         for (int q = 0; q < 8; ++q) C.v[q][0] = C.v[q][1] = C.v[q][2] = z;
         return;
     }
-    const int r0 = i0 * ny_, r1 = i1 * ny_;
+    // 32-bit byte offsets (a snapshot is below 4 GiB, checked on the host)
+    const unsigned NB = NodeBytes<ST>::v;
+    const unsigned rs = (unsigned)nzs_ * NB, ps = (unsigned)ny_ * rs;
+    const unsigned oi0 = (unsigned)i0 * ps, oi1 = (unsigned)i1 * ps;
+    const unsigned oj0 = (unsigned)j0 * rs, oj1 = (unsigned)j1 * rs;
     // rows in corner order (ci, cj): (0,0) (0,1) (1,0) (1,1)
-    const int row[4] = {(r0 + j0) * nzs_, (r0 + j1) * nzs_, (r1 + j0) * nzs_, (r1 + j1) * nzs_};
+    const unsigned row[4] = {oi0 + oj0, oi0 + oj1, oi1 + oj0, oi1 + oj1};
+    const unsigned k0b = (unsigned)k0 * NB, k1b = (unsigned)k1 * NB;
     // both snapshots' loads issue before any wait ('copy' reads V0 twice)
     float A[8][3], B[8][3];
-    load_corners<ST, ZH>(b0, row, k0, k1, A);
-    load_corners<ST, ZH>(bl.mode == BL_COPY ? b0 : b1, row, k0, k1, B);
+    load_corners<ST, ZH>(b0, row, k0b, k1b, A);
+    load_corners<ST, ZH>(bl.mode == BL_COPY ? b0 : b1, row, k0b, k1b, B);
     __asm__ volatile("" ::: "memory");                 // keep the loads ahead of the mode branch
     if (bl.mode == BL_COPY) {
 #pragma unroll

@@ -5,11 +5,16 @@
 //   query; particles stream through LDS in tiles of 256 and are reused by all
 //   256 queries of the block.  Terms are accumulated in float64.
 // k_kde_grid_cutoff: for the structured evaluation grid of evaluate_3d
-//   (meshgrid 'xy' order, index (iy*nx + ix)*nz + iz).  A block owns a
-//   4 x 4 x 16 tile of grid points and streams, through LDS, the particles of
-//   the hash-grid cells (cell size >= R) that overlap the tile's box grown by
-//   R; each query sums only particles with |q-p| <= R.  Dropped terms are each
-//   below norm*exp(-R^2/(2h^2)) (the mean over n bounds their total by it).
+//   (meshgrid 'xy' order, index (iy*nx + ix)*nz + iz).  A block owns an
+//   8 x 8 x 16 tile of grid points (a thread: one (x, y) column, 4 z points)
+//   and streams, through LDS, the particles of the hash-grid cells that
+//   overlap the tile's box grown by R; every particle within R of any tile
+//   point is summed (and some farther ones).  The Gaussian is separable on
+//   the lattice: per particle the block evaluates exp(-0.5 d^2 / h^2) once
+//   for each of the tile's 8 x, 8 y and 16 z coordinates, and a pair's term
+//   is the product of its three factors (a few ulp from exp of the sum, far
+//   inside the 1e-5 density bar).  Each dropped particle is farther than R
+//   from the point, so each dropped term is below norm*exp(-R^2/(2h^2)).
 #include "jt_common.h"
 
 namespace jt {
@@ -55,31 +60,38 @@ __global__ void __launch_bounds__(256)
 k_kde_grid_cutoff(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
                   const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
                   float* __restrict__ out) {
-    __shared__ float sp[256 * 3];
+    __shared__ float ex[256][8];
+    __shared__ float ey[256][8];
+    __shared__ float4 ez[256][4];
     const int tid = threadIdx.x;
-    const int tiles_z = (nz + 15) / 16, tiles_x = (nx + 3) / 4;
+    const int tiles_z = (nz + 15) / 16, tiles_x = (nx + 7) / 8;
     int b = blockIdx.x;
-    const int tz = b % tiles_z; b /= tiles_z;
-    const int tx = b % tiles_x; b /= tiles_x;
-    const int ty = b;
-    const int iz = tz * 16 + (tid & 15);
-    const int ix = tx * 4 + ((tid >> 4) & 3);
-    const int iy = ty * 4 + (tid >> 6);
-    const bool live = iz < nz && ix < nx && iy < ny;
-    const float qx = gx[min(ix, nx - 1)], qy = gy[min(iy, ny - 1)], qz = gz[min(iz, nz - 1)];
-    // tile box (grid axes are monotone) grown by R, as cell ranges
-    const int x0 = tx * 4, x1 = min(tx * 4 + 3, nx - 1);
-    const int y0 = ty * 4, y1 = min(ty * 4 + 3, ny - 1);
-    const int z0 = tz * 16, z1 = min(tz * 16 + 15, nz - 1);
-    float lo[3] = {fminf(gx[x0], gx[x1]) - R, fminf(gy[y0], gy[y1]) - R, fminf(gz[z0], gz[z1]) - R};
-    float hi[3] = {fmaxf(gx[x0], gx[x1]) + R, fmaxf(gy[y0], gy[y1]) + R, fmaxf(gz[z0], gz[z1]) + R};
+    const int bz = b % tiles_z; b /= tiles_z;
+    const int bx = b % tiles_x; b /= tiles_x;
+    const int by = b;
+    const int zq = tid & 3, txl = (tid >> 2) & 7, tyl = tid >> 5;
+    const int ix = bx * 8 + txl, iy = by * 8 + tyl, iz0 = bz * 16 + zq * 4;
+    // the tile's coordinates (clamped at the grid's end; clamped points are not written)
+    float xs[8], ys[8], zs[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { xs[i] = gx[min(bx * 8 + i, nx - 1)]; ys[i] = gy[min(by * 8 + i, ny - 1)]; }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) zs[k] = gz[min(bz * 16 + k, nz - 1)];
+    float lo[3] = {xs[0], ys[0], zs[0]}, hi[3] = {xs[0], ys[0], zs[0]};
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        lo[0] = fminf(lo[0], xs[i]); hi[0] = fmaxf(hi[0], xs[i]);
+        lo[1] = fminf(lo[1], ys[i]); hi[1] = fmaxf(hi[1], ys[i]);
+    }
+#pragma unroll
+    for (int k = 1; k < 16; ++k) { lo[2] = fminf(lo[2], zs[k]); hi[2] = fmaxf(hi[2], zs[k]); }
     int c0[3], c1[3];
     for (int a = 0; a < 3; ++a) {
-        c0[a] = clampi((int)floorf((lo[a] - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
-        c1[a] = clampi((int)floorf((hi[a] - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
+        c0[a] = clampi((int)floorf((lo[a] - R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
+        c1[a] = clampi((int)floorf((hi[a] + R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
     }
-    const float R2 = R * R;
-    double acc = 0.0;
+    const float nh = -0.5f * inv_h2;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int cx = c0[0]; cx <= c1[0]; ++cx)
         for (int cy = c0[1]; cy <= c1[1]; ++cy) {
             // cells (cx, cy, c0z..c1z) are consecutive ids: their points are one run
@@ -92,24 +104,47 @@ k_kde_grid_cutoff(const float* __restrict__ gx, int nx, const float* __restrict_
             }
             for (int base = s; base < e; base += 256) {
                 const int j = base + tid;
+                // this thread's particle: its factor for each tile coordinate
                 if (j < e) {
-                    sp[tid * 3] = g.spos[(size_t)j * 3];
-                    sp[tid * 3 + 1] = g.spos[(size_t)j * 3 + 1];
-                    sp[tid * 3 + 2] = g.spos[(size_t)j * 3 + 2];
+                    const float px = g.spos[(size_t)j * 3], py = g.spos[(size_t)j * 3 + 1];
+                    const float pz = g.spos[(size_t)j * 3 + 2];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const float dx = xs[i] - px, dy = ys[i] - py;
+                        ex[tid][i] = __expf(nh * (dx * dx));
+                        ey[tid][i] = __expf(nh * (dy * dy));
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float4 v;
+                        float dz;
+                        dz = zs[4 * q] - pz;     v.x = __expf(nh * (dz * dz));
+                        dz = zs[4 * q + 1] - pz; v.y = __expf(nh * (dz * dz));
+                        dz = zs[4 * q + 2] - pz; v.z = __expf(nh * (dz * dz));
+                        dz = zs[4 * q + 3] - pz; v.w = __expf(nh * (dz * dz));
+                        ez[tid][q] = v;
+                    }
                 }
                 __syncthreads();
                 const int cnt = min(256, e - base);
-                float part = 0.0f;
+                float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
                 for (int k = 0; k < cnt; ++k) {
-                    float dx = qx - sp[k * 3], dy = qy - sp[k * 3 + 1], dz = qz - sp[k * 3 + 2];
-                    float r2 = (dx * dx + dy * dy) + dz * dz;
-                    if (r2 <= R2) part += expf(-0.5f * (r2 * inv_h2));
+                    const float exy = ex[k][txl] * ey[k][tyl];
+                    const float4 z4 = ez[k][zq];
+                    part[0] = fmaf(exy, z4.x, part[0]);
+                    part[1] = fmaf(exy, z4.y, part[1]);
+                    part[2] = fmaf(exy, z4.z, part[2]);
+                    part[3] = fmaf(exy, z4.w, part[3]);
                 }
-                acc += (double)part;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] += (double)part[q];
                 __syncthreads();
             }
         }
-    if (live) out[((size_t)iy * nx + ix) * nz + iz] = (float)(acc * scale);
+    if (ix < nx && iy < ny)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (iz0 + q < nz) out[((size_t)iy * nx + ix) * nz + iz0 + q] = (float)(acc[q] * scale);
 }
 
 }  // namespace jt
@@ -136,8 +171,8 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
                                   const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
                                   float cutoff, float* out, void* stream) {
     JT_REQUIRE(grid && grid->dim == 3 && grid->key_mode == JT_KEY_X64, "needs a 3-D x64 hash grid");
-    JT_REQUIRE(grid->inv > 0.0f && cutoff > 0.0f && (1.0f / grid->inv) >= cutoff * 0.999999f,
-               "hash-grid cell size must be >= the cutoff radius");
+    JT_REQUIRE(grid->inv > 0.0f && cutoff > 0.0f && (1.0f / grid->inv) >= cutoff * 0.124999f,
+               "hash-grid cell size must be >= cutoff / 8");
     JT_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && n_total >= 1, "bad sizes");
     JT_REQUIRE(gx && gy && gz && out && grid->sorted_pos && grid->cell_start && grid->cell_end, "NULL pointer");
     KGrid g;
@@ -147,7 +182,7 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
     const double hh = (double)h;
     const float inv_h2 = (float)(1.0 / (hh * hh));
     const double norm = pow(2.0 * M_PI, -1.5);
-    long long blocks = (long long)((ny + 3) / 4) * ((nx + 3) / 4) * ((nz + 15) / 16);
+    long long blocks = (long long)((ny + 7) / 8) * ((nx + 7) / 8) * ((nz + 15) / 16);
     JT_REQUIRE(blocks < (1LL << 31), "grid too large");
     hipLaunchKernelGGL(k_kde_grid_cutoff, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, out);

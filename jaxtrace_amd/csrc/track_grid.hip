@@ -35,6 +35,12 @@ int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
     JT_REQUIRE(f->z_halo == 0 || f->z_halo == 1, "z_halo must be 0 or 1");
     long long nodes = (long long)f->nx * f->ny * (f->nz + f->z_halo);
     JT_REQUIRE(nodes * f->ncomp < (1LL << 31), "grid too large for 32-bit node offsets");
+    {   // the gather addresses one snapshot with 32-bit byte offsets
+        const long long eb = (f->dtype == JT_DTYPE_F32) ? 4 : 2;
+        if (nodes * f->ncomp * eb >= (1LL << 32))
+            return fail(JT_EUNSUPPORTED, "one snapshot is %lld bytes; at most 4 GiB per snapshot is supported",
+                        nodes * f->ncomp * eb);
+    }
     if (f->dtype == JT_DTYPE_F32) st = (f->ncomp == 3) ? ST_F32x3 : ST_F32x4;
     else if (f->dtype == JT_DTYPE_BF16) st = ST_BF16x4;
     else if (f->dtype == JT_DTYPE_F16) st = ST_F16x4;

@@ -249,9 +249,10 @@ class KDEEstimator:
     """kde.py:43-282: Gaussian KDE with Scott/Silverman bandwidth.
 
     ``cutoff_sigmas=None`` evaluates the reference's exhaustive all-pairs
-    sum.  A number c restricts ``evaluate_3d`` to points within c*h of each
-    grid point (hash grid with cell size c*h); every dropped term is below
-    (2pi)^(-3/2) exp(-c^2/2), and so is their mean."""
+    sum.  A number c makes ``evaluate_3d`` sum, for each grid point, every
+    particle within c*h of it (and some farther ones: those in the hash-grid
+    cells, of size c*h/2, around the point's block of the grid); every
+    dropped term is below (2pi)^(-3/2) exp(-c^2/2), and so is their mean."""
 
     def __init__(self, positions, bandwidth: Optional[float] = None, bandwidth_rule: str = "scott",
                  normalize: bool = True, plane: str = "xy", slab_position: float = 0.0,
@@ -393,7 +394,7 @@ class KDEEstimator:
             grid = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
             return torch.from_numpy(self._evaluate_grid(grid, self.P).reshape(ny, nx, nz))
         R = F32(float(self.cutoff_sigmas) * self.h)
-        hg = HashGridNeighbors(self._dev_points(self.P), float(R), 1, "x64", self.device)
+        hg = HashGridNeighbors(self._dev_points(self.P), float(R) / 2, 1, "x64", self.device)
         out = torch.empty((ny, nx, nz), dtype=torch.float32, device=self.device)
         gx, gy, gz = (_to_dev(a, self.device) for a in (x, y, z))
         L.check(lib.jt_kde_grid_cutoff(C.byref(hg._desc), self.N, L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,

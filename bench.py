@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--particles", type=int, default=N_PARTICLES)
     ap.add_argument("--steps-per-launch", type=int, default=50)
-    ap.add_argument("--sort-every", type=int, default=50)
+    ap.add_argument("--sort-every", type=int, default=25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)

@@ -184,12 +184,17 @@ __device__ __forceinline__ float sph_w(int kind, int dim, float r, float h) {
 // One wavefront per query.  mode 0: neighbour lists (idx, dist); mode 1: SPH.
 // LDS per wave: nc_pad 64-bit keys (d_sel bits << 32 | candidate) + nc_pad
 // candidate point indices, nc_pad = next pow2 of probes*cap.
+// SPH densities whose valid neighbours exceed K are marked in out_rho with
+// this NaN payload by k_sph_direct (arithmetic never produces it) and then
+// recomputed by k_query<1> with the top-K selection.
+#define JT_SPH_REDO 0x7fbadbadu
+
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_query(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K, int nc_pad,
         int* __restrict__ out_idx, float* __restrict__ out_dist,
         const float* __restrict__ masses, const float* __restrict__ hs, float hmean, int kern,
-        float* __restrict__ out_rho) {
+        float* __restrict__ out_rho, int redo_only) {
     extern __shared__ unsigned long long lds[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -198,6 +203,7 @@ k_query(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K, in
     int* cand = (int*)(lds + (size_t)waves * nc_pad) + (size_t)wv * nc_pad;
     const int qi = blockIdx.x * waves + wv;
     if (qi >= m) return;                       // whole wave exits together
+    if (MODE == 1 && redo_only && __float_as_uint(out_rho[qi]) != JT_SPH_REDO) return;
     const int D = g.dim;
     float q[3] = {0.f, 0.f, 0.f};
     for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
@@ -393,6 +399,59 @@ extern "C" int jt_hashgrid_build(const jt_hashgrid* h, const float* P, void* ws,
     return JT_OK;
 }
 
+// SPH density with no neighbour list: one wavefront per query sums m W(d, h)
+// over the valid candidates in the same per-lane order and with the same
+// wave reduction as k_query<1>, so where the valid candidates number at most
+// K (every one of them is among the K nearest) the value is k_query<1>'s.
+// Otherwise it writes JT_SPH_REDO for the top-K pass.  No LDS: occupancy is
+// bounded by registers alone.
+__global__ void __launch_bounds__(256)
+k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K,
+             const float* __restrict__ masses, const float* __restrict__ hs, float hmean, int kern,
+             float* __restrict__ out_rho) {
+    const int lane = threadIdx.x & 63;
+    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (qi >= m) return;                       // whole wave exits together
+    const int D = g.dim;
+    float q[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
+    int cq[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) cq[a] = cell_coord(q[a], g.origin[a], g.inv);
+    const int probes = (D == 3) ? 27 : 9;
+    const bool per_point = masses != nullptr || hs != nullptr;
+    int nvalid = 0;
+    float acc = 0.0f;
+    for (int p = 0; p < probes; ++p) {
+        int di = (D == 3) ? p / 9 - 1 : p / 3 - 1;
+        int dj = (D == 3) ? (p / 3) % 3 - 1 : p % 3 - 1;
+        int dk = (D == 3) ? p % 3 - 1 : 0;
+        int cell = (D == 3) ? probe_cell(g, cq[0] + di, cq[1] + dj, cq[2] + dk)
+                            : probe_cell(g, cq[0] + di, cq[1] + dj, 0);
+        if (cell < 0) continue;
+        const int st = g.cstart[cell], cnt = min(g.cend[cell] - st, cap);
+        for (int s = lane; s < cnt; s += 64) {
+            const float* pp = g.spos + (size_t)(st + s) * D;
+            float dx = pp[0] - q[0], dy = pp[1] - q[1];
+            float d2 = dx * dx + dy * dy;
+            if (D == 3) { float dz = pp[2] - q[2]; d2 = d2 + dz * dz; }
+            const float d = sqrtf(d2);
+            if (d <= r) {
+                ++nvalid;
+                float mass = 1.0f, h = hmean;
+                if (per_point) {
+                    const int pt = g.order[st + s];
+                    if (masses) mass = masses[pt];
+                    if (hs) h = hs[pt];
+                }
+                acc += mass * sph_w(kern, D, d, h);
+            }
+        }
+    }
+    nvalid = wave_sum_i(nvalid);
+    acc = wave_sum(acc);
+    if (lane == 0) out_rho[qi] = nvalid <= K ? acc : __uint_as_float(JT_SPH_REDO);
+}
+
 static int query_common(const jt_hashgrid* h, HGDev& g, int64_t m, int32_t cap, int32_t K, int& nc_pad, int& waves) {
     int rc = make_hg(h, g);
     if (rc) return rc;
@@ -422,7 +481,7 @@ extern "C" int jt_hashgrid_query(const jt_hashgrid* h, const float* Q, int64_t m
     size_t lds = (size_t)nc_pad * 12 * waves;
     hipLaunchKernelGGL(k_query<0>, dim3((unsigned)((m + waves - 1) / waves)), dim3(64 * waves), lds,
                        (hipStream_t)stream, g, Q, (int)m, radius, per_cell_cap, max_neighbors, nc_pad,
-                       out_idx, out_dist, nullptr, nullptr, 0.0f, 0, nullptr);
+                       out_idx, out_dist, nullptr, nullptr, 0.0f, 0, nullptr, 0);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }
@@ -492,10 +551,13 @@ extern "C" int jt_sph_eval(const jt_hashgrid* h, const float* Q, int64_t m, floa
     JT_REQUIRE(kernel >= JT_SPH_CUBIC && kernel <= JT_SPH_GAUSSIAN, "bad kernel %d", kernel);
     if (m == 0) return JT_OK;
     JT_REQUIRE(Q && out_rho, "NULL pointer");
+    hipLaunchKernelGGL(k_sph_direct, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, (hipStream_t)stream, g, Q,
+                       (int)m, radius, per_cell_cap, max_neighbors, masses, smoothing, h_mean, kernel, out_rho);
+    JT_HIP(hipGetLastError());
     size_t lds = (size_t)nc_pad * 12 * waves;
     hipLaunchKernelGGL(k_query<1>, dim3((unsigned)((m + waves - 1) / waves)), dim3(64 * waves), lds,
                        (hipStream_t)stream, g, Q, (int)m, radius, per_cell_cap, max_neighbors, nc_pad,
-                       nullptr, nullptr, masses, smoothing, h_mean, kernel, out_rho);
+                       nullptr, nullptr, masses, smoothing, h_mean, kernel, out_rho, 1);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

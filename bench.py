@@ -94,6 +94,9 @@ def run_chunks(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch,
 
 
 def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
+    """Warm-up, then the timed steps, in calls of spl steps (spl = the steps
+    per launch, capped at the sort interval so that one call is one cell sort
+    plus one tracking launch)."""
     dev = field.device
     n = x0.shape[0]
     total = args.warmup + args.steps
@@ -106,18 +109,22 @@ def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
     ws_bytes = int(lib.jt_track_grid_workspace_size(n, total, C.byref(o)))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = L.stream_handle()
+    spl = launch_steps(args, sort_every)
     x = x0.clone()
-    run_chunks(lib, L, desc, bc, x, step_times, args.warmup, args.steps_per_launch, ws, stream, torch, False,
-               sort_every)
+    run_chunks(lib, L, desc, bc, x, step_times, args.warmup, spl, ws, stream, torch, False, sort_every)
     torch.cuda.synchronize()
     x = x0.clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    durs = run_chunks(lib, L, desc, bc, x, step_times, args.steps, args.steps_per_launch, ws, stream, torch, True,
-                      sort_every)
+    durs = run_chunks(lib, L, desc, bc, x, step_times, args.steps, spl, ws, stream, torch, True, sort_every)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, durs, x
+
+
+def launch_steps(args, sort_every):
+    spl = args.steps_per_launch
+    return min(spl, sort_every) if sort_every > 0 else spl
 
 
 def cpu_baseline(host_data, times, grid, x0_host, n_part, n_steps):
@@ -406,7 +413,7 @@ def main():
     n = args.particles
     rate32 = n * args.steps / wall32
     ms_step = wall32 * 1e3 / args.steps
-    spl = args.steps_per_launch
+    spl = launch_steps(args, args.sort_every)
     full = [r for r in durs32 if r[2] == spl] or durs32
     avg_call_ms = float(np.mean([r[0] for r in full]))
     avg_launch_ms = float(np.mean([r[1] for r in full]))
@@ -443,8 +450,10 @@ def main():
     if os.path.exists(prof):
         try:
             tr = json.load(open(prof))
-            result["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
-            result["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+            # counters hold for one launch shape only
+            if tr.get("particles") == n and tr.get("steps_per_launch") == steps_per:
+                result["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
+                result["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
         except Exception:
             pass
 

@@ -55,7 +55,9 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
 #ifndef TILE_NODES
 #define TILE_NODES 1660            // 2 slots x 3 float64 per node + a spare: < 80 KiB, two blocks per CU
 #endif
+#ifndef ADJ_BLOCK
 #define ADJ_BLOCK 256
+#endif
 #ifndef ADJ_RESORT
 #define ADJ_RESORT 8
 #endif

@@ -173,38 +173,6 @@ __device__ __forceinline__ void load_pair(const void* base, int node, float a[6]
     }
 }
 
-// Blended values of nodes (node, node+1): same per-node arithmetic as corner().
-template <int ST>
-__device__ __forceinline__ void corner_pair(const void* b0, const void* b1, const Blend& bl,
-                                            float oma, int node, float v[3], float w[3]) {
-    if (bl.mode == BL_ZERO) { v[0] = v[1] = v[2] = w[0] = w[1] = w[2] = 0.0f; return; }
-    if (bl.mode == BL_NAN) {
-        v[0] = v[1] = v[2] = w[0] = w[1] = w[2] = __int_as_float(0x7fc00000);
-        return;
-    }
-    float a[6];
-    load_pair<ST>(b0, node, a);
-    if (bl.mode == BL_COPY) {
-        v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; w[0] = a[3]; w[1] = a[4]; w[2] = a[5];
-        return;
-    }
-    float c[6];
-    load_pair<ST>(b1, node, c);
-    const float al = bl.a;
-    float r[6];
-    if (bl.mode == BL_INTERP) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) r[k] = oma * a[k] + al * c[k];
-    } else if (bl.mode == BL_LEFT) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) r[k] = a[k] + al * (c[k] - a[k]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) r[k] = c[k] + al * (c[k] - a[k]);
-    }
-    v[0] = r[0]; v[1] = r[1]; v[2] = r[2]; w[0] = r[3]; w[1] = r[4]; w[2] = r[5];
-}
-
 // structured.py:176-220 for one axis.
 __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, int n,
                                            int& i0, int& i1, float& w, bool& valid) {
@@ -238,10 +206,6 @@ struct Corners {
     bool valid;
 };
 
-// The eight blended corners from node arrays b0/b1 (the two snapshots) with
-// y-row stride ny_ and z-row stride nzs_: the snapshots themselves, or a tile
-// of them staged in LDS (local indices and the tile's strides).
-// ZH: z-halo layout, the (k0, k0+1) corners are one contiguous pair.
 // Blend n node values of the two snapshots with the same per-value arithmetic
 // as corner(): (1-a) V0 + a V1 ('interp'), V0 + a (V1 - V0) / V1 + a (V1 - V0)
 // (linear extrapolation left / right of the series).  COPY, ZERO and NAN are
@@ -320,10 +284,9 @@ __device__ __forceinline__ void load_corners(const void* base, const unsigned ro
     }
 }
 
-// The eight blended corners from node arrays b0/b1 (the two snapshots) with
-// y-row stride ny_ and z-row stride nzs_: the snapshots themselves, or a tile
-// of them staged in LDS (local indices and the tile's strides).  The blend
-// mode is dispatched once, so all corner loads of a snapshot issue together.
+// The eight blended corners from snapshots b0/b1 (node arrays with y-row
+// stride ny_ and z-row stride nzs_).  The blend mode is dispatched once, so
+// all corner loads of both snapshots issue together.
 template <int ST, bool ZH>
 __device__ __forceinline__ void gather_from(const void* b0, const void* b1, const Blend& bl, int i0, int i1,
                                             int j0, int j1, int k0, int k1, int ny_, int nzs_, Corners& C) {

@@ -173,6 +173,24 @@ __device__ __forceinline__ void load_pair(const void* base, int node, float a[6]
     }
 }
 
+// The six values of a raw 16-bit node pair (node, node+1), 4 components each.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int ST>
+__device__ __forceinline__ void unpack_pair16(const u32x4 t, float a[6]) {
+    if constexpr (ST == ST_BF16x4) {
+        a[0] = bf16_to_f32((uint16_t)(t.x & 0xffffu)); a[1] = bf16_to_f32((uint16_t)(t.x >> 16));
+        a[2] = bf16_to_f32((uint16_t)(t.y & 0xffffu));
+        a[3] = bf16_to_f32((uint16_t)(t.z & 0xffffu)); a[4] = bf16_to_f32((uint16_t)(t.z >> 16));
+        a[5] = bf16_to_f32((uint16_t)(t.w & 0xffffu));
+    } else {
+        a[0] = f16_to_f32((uint16_t)(t.x & 0xffffu)); a[1] = f16_to_f32((uint16_t)(t.x >> 16));
+        a[2] = f16_to_f32((uint16_t)(t.y & 0xffffu));
+        a[3] = f16_to_f32((uint16_t)(t.z & 0xffffu)); a[4] = f16_to_f32((uint16_t)(t.z >> 16));
+        a[5] = f16_to_f32((uint16_t)(t.w & 0xffffu));
+    }
+}
+
 // structured.py:176-220 for one axis.
 __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, int n,
                                            int& i0, int& i1, float& w, bool& valid) {
@@ -306,9 +324,37 @@ __device__ __forceinline__ void gather_from(const void* b0, const void* b1, cons
     const unsigned k0b = (unsigned)k0 * NB, k1b = (unsigned)k1 * NB;
     // both snapshots' loads issue before any wait ('copy' reads V0 twice)
     float A[8][3], B[8][3];
-    load_corners<ST, ZH>(b0, row, k0b, k1b, A);
-    load_corners<ST, ZH>(bl.mode == BL_COPY ? b0 : b1, row, k0b, k1b, B);
-    __asm__ volatile("" ::: "memory");                 // keep the loads ahead of the mode branch
+    if constexpr (ZH && NodeBytes<ST>::v == 8u) {
+        // 16-bit z-pairs: the eight raw 16 B loads are all held until the
+        // asm below, so none waits for an earlier one's registers
+        const char* pa = (const char*)b0;
+        const char* pb = (const char*)(bl.mode == BL_COPY ? b0 : b1);
+        u32x4 ra[4], rb[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const U4 ta = *(const U4*)(pa + (row[r] + k0b));
+            const U4 tb = *(const U4*)(pb + (row[r] + k0b));
+            ra[r] = u32x4{ta.v[0], ta.v[1], ta.v[2], ta.v[3]};
+            rb[r] = u32x4{tb.v[0], tb.v[1], tb.v[2], tb.v[3]};
+        }
+        __asm__ volatile("" : "+v"(ra[0]), "+v"(ra[1]), "+v"(ra[2]), "+v"(ra[3]),
+                              "+v"(rb[0]), "+v"(rb[1]), "+v"(rb[2]), "+v"(rb[3]) :: "memory");
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float a[6], b[6];
+            unpack_pair16<ST>(ra[r], a);
+            unpack_pair16<ST>(rb[r], b);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                A[2 * r][c] = a[c]; A[2 * r + 1][c] = a[3 + c];
+                B[2 * r][c] = b[c]; B[2 * r + 1][c] = b[3 + c];
+            }
+        }
+    } else {
+        load_corners<ST, ZH>(b0, row, k0b, k1b, A);
+        load_corners<ST, ZH>(bl.mode == BL_COPY ? b0 : b1, row, k0b, k1b, B);
+        __asm__ volatile("" ::: "memory");             // keep the loads ahead of the mode branch
+    }
     if (bl.mode == BL_COPY) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) { C.v[q][0] = A[q][0]; C.v[q][1] = A[q][1]; C.v[q][2] = A[q][2]; }
@@ -368,7 +414,15 @@ __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
 // snapshots keep all 16 corner loads of a stage in flight without spilling at
 // 3 (168 VGPRs); 16-bit snapshots need half the registers per load and run
 // best at 4.
-template <int ST> struct TrackWaves { static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? 3 : 4; };
+#ifndef JT_WAVES_F32
+#define JT_WAVES_F32 3
+#endif
+#ifndef JT_WAVES_16
+#define JT_WAVES_16 4
+#endif
+template <int ST> struct TrackWaves {
+    static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_WAVES_F32 : JT_WAVES_16;
+};
 
 // ---- boundary epilogue (boundary.py) ----------------------------------------
 struct BcDev {
@@ -488,9 +542,11 @@ __device__ __forceinline__ void rk_step(F&& f, const Blend* __restrict__ tab, in
         f(b1, q, k);
 #pragma unroll
         for (int c = 0; c < 3; ++c) { acc[c] = acc[c] + 2.0f * k[c]; q[c] = p[c] + dth * k[c]; }
+        __asm__ volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]));   // sum now, not at the end
         f(b1, q, k);
 #pragma unroll
         for (int c = 0; c < 3; ++c) { acc[c] = acc[c] + 2.0f * k[c]; q[c] = p[c] + dt * k[c]; }
+        __asm__ volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]));
         f(b2, q, k);
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = p[c] + dt6 * (acc[c] + k[c]);   // rk4.py:44

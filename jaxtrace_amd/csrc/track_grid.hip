@@ -108,20 +108,43 @@ int make_bc_dev(const jt_boundary* b, BcDev& d) {
 
 // ---- kernels -------------------------------------------------------------
 
+// Where one tracking launch reads and writes its particles.  perm == nullptr:
+// in place (xin == xout), thread i owns slot i, whose caller index is idin[i]
+// (i without ids).  Otherwise thread i takes particle perm[i] of xin/idin (the
+// cell order just sorted) and leaves it in slot i of xout/idout, so the
+// re-ordering costs no pass of its own.
+struct Reorder {
+    const float* xin;
+    float* xout;
+    const int* idin;
+    int* idout;
+    const int* perm;
+};
 
-
+#ifndef JT_BLOCK_F32
+#define JT_BLOCK_F32 256
+#endif
+#ifndef JT_BLOCK_16
+#define JT_BLOCK_16 256
+#endif
+template <int ST> struct TrackBlock {
+    static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_BLOCK_F32 : JT_BLOCK_16;
+};
 
 template <int INTEG, int ST, bool ZH>
-__global__ void __launch_bounds__(256, TrackWaves<ST>::v)
-k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
+__global__ void __launch_bounds__(TrackBlock<ST>::v, TrackWaves<ST>::v)
+k_track(GridDev g, BcDev bc, Reorder ro, int n,
         const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
-        int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {
+        int rec_every, float* __restrict__ traj) {
     const int per = gridDim.x >> 3;                     // gridDim.x is a multiple of 8
     const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     const int i = lb * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
-    const size_t dst = ids ? (size_t)ids[i] : (size_t)i;
+    const int src = ro.perm ? ro.perm[i] : i;
+    float p[3] = {ro.xin[3 * (size_t)src], ro.xin[3 * (size_t)src + 1], ro.xin[3 * (size_t)src + 2]};
+    const int id = ro.idin ? ro.idin[src] : src;
+    if (ro.idout) ro.idout[i] = id;
+    const size_t dst = (size_t)id;
     auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
     for (int s = s0; s < s1; ++s) {
         rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);     // rk4.py:84-106 / rk2.py / euler.py
@@ -132,9 +155,9 @@ k_track(GridDev g, BcDev bc, float* __restrict__ xs, int n,
             o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
         }
     }
-    xs[3 * (size_t)i] = p[0];
-    xs[3 * (size_t)i + 1] = p[1];
-    xs[3 * (size_t)i + 2] = p[2];
+    ro.xout[3 * (size_t)i] = p[0];
+    ro.xout[3 * (size_t)i + 1] = p[1];
+    ro.xout[3 * (size_t)i + 2] = p[2];
 }
 
 template <int ST, bool ZH>
@@ -167,27 +190,47 @@ __device__ __forceinline__ unsigned cell_bits(float x, float o, float d, int n, 
     return (unsigned)c >> shift;
 }
 
+#ifndef JT_GRID_CURVE
+#define JT_GRID_CURVE 0
+#endif
+// Sort key of each particle's cell (coordinates >> shift, bits per axis):
+// Hilbert index (JT_GRID_CURVE 1) or Morton code (0).
 __global__ void __launch_bounds__(256)
-k_morton(GridDev g, const float* __restrict__ xs, int n, int shift, unsigned* __restrict__ keys,
-         int* __restrict__ perm) {
+k_cell_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int bits, unsigned* __restrict__ keys,
+            int* __restrict__ perm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned cx = cell_bits(xs[3 * (size_t)i], g.o[0], g.d[0], g.nx, shift);
     unsigned cy = cell_bits(xs[3 * (size_t)i + 1], g.o[1], g.d[1], g.ny, shift);
     unsigned cz = cell_bits(xs[3 * (size_t)i + 2], g.o[2], g.d[2], g.nz, shift);
+#if JT_GRID_CURVE
+    keys[i] = hilbert3(cx, cy, cz, bits);
+#else
     keys[i] = (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
+#endif
     perm[i] = i;
 }
 
+// One block of TrackBlock threads per particle range; the grid is padded to a
+// multiple of 8 for the kernel's XCD mapping.
+template <int INTEG, int ST, bool ZH>
+static void launch_track(hipStream_t s, const GridDev& g, const BcDev& bc, const Reorder& ro, int n,
+                         const Blend* tab, int s0, int s1, float dt, float dth, float dt6, int rec, float* traj) {
+    const unsigned B = TrackBlock<ST>::v;
+    const unsigned nb = ((unsigned)n + B - 1) / B;
+    hipLaunchKernelGGL((k_track<INTEG, ST, ZH>), dim3((nb + 7) / 8 * 8), dim3(B), 0, s, g, bc, ro, n, tab, s0, s1,
+                       dt, dth, dt6, rec, traj);
+}
+
 template <int INTEG, bool ZH>
-static void launch_track_st(int st, dim3 grid, dim3 block, hipStream_t s, const GridDev& g,
-                            const BcDev& bc, float* xs, int n, const Blend* tab, int s0, int s1,
-                            float dt, float dth, float dt6, int rec, float* traj, const int* ids) {
+static void launch_track_st(int st, hipStream_t s, const GridDev& g,
+                            const BcDev& bc, const Reorder& ro, int n, const Blend* tab, int s0, int s1,
+                            float dt, float dth, float dt6, int rec, float* traj) {
     switch (st) {
-    case ST_F32x3: hipLaunchKernelGGL((k_track<INTEG, ST_F32x3, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
-    case ST_F32x4: hipLaunchKernelGGL((k_track<INTEG, ST_F32x4, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
-    case ST_BF16x4: hipLaunchKernelGGL((k_track<INTEG, ST_BF16x4, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
-    default: hipLaunchKernelGGL((k_track<INTEG, ST_F16x4, ZH>), grid, block, 0, s, g, bc, xs, n, tab, s0, s1, dt, dth, dt6, rec, traj, ids); break;
+    case ST_F32x3: launch_track<INTEG, ST_F32x3, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    case ST_F32x4: launch_track<INTEG, ST_F32x4, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    case ST_BF16x4: launch_track<INTEG, ST_BF16x4, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    default: launch_track<INTEG, ST_F16x4, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
     }
 }
 
@@ -308,28 +351,29 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     const float dth = 0.5f * dt;       // rk4.py:90 (rk2.py:169 is dt*0.5, the same value)
     const float dt6 = dt / 6.0f;       // rk4.py:44
     const unsigned nb = (unsigned)((n + 255) / 256);
-    dim3 block(256), grid((nb + 7) / 8 * 8);
     dim3 eb(256), eg(nb);
     int maxn = g.nx > g.ny ? g.nx : g.ny;
     maxn = maxn > g.nz ? maxn : g.nz;
     int shift = 0;
-    while ((maxn - 1) >> shift >= 1024) ++shift;       // 10 bits per axis
+    while ((maxn - 1) >> shift >= 1024) ++shift;       // at most 10 bits per axis
+    int bits = 1;
+    while (((maxn - 1) >> shift) >= (1 << bits)) ++bits;
     float* cur = x;
     const int* cur_ids = nullptr;
     const int n_ev = (opts && opts->launch_events) ? opts->n_launch_events : 0;
     int launch = 0;
     for (int s0 = 0; s0 < n_steps;) {
         int s1 = s0 + spl < n_steps ? s0 + spl : n_steps;
+        Reorder ro{cur, cur, cur_ids, nullptr, nullptr};
         if (sort_every > 0) {
             int next_sort = (s0 / sort_every + 1) * sort_every;
             if (s1 > next_sort) s1 = next_sort;
             if (s0 % sort_every == 0) {
-                hipLaunchKernelGGL(k_morton, eg, eb, 0, s, g, cur, (int)n, shift, keys, perm);
-                JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0, 30, s));
+                hipLaunchKernelGGL(k_cell_keys, eg, eb, 0, s, g, cur, (int)n, shift, bits, keys, perm);
+                JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0, 3 * bits, s));
                 float* nx = (cur == xa) ? xb : xa;
                 int* nid = (cur_ids == ids) ? ids2 : ids;
-                hipLaunchKernelGGL(k_permute, eg, eb, 0, s, cur, cur_ids, sperm, (int)n, nx, nid);
-                JT_HIP(hipGetLastError());
+                ro = Reorder{cur, nx, cur_ids, nid, sperm};   // the launch below applies the permutation
                 cur = nx;
                 cur_ids = nid;
             }
@@ -338,16 +382,16 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
         if (2 * launch + 1 < n_ev) JT_HIP(hipEventRecord((hipEvent_t)opts->launch_events[2 * launch], s));
         switch (integrator) {
         case JT_RK4:
-            if (zh) launch_track_st<JT_RK4, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
-            else launch_track_st<JT_RK4, false>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            if (zh) launch_track_st<JT_RK4, true>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            else launch_track_st<JT_RK4, false>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
             break;
         case JT_RK2:
-            if (zh) launch_track_st<JT_RK2, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
-            else launch_track_st<JT_RK2, false>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            if (zh) launch_track_st<JT_RK2, true>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            else launch_track_st<JT_RK2, false>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
             break;
         default:
-            if (zh) launch_track_st<JT_EULER, true>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
-            else launch_track_st<JT_EULER, false>(st, grid, block, s, g, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+            if (zh) launch_track_st<JT_EULER, true>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            else launch_track_st<JT_EULER, false>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
             break;
         }
         JT_HIP(hipGetLastError());

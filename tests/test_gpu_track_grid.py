@@ -220,6 +220,26 @@ def test_cell_sorting_is_invisible(gpu, layout):
     np.testing.assert_array_equal(a.positions, b.positions)
 
 
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_cell_sorting_odd_grid_final_only(gpu, storage):
+    """Cell sort on a 45 x 19 x 70 grid (keys of 7 bits per axis), launches
+    longer than the sort interval, final positions only: the result equals
+    the unsorted run bitwise and comes back in the caller's order."""
+    rng = np.random.default_rng(31)
+    gx, gy, gz = (np.linspace(0, 1, m, dtype=F32) for m in (45, 19, 70))
+    times = np.array([0.0, 0.5, 1.0], F32)
+    data = (0.3 * rng.standard_normal((3, 45, 19, 70, 3))).astype(F32)
+    f = jt.StructuredTimeSeriesField(data, times, gx, gy, gz, bounds_handling="clamp", storage=storage,
+                                     layout="halo")
+    bc = jt.reflective_boundary([[0, 0, 0], [1, 1, 1]])
+    x0 = W.uniform_particles(70_001, 5)
+    a = jt.create_tracker("rk4", f, bc, record_final_only=True, sort_every=0).track_particles(x0, (0, 1), 33)
+    for se, spl in ((3, 10), (8, 8), (5, 1)):
+        b = jt.create_tracker("rk4", f, bc, record_final_only=True, sort_every=se,
+                              steps_per_launch=spl).track_particles(x0, (0, 1), 33)
+        np.testing.assert_array_equal(a.positions, b.positions)
+
+
 def test_time_series_field_nearest_node(gpu):
     """TimeSeriesField (time_series.py:82-473): blend every node, take the
     nearest node's value; exhaustive argmin with ties to the lowest index."""

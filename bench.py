@@ -56,47 +56,53 @@ def parse():
     return ap.parse_args()
 
 
-def run_chunks(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events, sort_every=0):
-    """Launch jt_track_grid over [0, n_steps) in chunks of spl steps; returns
-    per-launch event durations (ms) when record_events."""
-    durs = []
-    evs = []
+def launch_ranges(n_steps, spl, sort_every):
+    """The [s0, s1) step ranges of the tracking launches one jt_track_grid
+    call makes: spl steps at most, cut where a cell sort is due."""
+    out, s0 = [], 0
+    while s0 < n_steps:
+        s1 = min(s0 + spl, n_steps)
+        if sort_every > 0:
+            s1 = min(s1, (s0 // sort_every + 1) * sort_every)
+        out.append((s0, s1))
+        s0 = s1
+    return out
+
+
+def run_tracking(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events, sort_every=0):
+    """One jt_track_grid call over [0, n_steps), as track_particles makes it
+    (launches of up to spl steps, a cell sort every sort_every steps).  With
+    record_events, returns (call_ms / launch, launch_ms, steps) per launch: the
+    library records a pair of HIP events right around each tracking-kernel
+    launch, on the stream it launches on."""
     o = L.TrackOpts()
     o.steps_per_launch = spl
     o.sort_every = sort_every
     o.block = 0
-    s0 = 0
-    while s0 < n_steps:
-        k = min(spl, n_steps - s0)
-        st = C.c_void_p(step_times.data_ptr() + 4 * s0)
-        if record_events:
-            # e0/e1 bracket the whole call (cell sort + tracking kernel); the
-            # library records k0/k1 right around the tracking-kernel launch
-            e0, e1, k0, k1 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
-            k0.record()
-            k1.record()
-            evarr = (C.c_void_p * 2)(k0.cuda_event, k1.cuda_event)
-            o.launch_events = C.cast(evarr, C.POINTER(C.c_void_p))
-            o.n_launch_events = 2
-            e0.record()
-        rc = lib.jt_track_grid(C.byref(desc), C.byref(bc), L.JT_RK4, L.ptr(x), x.shape[0], st,
-                               C.c_float(np.float32(DT)), k, 1, None, C.byref(o), L.ptr(ws),
-                               ws.numel(), stream)
-        L.check(rc, "jt_track_grid")
-        if record_events:
-            e1.record()
-            evs.append((e0, e1, k0, k1, k))
-        s0 += k
+    ranges = launch_ranges(n_steps, spl, sort_every)
     if record_events:
-        torch.cuda.synchronize()
-        durs = [(a.elapsed_time(b), ka.elapsed_time(kb), k) for a, b, ka, kb, k in evs]
-    return durs
+        kev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * len(ranges))]
+        for e in kev:
+            e.record()                     # creates the underlying HIP event
+        evarr = (C.c_void_p * len(kev))(*[e.cuda_event for e in kev])
+        o.launch_events = C.cast(evarr, C.POINTER(C.c_void_p))
+        o.n_launch_events = len(kev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+    rc = lib.jt_track_grid(C.byref(desc), C.byref(bc), L.JT_RK4, L.ptr(x), x.shape[0], L.ptr(step_times),
+                           C.c_float(np.float32(DT)), n_steps, 1, None, C.byref(o), L.ptr(ws), ws.numel(), stream)
+    L.check(rc, "jt_track_grid")
+    if not record_events:
+        return []
+    e1.record()
+    torch.cuda.synchronize()
+    per_call = e0.elapsed_time(e1) / max(len(ranges), 1)
+    return [(per_call, kev[2 * k].elapsed_time(kev[2 * k + 1]), b - a) for k, (a, b) in enumerate(ranges)]
 
 
 def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
-    """Warm-up, then the timed steps, in calls of spl steps (spl = the steps
-    per launch, capped at the sort interval so that one call is one cell sort
-    plus one tracking launch)."""
+    """Warm-up, then the timed steps, each one jt_track_grid call (launches of
+    spl steps: the steps per launch, capped at the sort interval)."""
     dev = field.device
     n = x0.shape[0]
     total = args.warmup + args.steps
@@ -111,12 +117,12 @@ def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
     stream = L.stream_handle()
     spl = launch_steps(args, sort_every)
     x = x0.clone()
-    run_chunks(lib, L, desc, bc, x, step_times, args.warmup, spl, ws, stream, torch, False, sort_every)
+    run_tracking(lib, L, desc, bc, x, step_times, args.warmup, spl, ws, stream, torch, False, sort_every)
     torch.cuda.synchronize()
     x = x0.clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    durs = run_chunks(lib, L, desc, bc, x, step_times, args.steps, spl, ws, stream, torch, True, sort_every)
+    durs = run_tracking(lib, L, desc, bc, x, step_times, args.steps, spl, ws, stream, torch, True, sort_every)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, durs, x

@@ -16,6 +16,7 @@
 // XCD's L2 sees one contiguous (spatially compact, once sorted) particle range.
 #include "grid_sampler.h"
 #include "cell_sort.h"
+#include <algorithm>
 #include <cmath>
 
 namespace jt {
@@ -190,26 +191,65 @@ __device__ __forceinline__ unsigned cell_bits(float x, float o, float d, int n, 
     return (unsigned)c >> shift;
 }
 
-#ifndef JT_GRID_CURVE
-#define JT_GRID_CURVE 0
-#endif
-// Sort key of each particle's cell (coordinates >> shift, bits per axis):
-// Hilbert index (JT_GRID_CURVE 1) or Morton code (0).
+// Bit layout of a cell's sort key: key bit p is bit bit[p] of axis axis[p]
+// (0 x, 1 y, 2 z) of the cell index >> shift.
+struct KeyBits {
+    unsigned char axis[30], bit[30];
+    int nbits, shift;
+};
+
 __global__ void __launch_bounds__(256)
-k_cell_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int bits, unsigned* __restrict__ keys,
+k_cell_keys(GridDev g, const float* __restrict__ xs, int n, KeyBits kb, unsigned* __restrict__ keys,
             int* __restrict__ perm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    unsigned cx = cell_bits(xs[3 * (size_t)i], g.o[0], g.d[0], g.nx, shift);
-    unsigned cy = cell_bits(xs[3 * (size_t)i + 1], g.o[1], g.d[1], g.ny, shift);
-    unsigned cz = cell_bits(xs[3 * (size_t)i + 2], g.o[2], g.d[2], g.nz, shift);
-#if JT_GRID_CURVE
-    keys[i] = hilbert3(cx, cy, cz, bits);
-#else
-    keys[i] = (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
-#endif
+    const unsigned cx = cell_bits(xs[3 * (size_t)i], g.o[0], g.d[0], g.nx, kb.shift);
+    const unsigned cy = cell_bits(xs[3 * (size_t)i + 1], g.o[1], g.d[1], g.ny, kb.shift);
+    const unsigned cz = cell_bits(xs[3 * (size_t)i + 2], g.o[2], g.d[2], g.nz, kb.shift);
+    unsigned key = 0;
+#pragma unroll
+    for (int p = 0; p < 30; ++p) {                      // constant offsets: the layout sits in SGPRs
+        if (p < kb.nbits) {
+            const unsigned c = kb.axis[p] == 0 ? cx : (kb.axis[p] == 1 ? cy : cz);
+            key |= ((c >> kb.bit[p]) & 1u) << p;
+        }
+    }
+    keys[i] = key;
     perm[i] = i;
 }
+
+// Z-order (Morton) keys over the grid's cells with each axis's own bit count
+// (at most 10 bits per axis: coarser cells beyond 1024 nodes), preceded by
+// an optional low block of lz z-bits, ly y-bits and lx x-bits.  Bits follow
+// z, y, x from the least significant end, so 64 consecutive keys are a
+// 4 x 4 x 4 cell cube with no low block, or a 2 x 2 x 16 column with
+// (lz, ly, lx) = (4, 1, 1).
+static KeyBits make_key_bits(const GridDev& g, int lz, int ly, int lx) {
+    KeyBits kb{};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const int maxn = std::max(n[0], std::max(n[1], n[2]));
+    kb.shift = 0;
+    while ((maxn - 1) >> kb.shift >= 1024) ++kb.shift;
+    int nb[3], used[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+        nb[a] = 0;
+        while (((n[a] - 1) >> kb.shift) >= (1 << nb[a])) ++nb[a];
+    }
+    const int low[3] = {lx, ly, lz};
+    int p = 0;
+    for (int a = 2; a >= 0; --a)
+        for (int k = 0; k < low[a] && used[a] < nb[a]; ++k) { kb.axis[p] = (unsigned char)a; kb.bit[p] = (unsigned char)used[a]++; ++p; }
+    while (used[0] < nb[0] || used[1] < nb[1] || used[2] < nb[2])
+        for (int a = 2; a >= 0; --a)
+            if (used[a] < nb[a]) { kb.axis[p] = (unsigned char)a; kb.bit[p] = (unsigned char)used[a]++; ++p; }
+    if (p == 0) { kb.axis[0] = 0; kb.bit[0] = 0; p = 1; }   // one cell: all keys 0
+    kb.nbits = p;
+    return kb;
+}
+
+#ifndef JT_KEY_LOW
+#define JT_KEY_LOW 3, 2, 2
+#endif
 
 // One block of TrackBlock threads per particle range; the grid is padded to a
 // multiple of 8 for the kernel's XCD mapping.
@@ -352,12 +392,7 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     const float dt6 = dt / 6.0f;       // rk4.py:44
     const unsigned nb = (unsigned)((n + 255) / 256);
     dim3 eb(256), eg(nb);
-    int maxn = g.nx > g.ny ? g.nx : g.ny;
-    maxn = maxn > g.nz ? maxn : g.nz;
-    int shift = 0;
-    while ((maxn - 1) >> shift >= 1024) ++shift;       // at most 10 bits per axis
-    int bits = 1;
-    while (((maxn - 1) >> shift) >= (1 << bits)) ++bits;
+    const KeyBits kb = make_key_bits(g, JT_KEY_LOW);
     float* cur = x;
     const int* cur_ids = nullptr;
     const int n_ev = (opts && opts->launch_events) ? opts->n_launch_events : 0;
@@ -369,8 +404,8 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
             int next_sort = (s0 / sort_every + 1) * sort_every;
             if (s1 > next_sort) s1 = next_sort;
             if (s0 % sort_every == 0) {
-                hipLaunchKernelGGL(k_cell_keys, eg, eb, 0, s, g, cur, (int)n, shift, bits, keys, perm);
-                JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0, 3 * bits, s));
+                hipLaunchKernelGGL(k_cell_keys, eg, eb, 0, s, g, cur, (int)n, kb, keys, perm);
+                JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0, kb.nbits, s));
                 float* nx = (cur == xa) ? xb : xa;
                 int* nid = (cur_ids == ids) ? ids2 : ids;
                 ro = Reorder{cur, nx, cur_ids, nid, sperm};   // the launch below applies the permutation

@@ -84,7 +84,8 @@ typedef struct jt_grid_field {
     int32_t n_snap;
     int32_t nx, ny, nz;
     int32_t dtype;            /* JT_DTYPE_*                                    */
-    int32_t ncomp;            /* 3 (packed xyz) or 4 (xyz + pad)               */
+    int32_t ncomp;            /* 3 (packed xyz) or 4: float32 (x, y, z, pad);
+                                 16-bit (x, y, 0, z), the third half zero      */
     int32_t z_halo;           /* 1: each snapshot holds nz+1 z-planes; plane nz
                                  repeats plane 0 (periodic bounds) or plane
                                  nz-1 (other bounds), so the two z-corners of a

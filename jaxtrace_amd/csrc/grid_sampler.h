@@ -7,7 +7,10 @@
 
 namespace jt {
 
-// Snapshot storage layouts (element type x components per node).
+// Snapshot storage layouts (element type x components per node).  A 16-bit
+// node is four halves (x, y, 0, z): z is the high half of the second dword
+// and the low half is zero, so for bfloat16 that dword is z's float32 value
+// with no conversion.
 enum StoreKind { ST_F32x3 = 0, ST_F32x4 = 1, ST_BF16x4 = 2, ST_F16x4 = 3 };
 
 // Result of the scalar half of time_series.py:155-245 for one stage time.
@@ -102,12 +105,12 @@ __device__ __forceinline__ void load_node(const void* base, int node, float& a, 
         uint2 v = ((const uint2*)base)[node];
         a = bf16_to_f32((uint16_t)(v.x & 0xffffu));
         b = bf16_to_f32((uint16_t)(v.x >> 16));
-        c = bf16_to_f32((uint16_t)(v.y & 0xffffu));
+        c = bf16_hi_to_f32(v.y);
     } else {
         uint2 v = ((const uint2*)base)[node];
         a = f16_to_f32((uint16_t)(v.x & 0xffffu));
         b = f16_to_f32((uint16_t)(v.x >> 16));
-        c = f16_to_f32((uint16_t)(v.y & 0xffffu));
+        c = f16_to_f32((uint16_t)(v.y >> 16));
     }
 }
 
@@ -142,6 +145,24 @@ __device__ __forceinline__ void corner(const void* b0, const void* b1, const Ble
     }
 }
 
+// The six values of a raw 16-bit node pair (node, node+1), 4 components each.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int ST>
+__device__ __forceinline__ void unpack_pair16(const u32x4 t, float a[6]) {
+    if constexpr (ST == ST_BF16x4) {
+        a[0] = bf16_to_f32((uint16_t)(t.x & 0xffffu)); a[1] = bf16_to_f32((uint16_t)(t.x >> 16));
+        a[2] = bf16_hi_to_f32(t.y);
+        a[3] = bf16_to_f32((uint16_t)(t.z & 0xffffu)); a[4] = bf16_to_f32((uint16_t)(t.z >> 16));
+        a[5] = bf16_hi_to_f32(t.w);
+    } else {
+        a[0] = f16_to_f32((uint16_t)(t.x & 0xffffu)); a[1] = f16_to_f32((uint16_t)(t.x >> 16));
+        a[2] = f16_to_f32((uint16_t)(t.y >> 16));
+        a[3] = f16_to_f32((uint16_t)(t.z & 0xffffu)); a[4] = f16_to_f32((uint16_t)(t.z >> 16));
+        a[5] = f16_to_f32((uint16_t)(t.w >> 16));
+    }
+}
+
 // Two z-adjacent nodes (node, node+1) in one contiguous load (z-halo layout).
 struct alignas(4) F6 { float v[6]; };
 struct alignas(4) F8 { float v[8]; };
@@ -159,35 +180,7 @@ __device__ __forceinline__ void load_pair(const void* base, int node, float a[6]
         a[3] = t.v[4]; a[4] = t.v[5]; a[5] = t.v[6];
     } else {
         U4 t = *(const U4*)((const char*)base + (size_t)node * 8);
-        if constexpr (ST == ST_BF16x4) {
-            a[0] = bf16_to_f32((uint16_t)(t.v[0] & 0xffffu)); a[1] = bf16_to_f32((uint16_t)(t.v[0] >> 16));
-            a[2] = bf16_to_f32((uint16_t)(t.v[1] & 0xffffu));
-            a[3] = bf16_to_f32((uint16_t)(t.v[2] & 0xffffu)); a[4] = bf16_to_f32((uint16_t)(t.v[2] >> 16));
-            a[5] = bf16_to_f32((uint16_t)(t.v[3] & 0xffffu));
-        } else {
-            a[0] = f16_to_f32((uint16_t)(t.v[0] & 0xffffu)); a[1] = f16_to_f32((uint16_t)(t.v[0] >> 16));
-            a[2] = f16_to_f32((uint16_t)(t.v[1] & 0xffffu));
-            a[3] = f16_to_f32((uint16_t)(t.v[2] & 0xffffu)); a[4] = f16_to_f32((uint16_t)(t.v[2] >> 16));
-            a[5] = f16_to_f32((uint16_t)(t.v[3] & 0xffffu));
-        }
-    }
-}
-
-// The six values of a raw 16-bit node pair (node, node+1), 4 components each.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-template <int ST>
-__device__ __forceinline__ void unpack_pair16(const u32x4 t, float a[6]) {
-    if constexpr (ST == ST_BF16x4) {
-        a[0] = bf16_to_f32((uint16_t)(t.x & 0xffffu)); a[1] = bf16_to_f32((uint16_t)(t.x >> 16));
-        a[2] = bf16_to_f32((uint16_t)(t.y & 0xffffu));
-        a[3] = bf16_to_f32((uint16_t)(t.z & 0xffffu)); a[4] = bf16_to_f32((uint16_t)(t.z >> 16));
-        a[5] = bf16_to_f32((uint16_t)(t.w & 0xffffu));
-    } else {
-        a[0] = f16_to_f32((uint16_t)(t.x & 0xffffu)); a[1] = f16_to_f32((uint16_t)(t.x >> 16));
-        a[2] = f16_to_f32((uint16_t)(t.y & 0xffffu));
-        a[3] = f16_to_f32((uint16_t)(t.z & 0xffffu)); a[4] = f16_to_f32((uint16_t)(t.z >> 16));
-        a[5] = f16_to_f32((uint16_t)(t.w & 0xffffu));
+        unpack_pair16<ST>(u32x4{t.v[0], t.v[1], t.v[2], t.v[3]}, a);
     }
 }
 
@@ -215,6 +208,39 @@ __device__ __forceinline__ void axis_coord(const GridDev& g, int ax, float x, in
     w = f - (float)ii;
     i0 = min(max(ii, 0), n - 1);                       // gather index clamp
     i1 = min(max(jj, 0), n - 1);
+}
+
+// float -> int32 by v_cvt_i32_f32, which returns 0 for NaN: the value of
+// axis_coord's "(fl != fl) ? 0 : (int)fl" for every float below 2^31.
+__device__ __forceinline__ int cvt_i32_nan0(float f) {
+    int r;
+    __asm__("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+}
+
+// axis_coord for a periodic sampler whose spacing has an exact reciprocal
+// (inv != 0) on an axis of n >= 2 nodes, with the same values for every
+// input: f = jnp.mod(u * inv, n) lies in [0, n] (the last by rounding) or is
+// NaN, so floor(f) needs no lower clamp and (floor(f) + 1) mod n no clamp at
+// all; jnp.mod's three exact cases are selected without branches and only
+// |f| >= 2n or NaN takes the general path.
+__device__ __forceinline__ void axis_coord_pf(float x, float o, float inv, int n, float fn, int& i0, int& i1,
+                                              float& w) {
+    const float a = (x - o) * inv;
+    float f;
+    if (a > -fn && a < 2.0f * fn) {
+        const float lo = a + fn, hi = a - fn;
+        f = a < 0.0f ? lo : a;
+        f = a >= fn ? hi : f;
+    } else {
+        f = jmodf(a, fn);
+    }
+    const int ii = cvt_i32_nan0(floorf(f));
+    int jj = ii + 1;
+    jj = (jj >= n) ? jj - n : jj;
+    w = f - (float)ii;
+    i0 = min(ii, n - 1);
+    i1 = jj;
 }
 
 // The eight blended corner values around a point and its trilinear weights.
@@ -305,7 +331,10 @@ __device__ __forceinline__ void load_corners(const void* base, const unsigned ro
 // The eight blended corners from snapshots b0/b1 (node arrays with y-row
 // stride ny_ and z-row stride nzs_).  The blend mode is dispatched once, so
 // all corner loads of both snapshots issue together.
-template <int ST, bool ZH>
+// M24: the row offsets by 24-bit multiplies (full rate), valid when
+// nx * ny and nzs * node bytes are below 2^24 (the periodic fast path checks
+// it on the host).
+template <int ST, bool ZH, bool M24 = false>
 __device__ __forceinline__ void gather_from(const void* b0, const void* b1, const Blend& bl, int i0, int i1,
                                             int j0, int j1, int k0, int k1, int ny_, int nzs_, Corners& C) {
     if (bl.mode == BL_ZERO || bl.mode == BL_NAN) {
@@ -316,11 +345,20 @@ __device__ __forceinline__ void gather_from(const void* b0, const void* b1, cons
     }
     // 32-bit byte offsets (a snapshot is below 4 GiB, checked on the host)
     const unsigned NB = NodeBytes<ST>::v;
-    const unsigned rs = (unsigned)nzs_ * NB, ps = (unsigned)ny_ * rs;
-    const unsigned oi0 = (unsigned)i0 * ps, oi1 = (unsigned)i1 * ps;
-    const unsigned oj0 = (unsigned)j0 * rs, oj1 = (unsigned)j1 * rs;
-    // rows in corner order (ci, cj): (0,0) (0,1) (1,0) (1,1)
-    const unsigned row[4] = {oi0 + oj0, oi0 + oj1, oi1 + oj0, oi1 + oj1};
+    const unsigned rs = (unsigned)nzs_ * NB;
+    unsigned row[4];                                    // rows in corner order (ci, cj): (0,0) (0,1) (1,0) (1,1)
+    if constexpr (M24) {
+        const unsigned r0 = __umul24((unsigned)i0, (unsigned)ny_), r1 = __umul24((unsigned)i1, (unsigned)ny_);
+        row[0] = __umul24(r0 + (unsigned)j0, rs);
+        row[1] = __umul24(r0 + (unsigned)j1, rs);
+        row[2] = __umul24(r1 + (unsigned)j0, rs);
+        row[3] = __umul24(r1 + (unsigned)j1, rs);
+    } else {
+        const unsigned ps = (unsigned)ny_ * rs;
+        const unsigned oi0 = (unsigned)i0 * ps, oi1 = (unsigned)i1 * ps;
+        const unsigned oj0 = (unsigned)j0 * rs, oj1 = (unsigned)j1 * rs;
+        row[0] = oi0 + oj0; row[1] = oi0 + oj1; row[2] = oi1 + oj0; row[3] = oi1 + oj1;
+    }
     const unsigned k0b = (unsigned)k0 * NB, k1b = (unsigned)k1 * NB;
     // both snapshots' loads issue before any wait ('copy' reads V0 twice)
     float A[8][3], B[8][3];
@@ -369,15 +407,23 @@ __device__ __forceinline__ const void* snap_base(const GridDev& g, int snap) {
 }
 
 // Gather half of the trilinear sample (structured.py:176-232 after the blend).
-template <int ST, bool ZH = false>
+// PF: the periodic fast path (periodic bounds, exact reciprocal spacing on
+// every axis, every axis at least 2 nodes), chosen on the host.
+template <int ST, bool ZH = false, bool PF = false>
 __device__ __forceinline__ void gather_corners(const GridDev& g, const Blend& bl,
                                                float px, float py, float pz, Corners& C) {
     int i0, i1, j0, j1, k0, k1;
     C.valid = true;
-    axis_coord(g, 0, px, g.nx, i0, i1, C.wx, C.valid);
-    axis_coord(g, 1, py, g.ny, j0, j1, C.wy, C.valid);
-    axis_coord(g, 2, pz, g.nz, k0, k1, C.wz, C.valid);
-    gather_from<ST, ZH>(snap_base<ST>(g, bl.i0), snap_base<ST>(g, bl.i1), bl, i0, i1, j0, j1, k0, k1, g.ny,
+    if constexpr (PF) {
+        axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
+        axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
+        axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
+    } else {
+        axis_coord(g, 0, px, g.nx, i0, i1, C.wx, C.valid);
+        axis_coord(g, 1, py, g.ny, j0, j1, C.wy, C.valid);
+        axis_coord(g, 2, pz, g.nz, k0, k1, C.wz, C.valid);
+    }
+    gather_from<ST, ZH, PF>(snap_base<ST>(g, bl.i0), snap_base<ST>(g, bl.i1), bl, i0, i1, j0, j1, k0, k1, g.ny,
                         g.nzs, C);
 }
 
@@ -402,23 +448,137 @@ __device__ __forceinline__ void lerp_corners(const GridDev& g, const Corners& C,
 }
 
 // Blended trilinear sample u(x, t) (structured.py:167-250 after the blend).
-template <int ST, bool ZH = false>
+template <int ST, bool ZH = false, bool PF = false>
 __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
                                        float px, float py, float pz, float out[3]) {
     Corners C;
-    gather_corners<ST, ZH>(g, bl, px, py, pz, C);
+    gather_corners<ST, ZH, PF>(g, bl, px, py, pz, C);
     lerp_corners(g, C, out);
 }
 
-// Waves per SIMD the tracking kernels' register budget is sized for: float32
-// snapshots keep all 16 corner loads of a stage in flight without spilling at
-// 3 (168 VGPRs); 16-bit snapshots need half the registers per load and run
-// best at 4.
+// ---- corner reuse across stages (z-halo layout, periodic fast path) ------
+// Consecutive stage points of one particle mostly stay in one cell, and the
+// stored snapshots do not change during a launch, so a lane keeps the raw
+// corner records of its last sample in registers and reloads them only when
+// its cell (i0, i1, j0, j1, k0) or the snapshot pair changes.  The reloads
+// are exec-masked loads: lanes that reuse issue no memory requests, which is
+// the vector-memory address/data work the tracking kernel is bound by.  The
+// values used are the same stored values, so results are unchanged.
+template <int ST, bool W16 = (NodeBytes<ST>::v == 8u)>
+struct CornerCache {                 // float32 snapshots: the corner values
+    float A[8][3], B[8][3];
+    int i0, i1, j0, j1, k0;
+    const char* pa;
+    const char* pb;
+};
+template <int ST>
+struct CornerCache<ST, true> {       // 16-bit snapshots: the raw z-pair records
+    u32x4 ra[4], rb[4];
+    int i0, i1, j0, j1, k0;
+    const char* pa;
+    const char* pb;
+};
+
+template <int ST>
+__device__ __forceinline__ void cache_reset(CornerCache<ST>& cc) {
+    cc.i0 = -1; cc.i1 = -1; cc.j0 = -1; cc.j1 = -1; cc.k0 = -1;
+    cc.pa = nullptr; cc.pb = nullptr;
+}
+
+// sample<ST, true, true> through a CornerCache.
+template <int ST>
+__device__ __forceinline__ void sample_cached(const GridDev& g, const Blend& bl, float px, float py, float pz,
+                                              float out[3], CornerCache<ST>& cc) {
+    Corners C;
+    C.valid = true;
+    if (bl.mode == BL_ZERO || bl.mode == BL_NAN) {
+        const float z = bl.mode == BL_ZERO ? 0.0f : __int_as_float(0x7fc00000);
+        int i0, i1, j0, j1, k0, k1;
+        axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
+        axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
+        axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) C.v[q][0] = C.v[q][1] = C.v[q][2] = z;
+        lerp_corners(g, C, out);
+        return;
+    }
+    int i0, i1, j0, j1, k0, k1;
+    axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
+    axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
+    axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
+    const char* pa = (const char*)snap_base<ST>(g, bl.i0);
+    const char* pb = bl.mode == BL_COPY ? pa : (const char*)snap_base<ST>(g, bl.i1);
+    const bool same = pa == cc.pa && pb == cc.pb && i0 == cc.i0 && i1 == cc.i1 && j0 == cc.j0 && j1 == cc.j1 &&
+                      k0 == cc.k0;
+    if (!same) {
+        const unsigned NB = NodeBytes<ST>::v;
+        const unsigned rs = (unsigned)g.nzs * NB;
+        const unsigned r0 = (unsigned)__umul24((unsigned)i0, (unsigned)g.ny);
+        const unsigned r1 = (unsigned)__umul24((unsigned)i1, (unsigned)g.ny);
+        const unsigned row[4] = {(unsigned)__umul24(r0 + (unsigned)j0, rs), (unsigned)__umul24(r0 + (unsigned)j1, rs),
+                                 (unsigned)__umul24(r1 + (unsigned)j0, rs), (unsigned)__umul24(r1 + (unsigned)j1, rs)};
+        const unsigned k0b = (unsigned)k0 * NB;
+        if constexpr (NodeBytes<ST>::v == 8u) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const U4 ta = *(const U4*)(pa + (row[r] + k0b));
+                const U4 tb = *(const U4*)(pb + (row[r] + k0b));
+                cc.ra[r] = u32x4{ta.v[0], ta.v[1], ta.v[2], ta.v[3]};
+                cc.rb[r] = u32x4{tb.v[0], tb.v[1], tb.v[2], tb.v[3]};
+            }
+        } else {
+            load_corners<ST, true>(pa, row, k0b, 0u, cc.A);
+            load_corners<ST, true>(pb, row, k0b, 0u, cc.B);
+        }
+        cc.i0 = i0; cc.i1 = i1; cc.j0 = j0; cc.j1 = j1; cc.k0 = k0;
+    }
+    cc.pa = pa;
+    cc.pb = pb;
+    float A[8][3], B[8][3];
+    if constexpr (NodeBytes<ST>::v == 8u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float a[6], b[6];
+            unpack_pair16<ST>(cc.ra[r], a);
+            unpack_pair16<ST>(cc.rb[r], b);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                A[2 * r][c] = a[c]; A[2 * r + 1][c] = a[3 + c];
+                B[2 * r][c] = b[c]; B[2 * r + 1][c] = b[3 + c];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { A[q][c] = cc.A[q][c]; B[q][c] = cc.B[q][c]; }
+    }
+    if (bl.mode == BL_COPY) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { C.v[q][0] = A[q][0]; C.v[q][1] = A[q][1]; C.v[q][2] = A[q][2]; }
+    } else {
+        blend_n<8>(bl, A, B, C.v);
+    }
+    lerp_corners(g, C, out);
+}
+
+// Whether the periodic fast path applies to a grid (its row offsets use
+// 24-bit multiplies: nx * ny and nzs * node bytes below 2^24).
+inline bool periodic_fast(const GridDev& g, int node_bytes) {
+    return g.bmode == JT_BOUNDS_PERIODIC && g.inv[0] != 0.0f && g.inv[1] != 0.0f && g.inv[2] != 0.0f &&
+           g.nx >= 2 && g.ny >= 2 && g.nz >= 2 && (long long)g.nx * g.ny < (1LL << 24) &&
+           (long long)g.nzs * node_bytes < (1LL << 24);
+}
+
+// Waves per SIMD the tracking kernels' register budget is sized for (168
+// VGPRs at 3): the per-lane corner cache of the fast path holds 48 floats
+// (float32 snapshots) or 32 dwords of raw 16-bit records across stages, and
+// spills at 4 waves.
 #ifndef JT_WAVES_F32
 #define JT_WAVES_F32 3
 #endif
 #ifndef JT_WAVES_16
-#define JT_WAVES_16 4
+#define JT_WAVES_16 3
 #endif
 template <int ST> struct TrackWaves {
     static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_WAVES_F32 : JT_WAVES_16;
@@ -428,6 +588,11 @@ template <int ST> struct TrackWaves {
 struct BcDev {
     int kind[3];
     float lo[3], hi[3];
+    // periodic / reflective axes as one formula (apply_bc_axes): y = jnp.mod(x
+    // - lo, per), then y <= rw ? y : rw2 - y, then lo + y.  Periodic: per =
+    // width (1 if empty), rw = +inf; reflective: per = 2 * width (2 if
+    // empty), rw = width, rw2 = 2 * width.
+    float per[3], rw[3], rw2[3];
     int absorb_rows;
     float outside;
     int inlet, ia, ipos;
@@ -501,6 +666,41 @@ __device__ __forceinline__ void apply_bc(const BcDev& b, float p[3]) {
         for (int i = 0; i < 3; ++i) out = out || (p[i] < b.lo[i]) || (p[i] > b.hi[i]);
         if (out) { p[0] = p[1] = p[2] = b.outside; }
     }
+}
+
+// jnp.mod(a, m) for m > 0 with jmodf's values: its three exact cases by
+// selects, the general path only for |a| beyond them or NaN.
+__device__ __forceinline__ float jmod_pos(float a, float m) {
+    if (a > -m && a < 2.0f * m) {
+        const float lo = a + m, hi = a - m;
+        float f = a < 0.0f ? lo : a;
+        return a >= m ? hi : f;
+    }
+    return jmodf(a, m);
+}
+
+// apply_bc for boundaries of per-axis none / periodic / reflective /
+// clamping kinds only (no inlet, no absorbing rows), with the same values.
+__device__ __forceinline__ void apply_bc_axes(const BcDev& b, float p[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int k = b.kind[i];
+        if (k == JT_BC_PERIODIC || k == JT_BC_REFLECTIVE) {
+            float y = jmod_pos(p[i] - b.lo[i], b.per[i]);
+            y = (y <= b.rw[i]) ? y : b.rw2[i] - y;
+            p[i] = b.lo[i] + y;
+        } else if (k == JT_BC_CLAMP) {
+            p[i] = clipf(p[i], b.lo[i], b.hi[i]);
+        }
+    }
+}
+
+// Whether apply_bc_axes covers a boundary.
+inline bool bc_axes_only(const BcDev& b) {
+    if (b.inlet != JT_INLET_OFF || b.absorb_rows) return false;
+    for (int i = 0; i < 3; ++i)
+        if (b.kind[i] == JT_BC_ABSORB) return false;
+    return true;
 }
 
 // Stage-time blend table: 3 entries per step (stage origin, midpoint, end).

@@ -41,10 +41,18 @@ struct Arena {
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
+// A dword whose high half is a bfloat16 and whose low half is zero is that
+// value's float32 bit pattern.
+__device__ __forceinline__ float bf16_hi_to_f32(uint32_t w) {
+    return __uint_as_float(w);
+}
 __device__ __forceinline__ float f16_to_f32(uint16_t h) {
     return (float)__builtin_bit_cast(_Float16, h);
 }
 
+#ifndef JT_FMOD_OPAQUE
+#define JT_FMOD_OPAQUE 1
+#endif
 // jnp.mod on floats: fmod, then add the divisor when the signs differ.
 // For n > 0 the common cases are exact without fmod: a in [0, n) gives a;
 // a in (-n, 0) gives fl(a + n) (fmod returns a itself); a in [n, 2n) gives
@@ -55,8 +63,15 @@ __device__ __forceinline__ float jmodf(float a, float n) {
         if (a < 0.0f && a > -n) return a + n;
         if (a >= n && a < 2.0f * n) return a - n;
     }
-    float r = fmodf(a, n);
-    if (r != 0.0f && ((r < 0.0f) != (n < 0.0f))) r = r + n;
+    // the rare general case: the opaque copy of n keeps the compiler from
+    // hoisting fmodf's divisor set-up out of the callers' loops, where it
+    // would hold registers for every iteration
+    float m = n;
+#if JT_FMOD_OPAQUE
+    __asm__ volatile("" : "+v"(m));
+#endif
+    float r = fmodf(a, m);
+    if (r != 0.0f && ((r < 0.0f) != (m < 0.0f))) r = r + m;
     return r;
 }
 // jnp.clip (minimum(maximum(x, lo), hi)) keeping NaN.

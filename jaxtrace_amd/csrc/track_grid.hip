@@ -77,7 +77,10 @@ int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
 
 int make_bc_dev(const jt_boundary* b, BcDev& d) {
     if (b == nullptr) {
-        for (int i = 0; i < 3; ++i) { d.kind[i] = JT_BC_NONE; d.lo[i] = 0; d.hi[i] = 0; }
+        for (int i = 0; i < 3; ++i) {
+            d.kind[i] = JT_BC_NONE; d.lo[i] = 0; d.hi[i] = 0;
+            d.per[i] = 1.0f; d.rw[i] = INFINITY; d.rw2[i] = 0.0f;
+        }
         d.absorb_rows = 0; d.outside = 0; d.inlet = JT_INLET_OFF; d.ia = 0; d.ipos = 1;
         d.backstop = 0; d.coords[0] = d.coords[1] = nullptr; d.nc[0] = d.nc[1] = 0;
         return JT_OK;
@@ -88,6 +91,19 @@ int make_bc_dev(const jt_boundary* b, BcDev& d) {
         d.kind[i] = b->axis_kind[i];
         d.lo[i] = b->lo[i];
         d.hi[i] = b->hi[i];
+        // the constants of apply_bc_axes, in bc_axis's float32 operations
+        float w = b->hi[i] - b->lo[i];
+        w = w < 0.0f ? 0.0f : w;
+        const float ws = w > 0.0f ? w : 1.0f;
+        if (b->axis_kind[i] == JT_BC_REFLECTIVE) {
+            d.per[i] = 2.0f * ws;
+            d.rw[i] = w;
+            d.rw2[i] = 2.0f * w;
+        } else {
+            d.per[i] = ws;
+            d.rw[i] = INFINITY;
+            d.rw2[i] = 0.0f;
+        }
     }
     d.absorb_rows = b->absorb_rows;
     d.outside = b->outside_value;
@@ -132,7 +148,13 @@ template <int ST> struct TrackBlock {
     static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_BLOCK_F32 : JT_BLOCK_16;
 };
 
-template <int INTEG, int ST, bool ZH>
+#ifndef JT_REUSE
+#define JT_REUSE 1
+#endif
+
+// PF: the periodic sampler fast path (axis_coord_pf) and the per-axis
+// boundary epilogue (apply_bc_axes), both chosen on the host where they apply.
+template <int INTEG, int ST, bool ZH, bool PF>
 __global__ void __launch_bounds__(TrackBlock<ST>::v, TrackWaves<ST>::v)
 k_track(GridDev g, BcDev bc, Reorder ro, int n,
         const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
@@ -146,10 +168,15 @@ k_track(GridDev g, BcDev bc, Reorder ro, int n,
     const int id = ro.idin ? ro.idin[src] : src;
     if (ro.idout) ro.idout[i] = id;
     const size_t dst = (size_t)id;
-    auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
+    auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH, PF>(g, b, q[0], q[1], q[2], out); };
+    CornerCache<ST> cc;
+    cache_reset(cc);
+    auto fc = [&](const Blend& b, const float q[3], float out[3]) { sample_cached<ST>(g, b, q[0], q[1], q[2], out, cc); };
     for (int s = s0; s < s1; ++s) {
-        rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);     // rk4.py:84-106 / rk2.py / euler.py
-        apply_bc(bc, p);                                // tracker.py:238-239
+        if constexpr (PF && JT_REUSE) rk_step<INTEG>(fc, tab, s, p, dt, dth, dt6);   // rk4.py:84-106 / rk2.py /
+        else rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);                             // euler.py
+        if constexpr (PF) apply_bc_axes(bc, p);         // tracker.py:238-239
+        else apply_bc(bc, p);
         if (traj != nullptr && (s + 1) % rec_every == 0) {
             const size_t r = (size_t)((s + 1) / rec_every);
             float* o = traj + (r * (size_t)n + dst) * 3;
@@ -174,11 +201,13 @@ k_sample(GridDev g, const float* __restrict__ x, int n, float t, float* __restri
     out[3 * (size_t)i + 2] = v[2];
 }
 
+template <bool AXES>
 __global__ void __launch_bounds__(256) k_boundary(BcDev bc, float* __restrict__ x, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float p[3] = {x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2]};
-    apply_bc(bc, p);
+    if constexpr (AXES) apply_bc_axes(bc, p);
+    else apply_bc(bc, p);
     x[3 * (size_t)i] = p[0];
     x[3 * (size_t)i + 1] = p[1];
     x[3 * (size_t)i + 2] = p[2];
@@ -253,25 +282,43 @@ static KeyBits make_key_bits(const GridDev& g, int lz, int ly, int lx) {
 
 // One block of TrackBlock threads per particle range; the grid is padded to a
 // multiple of 8 for the kernel's XCD mapping.
-template <int INTEG, int ST, bool ZH>
+template <int INTEG, int ST, bool ZH, bool PF>
 static void launch_track(hipStream_t s, const GridDev& g, const BcDev& bc, const Reorder& ro, int n,
                          const Blend* tab, int s0, int s1, float dt, float dth, float dt6, int rec, float* traj) {
     const unsigned B = TrackBlock<ST>::v;
     const unsigned nb = ((unsigned)n + B - 1) / B;
-    hipLaunchKernelGGL((k_track<INTEG, ST, ZH>), dim3((nb + 7) / 8 * 8), dim3(B), 0, s, g, bc, ro, n, tab, s0, s1,
+#ifdef JT_F32_LDS
+    const size_t lds = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_F32_LDS : 0;
+#else
+    const size_t lds = 0;
+#endif
+    hipLaunchKernelGGL((k_track<INTEG, ST, ZH, PF>), dim3((nb + 7) / 8 * 8), dim3(B), lds, s, g, bc, ro, n, tab, s0, s1,
                        dt, dth, dt6, rec, traj);
 }
 
-template <int INTEG, bool ZH>
+template <int INTEG, bool ZH, bool PF>
 static void launch_track_st(int st, hipStream_t s, const GridDev& g,
                             const BcDev& bc, const Reorder& ro, int n, const Blend* tab, int s0, int s1,
                             float dt, float dth, float dt6, int rec, float* traj) {
     switch (st) {
-    case ST_F32x3: launch_track<INTEG, ST_F32x3, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
-    case ST_F32x4: launch_track<INTEG, ST_F32x4, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
-    case ST_BF16x4: launch_track<INTEG, ST_BF16x4, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
-    default: launch_track<INTEG, ST_F16x4, ZH>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    case ST_F32x3: launch_track<INTEG, ST_F32x3, ZH, PF>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    case ST_F32x4: launch_track<INTEG, ST_F32x4, ZH, PF>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    case ST_BF16x4: launch_track<INTEG, ST_BF16x4, ZH, PF>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
+    default: launch_track<INTEG, ST_F16x4, ZH, PF>(s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj); break;
     }
+}
+
+// The z-halo layout with the fast paths (periodic sampler with exact
+// reciprocal spacing, per-axis boundary kinds), the z-halo layout, or the
+// node-per-load layout.
+template <int INTEG>
+static void launch_track_any(int st, hipStream_t s, const GridDev& g, const BcDev& bc, const Reorder& ro, int n,
+                             const Blend* tab, int s0, int s1, float dt, float dth, float dt6, int rec,
+                             float* traj) {
+    if (g.nzs == g.nz) launch_track_st<INTEG, false, false>(st, s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj);
+    else if (periodic_fast(g, st == ST_F32x3 ? 12 : (st == ST_F32x4 ? 16 : 8)) && bc_axes_only(bc))
+        launch_track_st<INTEG, true, true>(st, s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj);
+    else launch_track_st<INTEG, true, false>(st, s, g, bc, ro, n, tab, s0, s1, dt, dth, dt6, rec, traj);
 }
 
 template <bool ZH>
@@ -315,8 +362,13 @@ extern "C" int jt_apply_boundary(const jt_boundary* bc, float* x, int64_t n, voi
     JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
     if (n == 0) return JT_OK;
     JT_REQUIRE(x != nullptr, "NULL x");
-    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, d, x, (int)n);
+    // the same fused epilogue the tracking kernels run on this boundary
+    if (bc_axes_only(d))
+        hipLaunchKernelGGL(k_boundary<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, d, x, (int)n);
+    else
+        hipLaunchKernelGGL(k_boundary<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, d, x, (int)n);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }
@@ -413,20 +465,16 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
                 cur_ids = nid;
             }
         }
-        const bool zh = g.nzs != g.nz;
         if (2 * launch + 1 < n_ev) JT_HIP(hipEventRecord((hipEvent_t)opts->launch_events[2 * launch], s));
         switch (integrator) {
         case JT_RK4:
-            if (zh) launch_track_st<JT_RK4, true>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
-            else launch_track_st<JT_RK4, false>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            launch_track_any<JT_RK4>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
             break;
         case JT_RK2:
-            if (zh) launch_track_st<JT_RK2, true>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
-            else launch_track_st<JT_RK2, false>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            launch_track_any<JT_RK2>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
             break;
         default:
-            if (zh) launch_track_st<JT_EULER, true>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
-            else launch_track_st<JT_EULER, false>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
+            launch_track_any<JT_EULER>(st, s, g, d, ro, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj);
             break;
         }
         JT_HIP(hipGetLastError());

@@ -99,17 +99,21 @@ class _GridSeries:
         # packed: (T,Nx,Ny,Nz,3) f32 as given; padded: 4 components per node;
         # halo/halo4: one extra z-plane (plane Nz repeats plane 0 for periodic
         # bounds, plane Nz-1 otherwise) so both z-corners of a cell are one load
+        # 16-bit nodes are (x, y, 0, z): z in the high half of the second
+        # dword over a zero low half, which the kernels read as float32 bits
         ncomp = 3 if (storage == "float32" and layout in ("packed", "halo")) else 4
         z_halo = 1 if layout in ("halo", "halo4") else 0
+        self._slots = [0, 1, 2] if storage == "float32" else [0, 1, 3]
         if ncomp == 3 and not z_halo:
             snaps = data.to(device=self.device, dtype=torch.float32).contiguous()
         else:
             snaps = torch.zeros((T, Nx, Ny, Nz + z_halo, ncomp), dtype=tdt, device=self.device)
             src = data.to(device=self.device, dtype=torch.float32)
-            snaps[:, :, :, :Nz, :3] = src.to(tdt)
-            if z_halo:
-                edge = 0 if mode == "periodic" else Nz - 1
-                snaps[:, :, :, Nz, :3] = src[:, :, :, edge, :].to(tdt)
+            for c, slot in enumerate(self._slots):
+                snaps[:, :, :, :Nz, slot] = src[..., c].to(tdt)
+                if z_halo:
+                    edge = 0 if mode == "periodic" else Nz - 1
+                    snaps[:, :, :, Nz, slot] = src[:, :, :, edge, c].to(tdt)
             del src
         self.z_halo = z_halo
         self._snaps = snaps
@@ -215,7 +219,7 @@ class StructuredTimeSeriesField(_GridSeries):
 
     def with_storage(self, storage: str, layout: str = "halo") -> "StructuredTimeSeriesField":
         """Same field, snapshots re-stored as ``storage`` (RNE from float32)."""
-        src = self._snaps[:, :, :, :self.Nz, :3].to(torch.float32)
+        src = self._snaps[:, :, :, :self.Nz, self._slots].to(torch.float32)
         return StructuredTimeSeriesField(src, self.times, self.grid_x, self.grid_y, self.grid_z,
                                          self.bounds_handling, self.interpolation,
                                          self.extrapolation, storage, layout, self.device)

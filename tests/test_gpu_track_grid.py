@@ -260,3 +260,28 @@ def test_time_series_field_nearest_node(gpu):
     fn = lambda p, t: OF.nearest_node_sample(data, times, nodes, p, t)
     tt, pos = OT.track(x0, (0.0, 0.5), 11, fn, lambda p: OB.per_axis(p, [[0, 0, 0], [1, 1, 1]], ["clamping"] * 3))
     np.testing.assert_array_equal(traj.positions, pos)
+
+
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_periodic_fast_path_edges(gpu, storage):
+    """The periodic sampler on a power-of-two grid with the z-halo layout
+    takes the kernel's periodic fast path; the packed layout takes the general
+    index code.  Start points outside the box (wrapped by the sampler only:
+    the boundary is open), a hair below 0 (jnp.mod rounds to n), at 1, beyond
+    two periods (the general fmod path), -0.0 and NaN, tracked through RK4:
+    both layouts must reproduce the oracle bitwise, step by step."""
+    from jaxtrace_amd.boundary import BoundaryCondition
+    data, times, g, x0 = _cfg1(3000)
+    x0[:10] = [[-1e-9, 0.5, 0.25], [1.0, 1.0 - 2 ** -24, 0.0], [2.75, -3.5, 7.0], [-0.0, -0.0, -0.0],
+               [-1.0 + 1e-7, 2.0, -2.0 + 1e-7], [np.nan, 0.5, 0.5], [0.5, np.nan, 0.5], [1e6, -1e6, 3.0],
+               [31.0 / 32, 1.0 / 32, 0.5 - 2 ** -26], [-2 ** -30, 1.0 + 2 ** -23, -0.5]]
+    ref = _rounded(data, storage)
+    meta = OF.GridMeta(g, g, g)
+    field_fn = lambda p, t: OF.structured_series_sample_corners(ref, times, meta, p, t, "periodic")
+    open_bc = BoundaryCondition("_open_bc", np.array([[0, 0, 0], [1, 1, 1]], F32))
+    tt, pos = OT.track(x0, (0.0, 0.1), 11, field_fn, lambda p: p, "rk4")
+    for layout in ("halo", "packed"):
+        f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", storage=storage,
+                                         layout=layout)
+        traj = jt.create_tracker("rk4", f, open_bc).track_particles(x0, (0.0, 0.1), 11)
+        np.testing.assert_array_equal(traj.positions, pos, err_msg=layout)

@@ -204,7 +204,7 @@ def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32):
     out = {}
     for storage in ("float32", "bfloat16"):
         res = {}
-        for rep in range(2):                      # first repetition warms up
+        for rep in range(3):                      # the first two repetitions warm up
             U = data.clone().requires_grad_(True)
             X = x0.clone().requires_grad_(True)
             torch.cuda.synchronize()

@@ -53,16 +53,25 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
 
 // ---- LDS box -----------------------------------------------------------------
 #ifndef TILE_NODES
-#define TILE_NODES 1660            // 2 slots x 3 float64 per node + a spare: < 80 KiB, two blocks per CU
+#define TILE_NODES 3320            // 2 slots x 3 float64 per node + a spare: the CU's 160 KiB, one block per CU
 #endif
 #ifndef ADJ_BLOCK
-#define ADJ_BLOCK 256
+#define ADJ_BLOCK 512              // 8 waves: two per SIMD, as the registers allow
 #endif
 #ifndef ADJ_RESORT
 #define ADJ_RESORT 8
 #endif
 #ifndef ADJ_MARGIN
 #define ADJ_MARGIN 0
+#endif
+#ifndef ADJ_REC_WAVES
+#define ADJ_REC_WAVES 4
+#endif
+#ifndef ADJ_MINW
+#define ADJ_MINW 1
+#endif
+#ifndef JT_ADJ_REUSE
+#define JT_ADJ_REUSE 0             // the corner cache costs the reverse kernel its second wave per SIMD
 #endif
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -75,10 +84,20 @@ __device__ __forceinline__ int wave_max_i(int v) {
 }
 
 struct Tile {
-    int b[3], e[3];     // box origin (node indices) and extents
+    int m[3];           // cell the offsets are taken from
+    int b[3], e[3];     // box origin (wrapped node offsets from m) and extents
     int smin;           // snapshot of slot 0 (slot 1 is smin + 1)
     int vol;            // e0*e1*e2 (0: no box)
 };
+
+// v in (-n, n) wrapped to [-(n/2), n - n/2): a node offset from the box's
+// origin cell that is the same node whichever way round a periodic axis it
+// is counted.
+__device__ __forceinline__ int wrap_off(int v, int n) {
+    const int h = n >> 1;
+    v = v < -h ? v + n : v;
+    return v >= n - h ? v - n : v;
+}
 
 struct Stencil {
     int I[2], J[2], K[2];
@@ -90,11 +109,13 @@ struct Stencil {
 // trilinear weights' derivatives are +-1/d per axis (0 where a non-periodic
 // axis is clipped).  depends = false where u is the constant 0 of the 'zero'
 // bounds mode.
-template <int ST, bool ZH>
+// PF: the fast path's gather through the per-lane corner cache cc.
+template <int ST, bool ZH, bool PF = false>
 __device__ __forceinline__ void sample_jac(const GridDev& g, const Blend& bl, const float q[3], float u[3],
-                                           float J[9], bool& depends) {
+                                           float J[9], bool& depends, CornerCache<ST>& cc) {
     Corners C;
-    gather_corners<ST, ZH>(g, bl, q[0], q[1], q[2], C);
+    if constexpr (PF && JT_ADJ_REUSE) gather_cached<ST>(g, bl, q[0], q[1], q[2], C, cc);
+    else gather_corners<ST, ZH, PF>(g, bl, q[0], q[1], q[2], C);
     lerp_corners(g, C, u);
     depends = !(!C.valid && g.bmode == JT_BOUNDS_ZERO);
     float gd[3];
@@ -185,9 +206,12 @@ __device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, con
     const int tv = tl.vol * 3;
     const int spare = 2 * TILE_NODES * 3;
     unsigned miss = 0;
+    const int oi[2] = {wrap_off(r.I[0] - tl.m[0], g.nx) - tl.b[0], wrap_off(r.I[1] - tl.m[0], g.nx) - tl.b[0]};
+    const int oj[2] = {wrap_off(r.J[0] - tl.m[1], g.ny) - tl.b[1], wrap_off(r.J[1] - tl.m[1], g.ny) - tl.b[1]};
+    const int ok_[2] = {wrap_off(r.K[0] - tl.m[2], g.nz) - tl.b[2], wrap_off(r.K[1] - tl.m[2], g.nz) - tl.b[2]};
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int li = r.I[q >> 2] - tl.b[0], lj = r.J[(q >> 1) & 1] - tl.b[1], lk = r.K[q & 1] - tl.b[2];
+        const int li = oi[q >> 2], lj = oj[(q >> 1) & 1], lk = ok_[q & 1];
         const bool in = (unsigned)li < (unsigned)tl.e[0] && (unsigned)lj < (unsigned)tl.e[1] &&
                         (unsigned)lk < (unsigned)tl.e[2];
         const int loc = ((li * tl.e[1] + lj) * tl.e[2] + lk) * 3;
@@ -224,8 +248,11 @@ __device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, con
 // Forward recompute of one checkpoint segment [sa, sb): thread t takes
 // particle perm[t] (cell-sorted) and stores the state before each step at
 // seg[(s - sa) * n + t] (thread order: coalesced stores).
-template <int ST, bool ZH>
-__global__ void __launch_bounds__(256, TrackWaves<ST>::v)
+// PF: the tracking kernel's fast path (periodic sampler with exact reciprocal
+// spacing, z-halo layout, per-axis boundary kinds) with its per-lane corner
+// reuse: the same values as the forward run.
+template <int ST, bool ZH, bool PF = false>
+__global__ void __launch_bounds__(256, ADJ_REC_WAVES)
 k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const int* __restrict__ perm,
                 const Blend* __restrict__ tab, int sa, int sb, float dt, float dth, float dt6,
                 float* __restrict__ seg) {
@@ -236,12 +263,18 @@ k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const
     const int pid = perm[t];
     float x[3] = {x0s[3 * (size_t)pid], x0s[3 * (size_t)pid + 1], x0s[3 * (size_t)pid + 2]};
     auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH>(g, b, q[0], q[1], q[2], out); };
+    CornerCache<ST> cc;
+    cache_reset(cc);
 #pragma unroll 1
     for (int s = sa; s < sb; ++s) {
         float* o = seg + ((size_t)(s - sa) * n + t) * 3;
         o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
-        rk_step<JT_RK4>(f, tab, s, x, dt, dth, dt6);
-        apply_bc(bc, x);
+        if constexpr (PF) {
+            fast_step<JT_RK4, ST>(g, bc, tab, s, x, dt, dth, dt6, cc);
+        } else {
+            rk_step<JT_RK4>(f, tab, s, x, dt, dth, dt6);
+            apply_bc(bc, x);
+        }
     }
 }
 
@@ -249,8 +282,8 @@ k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const
 // segment slot slot = order[t] (slots sorted by their cell at step s_lo, so a
 // block's particles start in a small region), particle pid = seg_pid[slot];
 // its adjoint lives in lam_io[pid] between launches.
-template <int ST, bool ZH>
-__global__ void __launch_bounds__(ADJ_BLOCK)
+template <int ST, bool ZH, bool PF = false>
+__global__ void __launch_bounds__(ADJ_BLOCK, ADJ_MINW)
 k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const int* __restrict__ seg_pid,
               const Blend* __restrict__ tab, int sa, int s_lo, int s_hi, float dt, float dth, float dt6,
               const float* __restrict__ seg, float* __restrict__ lam_io, float* __restrict__ gU,
@@ -264,47 +297,62 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     const bool live = t < n;                           // every other thread stays: the block synchronises
     const int slot = live ? order[t] : order[0];
     const int nn[3] = {g.nx, g.ny, g.nz};
-    // box: the block's start cells widened by the stencil (one node above)
-    // and ADJ_MARGIN cells of motion each way
-    int c0[3];
-    {
-        const float* xi = seg + ((size_t)(s_lo - sa) * n + slot) * 3;
+    // box: offsets (wrapped to half a grid each way, so a block that meets a
+    // periodic face stays compact) from the start cell m of the block's
+    // middle particle, spanning the block's start cells, widened by the
+    // stencil (one node above) and ADJ_MARGIN cells each way.  (Spanning every
+    // step's cells of the sub-segment instead measured slower: the larger
+    // boxes outgrow the LDS and are clamped.)
+    auto cell_of = [&](const float* x, int c[3]) {
         for (int d = 0; d < 3; ++d) {
-            float f = floorf((xi[d] - g.o[d]) / g.d[d]);
-            c0[d] = (f != f) ? 0 : (int)fminf(fmaxf(f, 0.0f), (float)(nn[d] - 1));
+            float f = floorf((x[d] - g.o[d]) / g.d[d]);
+            c[d] = (f != f) ? 0 : (int)fminf(fmaxf(f, 0.0f), (float)(nn[d] - 1));
         }
-    }
+    };
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int d = 0; d < 3; ++d) {
-        int l = wave_min_i(live ? c0[d] : 0x7fffffff), h = wave_max_i(live ? c0[d] : -1);
-        if (lane == 0) { sred[wv * 6 + d] = l; sred[wv * 6 + 3 + d] = h; }
-    }
+    constexpr int NW = ADJ_BLOCK / 64;
     const int mid = (min(n - 1, lb * ADJ_BLOCK + ADJ_BLOCK / 2) & (ADJ_BLOCK - 1));
-    if ((int)threadIdx.x == mid)
-        for (int d = 0; d < 3; ++d) sred[(ADJ_BLOCK / 64) * 6 + d] = c0[d];
+    if ((int)threadIdx.x == mid) {
+        int c[3];
+        cell_of(seg + ((size_t)(s_lo - sa) * n + slot) * 3, c);
+        for (int d = 0; d < 3; ++d) sred[NW * 6 + d] = c[d];
+    }
     for (int e = threadIdx.x; e < 2 * TILE_NODES * 3 + 3; e += blockDim.x) tile[e] = 0.0;
     __syncthreads();
     Tile tl;
+    for (int d = 0; d < 3; ++d) tl.m[d] = sred[NW * 6 + d];
+    {
+        int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
+        if (live) {
+            int c[3];
+            cell_of(seg + ((size_t)(s_lo - sa) * n + slot) * 3, c);
+            for (int d = 0; d < 3; ++d) lo[d] = hi[d] = wrap_off(c[d] - tl.m[d], nn[d]);
+        }
+        for (int d = 0; d < 3; ++d) {
+            const int l = wave_min_i(lo[d]), h = wave_max_i(hi[d]);
+            if (lane == 0) { sred[wv * 6 + d] = l; sred[wv * 6 + 3 + d] = h; }
+        }
+    }
+    __syncthreads();
     {
         long long vol = 1;
         int ext[3];
         for (int d = 0; d < 3; ++d) {
             int l = sred[d], h = sred[3 + d];
-            for (int w = 1; w < ADJ_BLOCK / 64; ++w) { l = min(l, sred[w * 6 + d]); h = max(h, sred[w * 6 + 3 + d]); }
-            l = max(l - ADJ_MARGIN, 0);
-            h = min(h + 1 + ADJ_MARGIN, nn[d] - 1);
+            for (int w = 1; w < NW; ++w) { l = min(l, sred[w * 6 + d]); h = max(h, sred[w * 6 + 3 + d]); }
+            l -= ADJ_MARGIN;
+            h += 1 + ADJ_MARGIN;
             tl.b[d] = l;
             ext[d] = h - l + 1;
             vol *= (ext[d] > 0 ? ext[d] : 0);
         }
         if (vol > TILE_NODES) {                        // a cube of the largest fitting side around the
-            int cap = 1;                               // block's middle particle
+            int cap = 1;                               // block's middle particle (offset 0)
             while ((cap + 1) * (cap + 1) * (cap + 1) <= TILE_NODES) ++cap;
             vol = 1;
             for (int d = 0; d < 3; ++d) {
                 if (ext[d] > cap) {
-                    const int m = sred[(ADJ_BLOCK / 64) * 6 + d];
-                    int b0 = m - cap / 2;
+                    int b0 = -(cap / 2);
                     b0 = max(b0, tl.b[d]);
                     b0 = min(b0, tl.b[d] + ext[d] - cap);
                     tl.b[d] = b0;
@@ -316,8 +364,8 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
         for (int d = 0; d < 3; ++d) tl.e[d] = ext[d];
         tl.vol = vol > 0 ? (int)vol : 0;
         int smin = tab[3 * s_lo].i0;                   // snapshot slots: the sub-segment's lowest index
-        for (int s = s_lo; s < s_hi; ++s)
-            for (int k = 0; k < 3; ++k) smin = min(smin, min(tab[3 * s + k].i0, tab[3 * s + k].i1));
+        for (int st = s_lo; st < s_hi; ++st)
+            for (int k = 0; k < 3; ++k) smin = min(smin, min(tab[3 * st + k].i0, tab[3 * st + k].i1));
         tl.smin = smin;
         if (tl.vol == 0) { tl.e[0] = tl.e[1] = tl.e[2] = 0; }
     }
@@ -326,6 +374,8 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     if (live) { lam[0] = lam_io[3 * (size_t)pid]; lam[1] = lam_io[3 * (size_t)pid + 1]; lam[2] = lam_io[3 * (size_t)pid + 2]; }
     RunAcc run;
     run.live = false;
+    CornerCache<ST> cc;
+    cache_reset(cc);
     if (live) {
         for (int s = s_hi - 1; s >= s_lo; --s) {
             const float* xi = seg + ((size_t)(s - sa) * n + slot) * 3;
@@ -343,7 +393,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
                 const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
                 float k[3], Jn[9];
                 bool dn;
-                sample_jac<ST, ZH>(g, b, q, k, Jn, dn);
+                sample_jac<ST, ZH, PF>(g, b, q, k, Jn, dn, cc);
                 if (r == 3) {
                     float xp[3];
 #pragma unroll
@@ -392,9 +442,18 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
                     Stencil stc;
                     bool valid = true;
                     float w;
-                    axis_coord(g, 0, cur[0], g.nx, stc.I[0], stc.I[1], w, valid); stc.wx[0] = 1.0f - w; stc.wx[1] = w;
-                    axis_coord(g, 1, cur[1], g.ny, stc.J[0], stc.J[1], w, valid); stc.wy[0] = 1.0f - w; stc.wy[1] = w;
-                    axis_coord(g, 2, cur[2], g.nz, stc.K[0], stc.K[1], w, valid); stc.wz[0] = 1.0f - w; stc.wz[1] = w;
+                    if constexpr (PF) {
+                        axis_coord_pf(cur[0], g.o[0], g.inv[0], g.nx, (float)g.nx, stc.I[0], stc.I[1], w);
+                        stc.wx[0] = 1.0f - w; stc.wx[1] = w;
+                        axis_coord_pf(cur[1], g.o[1], g.inv[1], g.ny, (float)g.ny, stc.J[0], stc.J[1], w);
+                        stc.wy[0] = 1.0f - w; stc.wy[1] = w;
+                        axis_coord_pf(cur[2], g.o[2], g.inv[2], g.nz, (float)g.nz, stc.K[0], stc.K[1], w);
+                        stc.wz[0] = 1.0f - w; stc.wz[1] = w;
+                    } else {
+                        axis_coord(g, 0, cur[0], g.nx, stc.I[0], stc.I[1], w, valid); stc.wx[0] = 1.0f - w; stc.wx[1] = w;
+                        axis_coord(g, 1, cur[1], g.ny, stc.J[0], stc.J[1], w, valid); stc.wy[0] = 1.0f - w; stc.wy[1] = w;
+                        axis_coord(g, 2, cur[2], g.nz, stc.K[0], stc.K[1], w, valid); stc.wz[0] = 1.0f - w; stc.wz[1] = w;
+                    }
                     const bool same = run.live && stc.I[0] == run.I[0] && stc.J[0] == run.J[0] &&
                                       stc.K[0] == run.K[0] && stc.I[1] == run.I[1] && stc.J[1] == run.J[1] &&
                                       stc.K[1] == run.K[1] && b.i0 == run.snap[0] && b.i1 == run.snap[1];
@@ -431,10 +490,11 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
         const int sl = e >= tv;
         const int r = e - sl * tv;
         const int node = r / 3, comp = r - node * 3;
-        const int k = node % tl.e[2] + tl.b[2];
+        auto wrap_node = [](int v, int n) { v = v < 0 ? v + n : v; return v >= n ? v - n : v; };
+        const int k = wrap_node(tl.m[2] + node % tl.e[2] + tl.b[2], g.nz);
         const int t2 = node / tl.e[2];
-        const int j = t2 % tl.e[1] + tl.b[1];
-        const int i = t2 / tl.e[1] + tl.b[0];
+        const int j = wrap_node(tl.m[1] + t2 % tl.e[1] + tl.b[1], g.ny);
+        const int i = wrap_node(tl.m[0] + t2 / tl.e[1] + tl.b[0], g.nx);
         atomicAdd(gU + (size_t)(tl.smin + sl) * gsnap + ((size_t)(i * g.ny + j) * g.nz + k) * 3 + comp, v);
     }
 }
@@ -559,14 +619,28 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     dim3 g256x((unsigned)((((n + 255) / 256) + 7) / 8 * 8));   // multiple of 8 for the XCD remap
     dim3 badj(ADJ_BLOCK), gadj((unsigned)((((n + ADJ_BLOCK - 1) / ADJ_BLOCK) + 7) / 8 * 8));
     const bool zh = g.nzs != g.nz;
+    const bool pf = zh && periodic_fast(g, st == ST_F32x3 ? 12 : (st == ST_F32x4 ? 16 : 8)) && bc_axes_only(d);
     for (int c = n_seg - 1; c >= 0; --c) {
         const int sa = c * ckpt_every;
         const int sb = sa + ckpt_every < n_steps ? sa + ckpt_every : n_steps;
         const float* xc = ckpt + (size_t)c * n * 3;
         hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, xc, (int)n, shift, bits, keys, idx);
         JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, end_bit, s));
-        JT_ADJ_DISPATCH(k_adj_recompute, g256x, b256, g, d, xc, (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6,
-                        seg);
+        if (pf) {
+            switch (st) {
+            case ST_F32x3: hipLaunchKernelGGL((k_adj_recompute<ST_F32x3, true, true>), g256x, b256, 0, s, g, d, xc,
+                                              (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
+            case ST_F32x4: hipLaunchKernelGGL((k_adj_recompute<ST_F32x4, true, true>), g256x, b256, 0, s, g, d, xc,
+                                              (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
+            case ST_BF16x4: hipLaunchKernelGGL((k_adj_recompute<ST_BF16x4, true, true>), g256x, b256, 0, s, g, d, xc,
+                                               (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
+            default: hipLaunchKernelGGL((k_adj_recompute<ST_F16x4, true, true>), g256x, b256, 0, s, g, d, xc,
+                                        (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
+            }
+        } else {
+            JT_ADJ_DISPATCH(k_adj_recompute, g256x, b256, g, d, xc, (int)n, (const int*)perm, tab, sa, sb, dt, dth,
+                            dt6, seg);
+        }
         JT_HIP(hipGetLastError());
         // sub-segments of at most R steps, cut where a third snapshot would enter
         int s_hi = sb;
@@ -590,8 +664,25 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
             hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift,
                                bits, keys, idx);
             JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, order, (int)n, 0, end_bit, s));
-            JT_ADJ_DISPATCH(k_adj_reverse, gadj, badj, g, d, (int)n, (const int*)order, (const int*)perm, tab, sa,
-                            s_lo, s_hi, dt, dth, dt6, (const float*)seg, lam, grad_U, grad_x0);
+            if (pf) {
+                switch (st) {
+                case ST_F32x3: hipLaunchKernelGGL((k_adj_reverse<ST_F32x3, true, true>), gadj, badj, 0, s, g, d, (int)n,
+                                                  (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth, dt6,
+                                                  (const float*)seg, lam, grad_U, grad_x0); break;
+                case ST_F32x4: hipLaunchKernelGGL((k_adj_reverse<ST_F32x4, true, true>), gadj, badj, 0, s, g, d, (int)n,
+                                                  (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth, dt6,
+                                                  (const float*)seg, lam, grad_U, grad_x0); break;
+                case ST_BF16x4: hipLaunchKernelGGL((k_adj_reverse<ST_BF16x4, true, true>), gadj, badj, 0, s, g, d,
+                                                   (int)n, (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt,
+                                                   dth, dt6, (const float*)seg, lam, grad_U, grad_x0); break;
+                default: hipLaunchKernelGGL((k_adj_reverse<ST_F16x4, true, true>), gadj, badj, 0, s, g, d, (int)n,
+                                            (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth, dt6,
+                                            (const float*)seg, lam, grad_U, grad_x0); break;
+                }
+            } else {
+                JT_ADJ_DISPATCH(k_adj_reverse, gadj, badj, g, d, (int)n, (const int*)order, (const int*)perm, tab, sa,
+                                s_lo, s_hi, dt, dth, dt6, (const float*)seg, lam, grad_U, grad_x0);
+            }
             JT_HIP(hipGetLastError());
             s_hi = s_lo;
         }

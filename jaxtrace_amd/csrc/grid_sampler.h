@@ -485,29 +485,24 @@ __device__ __forceinline__ void cache_reset(CornerCache<ST>& cc) {
     cc.pa = nullptr; cc.pb = nullptr;
 }
 
-// sample<ST, true, true> through a CornerCache.
-template <int ST>
-__device__ __forceinline__ void sample_cached(const GridDev& g, const Blend& bl, float px, float py, float pz,
-                                              float out[3], CornerCache<ST>& cc) {
-    Corners C;
+// gather_corners<ST, true, true> through a CornerCache.  IO: the caller
+// knows the blend is 'interp' (a per-step uniform test), so no mode branches.
+template <int ST, bool IO = false>
+__device__ __forceinline__ void gather_cached(const GridDev& g, const Blend& bl, float px, float py, float pz,
+                                              Corners& C, CornerCache<ST>& cc) {
     C.valid = true;
-    if (bl.mode == BL_ZERO || bl.mode == BL_NAN) {
-        const float z = bl.mode == BL_ZERO ? 0.0f : __int_as_float(0x7fc00000);
-        int i0, i1, j0, j1, k0, k1;
-        axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
-        axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
-        axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) C.v[q][0] = C.v[q][1] = C.v[q][2] = z;
-        lerp_corners(g, C, out);
-        return;
-    }
     int i0, i1, j0, j1, k0, k1;
     axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
     axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
     axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
+    if (!IO && (bl.mode == BL_ZERO || bl.mode == BL_NAN)) {
+        const float z = bl.mode == BL_ZERO ? 0.0f : __int_as_float(0x7fc00000);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) C.v[q][0] = C.v[q][1] = C.v[q][2] = z;
+        return;
+    }
     const char* pa = (const char*)snap_base<ST>(g, bl.i0);
-    const char* pb = bl.mode == BL_COPY ? pa : (const char*)snap_base<ST>(g, bl.i1);
+    const char* pb = (!IO && bl.mode == BL_COPY) ? pa : (const char*)snap_base<ST>(g, bl.i1);
     const bool same = pa == cc.pa && pb == cc.pb && i0 == cc.i0 && i1 == cc.i1 && j0 == cc.j0 && j1 == cc.j1 &&
                       k0 == cc.k0;
     if (!same) {
@@ -553,12 +548,26 @@ __device__ __forceinline__ void sample_cached(const GridDev& g, const Blend& bl,
 #pragma unroll
             for (int c = 0; c < 3; ++c) { A[q][c] = cc.A[q][c]; B[q][c] = cc.B[q][c]; }
     }
-    if (bl.mode == BL_COPY) {
+    if constexpr (IO) {
+        const float al = bl.a, oma = 1.0f - bl.a;      // blend_n's 'interp' arithmetic
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) C.v[q][c] = oma * A[q][c] + al * B[q][c];
+    } else if (bl.mode == BL_COPY) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) { C.v[q][0] = A[q][0]; C.v[q][1] = A[q][1]; C.v[q][2] = A[q][2]; }
     } else {
         blend_n<8>(bl, A, B, C.v);
     }
+}
+
+// sample<ST, true, true> through a CornerCache.
+template <int ST, bool IO = false>
+__device__ __forceinline__ void sample_cached(const GridDev& g, const Blend& bl, float px, float py, float pz,
+                                              float out[3], CornerCache<ST>& cc) {
+    Corners C;
+    gather_cached<ST, IO>(g, bl, px, py, pz, C, cc);
     lerp_corners(g, C, out);
 }
 
@@ -570,15 +579,17 @@ inline bool periodic_fast(const GridDev& g, int node_bytes) {
            (long long)g.nzs * node_bytes < (1LL << 24);
 }
 
-// Waves per SIMD the tracking kernels' register budget is sized for (168
-// VGPRs at 3): the per-lane corner cache of the fast path holds 48 floats
-// (float32 snapshots) or 32 dwords of raw 16-bit records across stages, and
-// spills at 4 waves.
+// Waves per SIMD the tracking kernels' register budget is sized for (128
+// VGPRs at 4).  The per-lane corner cache of the fast path holds 48 floats
+// (float32 snapshots) or 32 dwords of raw 16-bit records across stages; at
+// 4 waves that spills 92 B (float32) / 40 B (16-bit) per lane and still runs
+// faster than the spill-free 3-wave build (9.8 against 10.4 ms and 7.9
+// against 8.9 ms per 25 steps of 2^24 particles); 5 waves is far slower.
 #ifndef JT_WAVES_F32
-#define JT_WAVES_F32 3
+#define JT_WAVES_F32 4
 #endif
 #ifndef JT_WAVES_16
-#define JT_WAVES_16 3
+#define JT_WAVES_16 4
 #endif
 template <int ST> struct TrackWaves {
     static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_WAVES_F32 : JT_WAVES_16;
@@ -765,6 +776,31 @@ __device__ __forceinline__ void rk_step(F&& f, const Blend* __restrict__ tab, in
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = p[c] + dt * v[c];
     }
+}
+
+// One step of the fast path (z-halo layout, periodic sampler with exact
+// reciprocal spacing, per-axis boundary kinds) through the per-lane corner
+// cache, then the boundary.  A step whose three stage blends are all
+// 'interp' (a uniform test) runs a copy with no blend-mode branches; the
+// rare other steps sample without the cache and leave it empty, so the cache
+// registers are not live across that copy.
+template <int INTEG, int ST>
+__device__ __forceinline__ void fast_step(const GridDev& g, const BcDev& bc, const Blend* __restrict__ tab, int s,
+                                          float p[3], float dt, float dth, float dt6, CornerCache<ST>& cc) {
+    const Blend* t = tab + 3 * s;
+    if (t[0].mode == BL_INTERP && t[1].mode == BL_INTERP && t[2].mode == BL_INTERP) {
+        auto fi = [&](const Blend& b, const float q[3], float out[3]) {
+            sample_cached<ST, true>(g, b, q[0], q[1], q[2], out, cc);
+        };
+        rk_step<INTEG>(fi, tab, s, p, dt, dth, dt6);
+    } else {
+        auto f = [&](const Blend& b, const float q[3], float out[3]) {
+            sample<ST, true, true>(g, b, q[0], q[1], q[2], out);
+        };
+        rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);
+        cache_reset(cc);
+    }
+    apply_bc_axes(bc, p);                               // tracker.py:238-239
 }
 
 // Host-side translation of the public descriptors.

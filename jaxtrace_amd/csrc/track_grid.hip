@@ -148,12 +148,8 @@ template <int ST> struct TrackBlock {
     static constexpr int v = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_BLOCK_F32 : JT_BLOCK_16;
 };
 
-#ifndef JT_REUSE
-#define JT_REUSE 1
-#endif
-
-// PF: the periodic sampler fast path (axis_coord_pf) and the per-axis
-// boundary epilogue (apply_bc_axes), both chosen on the host where they apply.
+// PF: the fast path (fast_step: periodic sampler index math, per-lane corner
+// reuse, per-axis boundary epilogue), chosen on the host where it applies.
 template <int INTEG, int ST, bool ZH, bool PF>
 __global__ void __launch_bounds__(TrackBlock<ST>::v, TrackWaves<ST>::v)
 k_track(GridDev g, BcDev bc, Reorder ro, int n,
@@ -171,12 +167,13 @@ k_track(GridDev g, BcDev bc, Reorder ro, int n,
     auto f = [&](const Blend& b, const float q[3], float out[3]) { sample<ST, ZH, PF>(g, b, q[0], q[1], q[2], out); };
     CornerCache<ST> cc;
     cache_reset(cc);
-    auto fc = [&](const Blend& b, const float q[3], float out[3]) { sample_cached<ST>(g, b, q[0], q[1], q[2], out, cc); };
     for (int s = s0; s < s1; ++s) {
-        if constexpr (PF && JT_REUSE) rk_step<INTEG>(fc, tab, s, p, dt, dth, dt6);   // rk4.py:84-106 / rk2.py /
-        else rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);                             // euler.py
-        if constexpr (PF) apply_bc_axes(bc, p);         // tracker.py:238-239
-        else apply_bc(bc, p);
+        if constexpr (PF) {
+            fast_step<INTEG, ST>(g, bc, tab, s, p, dt, dth, dt6, cc);
+        } else {
+            rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);     // rk4.py:84-106 / rk2.py / euler.py
+            apply_bc(bc, p);                                // tracker.py:238-239
+        }
         if (traj != nullptr && (s + 1) % rec_every == 0) {
             const size_t r = (size_t)((s + 1) / rec_every);
             float* o = traj + (r * (size_t)n + dst) * 3;

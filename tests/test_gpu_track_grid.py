@@ -285,3 +285,25 @@ def test_periodic_fast_path_edges(gpu, storage):
                                          layout=layout)
         traj = jt.create_tracker("rk4", f, open_bc).track_particles(x0, (0.0, 0.1), 11)
         np.testing.assert_array_equal(traj.positions, pos, err_msg=layout)
+
+
+@pytest.mark.parametrize("extrapolation", ["constant", "linear", "zero", "nan"])
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_fast_path_leaves_snapshot_range(gpu, storage, extrapolation):
+    """Tracking on the fast path (periodic power-of-two grid, z-halo layout)
+    from inside the snapshot times to beyond the last one, so the run switches
+    mid-way from 'interp' steps to extrapolated ones (copy / linear / zero /
+    NaN blends) and back through the per-lane corner cache: every step
+    bitwise equal to the oracle."""
+    data, times, g, x0 = _cfg1(3000)
+    ref = _rounded(data, storage)
+    meta = OF.GridMeta(g, g, g)
+    f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", storage=storage,
+                                     extrapolation=extrapolation)
+    bc = jt.periodic_boundary([[0, 0, 0], [1, 1, 1]])
+    traj = jt.create_tracker("rk4", f, bc).track_particles(x0, (0.9, 1.15), 26)
+    field_fn = lambda p, t: OF.structured_series_sample_corners(ref, times, meta, p, t, "periodic",
+                                                                extrapolation=extrapolation)
+    tt, pos = OT.track(x0, (0.9, 1.15), 26, field_fn,
+                       lambda p: OB.per_axis(p, [[0, 0, 0], [1, 1, 1]], ["periodic"] * 3), "rk4")
+    np.testing.assert_array_equal(traj.positions, pos)

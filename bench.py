@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import sys
@@ -30,6 +31,15 @@ N_PARTICLES = 1 << 24
 GRID_N = 256
 N_SNAP = 16
 DT = 2e-3
+
+
+def kernel_source_sha(files) -> str:
+    """sha256 over the named kernel sources (paths relative to the repository)."""
+    h = hashlib.sha256()
+    for f in files:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 
 def algorithmic_bytes_per_particle_step(elem_bytes: int) -> int:
@@ -447,7 +457,7 @@ def main():
                    "snapshot_layout": "z-halo"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_track<RK4,f32x3,z-halo>", "launch_ms": avg_launch_ms,
+                     "kernel": "k_track<RK4,f32x3,z-halo,fast path>", "launch_ms": avg_launch_ms,
                      "call_ms_with_cell_sort": avg_call_ms,
                      "particle_steps_per_launch": n * steps_per,
                      "bytes_per_particle_step": bpps32},
@@ -456,10 +466,18 @@ def main():
     if os.path.exists(prof):
         try:
             tr = json.load(open(prof))
-            # counters hold for one launch shape only
-            if tr.get("particles") == n and tr.get("steps_per_launch") == steps_per:
+            # counters hold for one launch shape of one kernel build only
+            if (tr.get("particles") == n and tr.get("steps_per_launch") == steps_per
+                    and tr.get("source_sha256") == kernel_source_sha(tr.get("source_files", []))):
+                f32c = tr.get("float32_25_step_launches", {})
                 result["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
                 result["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+                result["roofline"]["measured"] = {
+                    k: f32c.get(k) for k in ("l2_hit_rate", "ta_busy_fraction", "td_busy_fraction",
+                                             "valu_insts_per_wave_step", "vmem_read_insts_per_wave_step")}
+                result["roofline"]["measured"]["note"] = (
+                    "rocprofv3 counters of this kernel build at this launch shape: the kernel is bound by its "
+                    "vector-memory address/data units (TA/TD busy), not by HBM bandwidth")
         except Exception:
             pass
 

@@ -464,16 +464,12 @@ __device__ __forceinline__ void sample(const GridDev& g, const Blend& bl,
 // are exec-masked loads: lanes that reuse issue no memory requests, which is
 // the vector-memory address/data work the tracking kernel is bound by.  The
 // values used are the same stored values, so results are unchanged.
-template <int ST, bool W16 = (NodeBytes<ST>::v == 8u)>
-struct CornerCache {                 // float32 snapshots: the corner values
-    float A[8][3], B[8][3];
-    int i0, i1, j0, j1, k0;
-    const char* pa;
-    const char* pb;
-};
+// 16-bit records are unpacked inside the reload, so a stage where no lane
+// reloads skips the unpack too (measured faster than caching the raw records,
+// which take 16 fewer registers).
 template <int ST>
-struct CornerCache<ST, true> {       // 16-bit snapshots: the raw z-pair records
-    u32x4 ra[4], rb[4];
+struct CornerCache {
+    float A[8][3], B[8][3];          // the two snapshots' corner values
     int i0, i1, j0, j1, k0;
     const char* pa;
     const char* pb;
@@ -514,12 +510,24 @@ __device__ __forceinline__ void gather_cached(const GridDev& g, const Blend& bl,
                                  (unsigned)__umul24(r1 + (unsigned)j0, rs), (unsigned)__umul24(r1 + (unsigned)j1, rs)};
         const unsigned k0b = (unsigned)k0 * NB;
         if constexpr (NodeBytes<ST>::v == 8u) {
+            u32x4 ra[4], rb[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const U4 ta = *(const U4*)(pa + (row[r] + k0b));
                 const U4 tb = *(const U4*)(pb + (row[r] + k0b));
-                cc.ra[r] = u32x4{ta.v[0], ta.v[1], ta.v[2], ta.v[3]};
-                cc.rb[r] = u32x4{tb.v[0], tb.v[1], tb.v[2], tb.v[3]};
+                ra[r] = u32x4{ta.v[0], ta.v[1], ta.v[2], ta.v[3]};
+                rb[r] = u32x4{tb.v[0], tb.v[1], tb.v[2], tb.v[3]};
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float a[6], b[6];
+                unpack_pair16<ST>(ra[r], a);
+                unpack_pair16<ST>(rb[r], b);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    cc.A[2 * r][c] = a[c]; cc.A[2 * r + 1][c] = a[3 + c];
+                    cc.B[2 * r][c] = b[c]; cc.B[2 * r + 1][c] = b[3 + c];
+                }
             }
         } else {
             load_corners<ST, true>(pa, row, k0b, 0u, cc.A);
@@ -529,25 +537,8 @@ __device__ __forceinline__ void gather_cached(const GridDev& g, const Blend& bl,
     }
     cc.pa = pa;
     cc.pb = pb;
-    float A[8][3], B[8][3];
-    if constexpr (NodeBytes<ST>::v == 8u) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            float a[6], b[6];
-            unpack_pair16<ST>(cc.ra[r], a);
-            unpack_pair16<ST>(cc.rb[r], b);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                A[2 * r][c] = a[c]; A[2 * r + 1][c] = a[3 + c];
-                B[2 * r][c] = b[c]; B[2 * r + 1][c] = b[3 + c];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) { A[q][c] = cc.A[q][c]; B[q][c] = cc.B[q][c]; }
-    }
+    const float (*A)[3] = cc.A;
+    const float (*B)[3] = cc.B;
     if constexpr (IO) {
         const float al = bl.a, oma = 1.0f - bl.a;      // blend_n's 'interp' arithmetic
 #pragma unroll
@@ -581,10 +572,10 @@ inline bool periodic_fast(const GridDev& g, int node_bytes) {
 
 // Waves per SIMD the tracking kernels' register budget is sized for (128
 // VGPRs at 4).  The per-lane corner cache of the fast path holds 48 floats
-// (float32 snapshots) or 32 dwords of raw 16-bit records across stages; at
-// 4 waves that spills 92 B (float32) / 40 B (16-bit) per lane and still runs
-// faster than the spill-free 3-wave build (9.8 against 10.4 ms and 7.9
-// against 8.9 ms per 25 steps of 2^24 particles); 5 waves is far slower.
+// across stages; at 4 waves that spills 92 B (float32) / 108 B (16-bit) per
+// lane and still runs faster than the spill-free 3-wave build (9.8 against
+// 10.4 ms and 7.7 against 8.9 ms per 25 steps of 2^24 particles); 5 waves is
+// far slower.
 #ifndef JT_WAVES_F32
 #define JT_WAVES_F32 4
 #endif

@@ -72,6 +72,11 @@ extern "C" {
 #define JT_INLET_OFF  0
 #define JT_INLET_GRID 1   /* nearest inlet-grid node (:1249-1318)          */
 #define JT_INLET_ZERO 2   /* 'grid' without concentrations: off-axis = 0    */
+#define JT_INLET_HOST 3   /* 'uniform'/'gaussian': the off-axis coordinates of
+                             the exited particles are drawn on the host
+                             (np.random, :1555-1571) and placed by
+                             jt_inlet_place; not fused into the tracking
+                             kernels, which reject it                         */
 
 /* ---- descriptors (host structs, passed by pointer) ---------------------- */
 
@@ -139,6 +144,23 @@ int jt_grid_sample(const jt_grid_field* field, const float* x, int64_t n,
  * clamping/absorbing factories :215-337, mixed_boundary :377-438,
  * continuous_inlet_boundary_factory :1444-1601).  In place on x. */
 int jt_apply_boundary(const jt_boundary* bc, float* x, int64_t n, void* stream);
+
+/* Continuous inlet with host-drawn replacements (JT_INLET_HOST), split at
+ * the point where continuous_inlet_boundary_factory's closure draws from
+ * np.random (boundary.py:1524-1571).  jt_inlet_exits marks the particles
+ * that left through the outlet face (:1526-1537) and gives each its rank
+ * among them in particle order (rank[i], defined where x[i] exited) and the
+ * count (*n_exited, device int32).  The host then draws the replacement
+ * coordinates in the reference's order: n_exited values for the lower
+ * off-axis, then n_exited for the higher one.  jt_inlet_place puts exited
+ * particle i at the inlet face with draws[rank[i]] and
+ * draws[n_exited + rank[i]], then applies the backstop and the off-axis
+ * reflection to every particle (:1576-1597).  In place on x. */
+size_t jt_inlet_exits_workspace_size(int64_t n);
+int jt_inlet_exits(const jt_boundary* bc, const float* x, int64_t n, int32_t* rank, int32_t* n_exited,
+                   void* ws, size_t ws_bytes, void* stream);
+int jt_inlet_place(const jt_boundary* bc, float* x, int64_t n, const int32_t* rank, const float* draws,
+                   int64_t n_exited, void* stream);
 
 size_t jt_track_grid_workspace_size(int64_t n, int32_t n_steps,
                                     const jt_track_opts* opts);

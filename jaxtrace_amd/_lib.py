@@ -24,7 +24,7 @@ JT_BOUNDS = {"clamp": 0, "zero": 1, "nan": 2, "periodic": 3}
 JT_TIME = {"linear": 0, "nearest": 1}
 JT_EXTRAP = {"constant": 0, "linear": 1, "nan": 2, "zero": 3}
 JT_BC_NONE, JT_BC_PERIODIC, JT_BC_REFLECTIVE, JT_BC_CLAMP, JT_BC_ABSORB = 0, 1, 2, 3, 4
-JT_INLET_OFF, JT_INLET_GRID, JT_INLET_ZERO = 0, 1, 2
+JT_INLET_OFF, JT_INLET_GRID, JT_INLET_ZERO, JT_INLET_HOST = 0, 1, 2, 3
 
 c_float3 = C.c_float * 3
 c_int3 = C.c_int32 * 3
@@ -95,6 +95,11 @@ _SIGS = {
     "jt_grid_sample": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_int64, C.c_float,
                                  C.c_void_p, C.c_void_p]),
     "jt_apply_boundary": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p]),
+    "jt_inlet_exits_workspace_size": (C.c_size_t, [C.c_int64]),
+    "jt_inlet_exits": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_size_t, C.c_void_p]),
+    "jt_inlet_place": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                 C.c_int64, C.c_void_p]),
     "jt_track_grid_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.POINTER(TrackOpts)]),
     "jt_track_grid": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
                                 C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,

@@ -96,9 +96,55 @@ class BoundaryCondition:
         x = (positions.detach().to(device=dev, dtype=torch.float32) if is_torch
              else torch.from_numpy(np.asarray(positions, dtype=F32)).to(dev))
         x = x.reshape(-1, 3).clone().contiguous()
-        L.check(lib.jt_apply_boundary(C.byref(self.descriptor(dev)), L.ptr(x), x.shape[0],
-                                      L.stream_handle()), "jt_apply_boundary")
+        if self.host_inlet:
+            self._apply_host_inlet(x)
+        else:
+            L.check(lib.jt_apply_boundary(C.byref(self.descriptor(dev)), L.ptr(x), x.shape[0],
+                                          L.stream_handle()), "jt_apply_boundary")
         return x if is_torch else x.cpu().numpy()
+
+    @property
+    def host_inlet(self) -> bool:
+        """A continuous inlet whose replacements are drawn on the host
+        ('uniform'/'gaussian', boundary.py:1552-1571)."""
+        return self.inlet is not None and self.inlet["mode"] == L.JT_INLET_HOST
+
+    def _draw_inlet(self, n_exited: int) -> np.ndarray:
+        """The off-axis coordinates of n_exited replacements, drawn from the
+        global np.random state with the reference's calls in its order: all
+        of the lower off-axis, then all of the higher (boundary.py:1555-1571)."""
+        a = self.inlet["axis"]
+        cols = []
+        for i in range(3):
+            if i == a:
+                continue
+            dim_min, dim_max = self.bounds[0, i], self.bounds[1, i]
+            if self.inlet["distribution"] == "uniform":
+                cols.append(np.random.uniform(dim_min, dim_max, n_exited).astype(np.float32))
+            else:
+                center = (dim_min + dim_max) / 2
+                sigma = (dim_max - dim_min) / 6
+                cols.append(np.clip(np.random.normal(center, sigma, n_exited), dim_min, dim_max).astype(np.float32))
+        return np.concatenate(cols)
+
+    def _apply_host_inlet(self, x: torch.Tensor) -> None:
+        """In place on the device positions x (n, 3): rank the particles that
+        left through the outlet face, draw their replacements on the host, and
+        place them (jt_inlet_exits, np.random, jt_inlet_place)."""
+        lib = L.load()
+        dev = x.device
+        n = x.shape[0]
+        d = self.descriptor(dev)
+        rank = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+        wsb = int(lib.jt_inlet_exits_workspace_size(n))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        L.check(lib.jt_inlet_exits(C.byref(d), L.ptr(x), n, L.ptr(rank), L.ptr(cnt), L.ptr(ws), wsb,
+                                   L.stream_handle()), "jt_inlet_exits")
+        m = int(cnt.item())
+        draws = torch.from_numpy(self._draw_inlet(m)).to(dev) if m > 0 else None
+        L.check(lib.jt_inlet_place(C.byref(d), L.ptr(x), n, L.ptr(rank), L.ptr(draws), m, L.stream_handle()),
+                "jt_inlet_place")
 
     def __repr__(self):
         return f"BoundaryCondition({self.__name__}, kinds={self.axis_kind}, inlet={self.inlet is not None})"
@@ -164,14 +210,16 @@ def continuous_inlet_boundary_factory(bounds, flow_axis: str = "x", flow_directi
                   for i in others]
         mode = L.JT_INLET_GRID
     elif inlet_distribution in ("uniform", "gaussian"):
-        raise NotImplementedError(
-            "continuous inlet with random replacement ('uniform'/'gaussian') draws from the host "
-            "np.random state every step; the fused GPU path supports the reproducible 'grid' "
-            "distribution")
+        # the replacements come from the host np.random state every step
+        # (:1552-1571): the tracker steps with an open boundary and applies
+        # this one between steps (jt_inlet_exits / np.random / jt_inlet_place)
+        coords = [np.zeros(1, F32), np.zeros(1, F32)]
+        mode = L.JT_INLET_HOST
     else:
         coords = [np.zeros(1, F32), np.zeros(1, F32)]
         mode = L.JT_INLET_ZERO
-    inlet = {"mode": mode, "axis": a, "positive": positive, "backstop": backstop, "coords": coords}
+    inlet = {"mode": mode, "axis": a, "positive": positive, "backstop": backstop, "coords": coords,
+             "distribution": inlet_distribution}
     return BoundaryCondition("_continuous_inlet_bc", b, (L.JT_BC_NONE,) * 3, inlet=inlet)
 
 

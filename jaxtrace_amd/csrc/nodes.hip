@@ -123,6 +123,9 @@ extern "C" int jt_track_nodes(const jt_nodes_field* field, const jt_boundary* bc
     JT_REQUIRE(integrator >= JT_EULER && integrator <= JT_RK4, "bad integrator %d", integrator);
     JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
     JT_REQUIRE(n_steps >= 0 && record_every >= 1, "bad n_steps / record_every");
+    if (d.inlet == JT_INLET_HOST)
+        return fail(JT_EUNSUPPORTED, "a host-drawn inlet boundary is not fused into tracking; step with an open "
+                                     "boundary and apply it with jt_inlet_exits/jt_inlet_place");
     JT_REQUIRE(n == 0 || x != nullptr, "NULL x");
     JT_REQUIRE(n_steps == 0 || step_times != nullptr, "NULL step_times");
     hipStream_t s = (hipStream_t)stream;

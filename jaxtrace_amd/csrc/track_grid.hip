@@ -107,7 +107,7 @@ int make_bc_dev(const jt_boundary* b, BcDev& d) {
     }
     d.absorb_rows = b->absorb_rows;
     d.outside = b->outside_value;
-    JT_REQUIRE(b->inlet >= JT_INLET_OFF && b->inlet <= JT_INLET_ZERO, "bad inlet mode %d", b->inlet);
+    JT_REQUIRE(b->inlet >= JT_INLET_OFF && b->inlet <= JT_INLET_HOST, "bad inlet mode %d", b->inlet);
     d.inlet = b->inlet;
     d.ia = b->inlet_axis;
     d.ipos = b->inlet_positive ? 1 : 0;
@@ -205,6 +205,52 @@ __global__ void __launch_bounds__(256) k_boundary(BcDev bc, float* __restrict__ 
     float p[3] = {x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2]};
     if constexpr (AXES) apply_bc_axes(bc, p);
     else apply_bc(bc, p);
+    x[3 * (size_t)i] = p[0];
+    x[3 * (size_t)i + 1] = p[1];
+    x[3 * (size_t)i + 2] = p[2];
+}
+
+// ---- continuous inlet with host-drawn replacements -------------------------
+__device__ __forceinline__ bool inlet_exited(const BcDev& b, const float* p) {
+    return b.ipos ? (p[b.ia] > b.hi[b.ia]) : (p[b.ia] < b.lo[b.ia]);   // boundary.py:1526-1537
+}
+
+__global__ void __launch_bounds__(256)
+k_inlet_flags(BcDev bc, const float* __restrict__ x, int n, int* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = inlet_exited(bc, x + 3 * (size_t)i) ? 1 : 0;
+}
+
+__global__ void k_inlet_count(const int* __restrict__ flags, const int* __restrict__ rank, int n,
+                              int* __restrict__ count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[0] = rank[n - 1] + flags[n - 1];
+}
+
+__global__ void __launch_bounds__(256)
+k_inlet_place(BcDev b, float* __restrict__ x, int n, const int* __restrict__ rank, const float* __restrict__ draws,
+              long long n_exited) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2]};
+    const int a = b.ia;
+    const int o0 = (a == 0) ? 1 : 0;
+    const int o1 = (a == 2) ? 1 : 2;
+    if (inlet_exited(b, p)) {                           // :1552-1571
+        const long long r = rank[i];
+        p[a] = b.ipos ? b.lo[a] : b.hi[a];
+        p[o0] = draws[r];
+        p[o1] = draws[n_exited + r];
+    }
+    if (b.ipos) { if (p[a] < b.lo[a]) p[a] = b.backstop; }   // :1576-1585
+    else        { if (p[a] > b.hi[a]) p[a] = b.backstop; }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {                       // :1588-1597
+        if (k == a) continue;
+        float v = p[k];
+        if (v < b.lo[k]) p[k] = 2.0f * b.lo[k] - v;
+        else if (v > b.hi[k]) p[k] = 2.0f * b.hi[k] - v;
+    }
     x[3 * (size_t)i] = p[0];
     x[3 * (size_t)i + 1] = p[1];
     x[3 * (size_t)i + 2] = p[2];
@@ -351,11 +397,67 @@ extern "C" int jt_grid_sample(const jt_grid_field* field, const float* x, int64_
     return JT_OK;
 }
 
+extern "C" size_t jt_inlet_exits_workspace_size(int64_t n) {
+    Arena a(nullptr, 0);
+    const int nn = (int)(n > 0 ? n : 1);
+    a.take<int>(nn);
+    size_t temp = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const int*)nullptr, (int*)nullptr, nn);
+    a.take<char>(temp);
+    return a.used + 256;
+}
+
+extern "C" int jt_inlet_exits(const jt_boundary* bc, const float* x, int64_t n, int32_t* rank, int32_t* n_exited,
+                              void* ws, size_t ws_bytes, void* stream) {
+    BcDev d;
+    JT_REQUIRE(bc != nullptr, "boundary descriptor is NULL");
+    int rc = make_bc_dev(bc, d);
+    if (rc) return rc;
+    JT_REQUIRE(d.inlet == JT_INLET_HOST, "jt_inlet_exits needs a JT_INLET_HOST boundary");
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    JT_REQUIRE(n_exited != nullptr, "NULL n_exited");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) { JT_HIP(hipMemsetAsync(n_exited, 0, sizeof(int32_t), s)); return JT_OK; }
+    JT_REQUIRE(x && rank, "NULL x/rank");
+    Arena arena(ws, ws_bytes);
+    int* flags = arena.take<int>(n);
+    size_t temp = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const int*)nullptr, (int*)nullptr, (int)n);
+    char* tmp = arena.take<char>(temp);
+    if (!flags || !tmp || !arena.ok())
+        return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", jt_inlet_exits_workspace_size(n), ws_bytes);
+    const dim3 blk(256), grd((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_inlet_flags, grd, blk, 0, s, d, x, (int)n, flags);
+    JT_HIP(hipGetLastError());
+    JT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, temp, flags, rank, (int)n, s));
+    hipLaunchKernelGGL(k_inlet_count, dim3(1), dim3(1), 0, s, flags, rank, (int)n, n_exited);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+extern "C" int jt_inlet_place(const jt_boundary* bc, float* x, int64_t n, const int32_t* rank, const float* draws,
+                              int64_t n_exited, void* stream) {
+    BcDev d;
+    JT_REQUIRE(bc != nullptr, "boundary descriptor is NULL");
+    int rc = make_bc_dev(bc, d);
+    if (rc) return rc;
+    JT_REQUIRE(d.inlet == JT_INLET_HOST, "jt_inlet_place needs a JT_INLET_HOST boundary");
+    JT_REQUIRE(n >= 0 && n < (1LL << 31) && n_exited >= 0 && n_exited <= n, "n / n_exited out of range");
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(x && rank && (n_exited == 0 || draws), "NULL x/rank/draws");
+    hipLaunchKernelGGL(k_inlet_place, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d, x,
+                       (int)n, rank, draws, (long long)n_exited);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
 extern "C" int jt_apply_boundary(const jt_boundary* bc, float* x, int64_t n, void* stream) {
     BcDev d;
     JT_REQUIRE(bc != nullptr, "boundary descriptor is NULL");
     int rc = make_bc_dev(bc, d);
     if (rc) return rc;
+    if (d.inlet == JT_INLET_HOST)
+        return fail(JT_EUNSUPPORTED, "a host-drawn inlet boundary is applied with jt_inlet_exits/jt_inlet_place");
     JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
     if (n == 0) return JT_OK;
     JT_REQUIRE(x != nullptr, "NULL x");
@@ -404,6 +506,9 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
     JT_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
     JT_REQUIRE(record_every >= 1, "record_every must be >= 1");
+    if (d.inlet == JT_INLET_HOST)
+        return fail(JT_EUNSUPPORTED, "a host-drawn inlet boundary is not fused into tracking; step with an open "
+                                     "boundary and apply it with jt_inlet_exits/jt_inlet_place");
     JT_REQUIRE(n == 0 || x != nullptr, "NULL x");
     JT_REQUIRE(n_steps == 0 || step_times != nullptr, "NULL step_times");
     JT_REQUIRE(opts == nullptr || (opts->sort_every >= 0 && opts->steps_per_launch >= 0), "bad opts");

@@ -177,10 +177,13 @@ class ParticleTracker:
         traj = torch.empty((n_rec, n, 3), dtype=torch.float32, device=dev)
         state = x.clone()
         t0 = time.perf_counter()
-        L.check(entry(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
-                      L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
-                      rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
-                      L.stream_handle()), name)
+        if self.boundary.host_inlet:
+            self._track_host_inlet(entry, name, desc, state, n, step_times, dt, n_steps, rec, traj)
+        else:
+            L.check(entry(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
+                          L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
+                          rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
+                          L.stream_handle()), name)
         velocities = None
         if opt.record_velocities:
             velocities = torch.empty_like(traj)
@@ -197,6 +200,31 @@ class ParticleTracker:
                 velocities = velocities.cpu().numpy()
         meta["wall_s"] = time.perf_counter() - t0
         return Trajectory(traj, out_times, velocities, meta)
+
+
+    def _track_host_inlet(self, entry, name, desc, state, n, step_times, dt, n_steps, rec, traj):
+        """A continuous inlet with host-drawn replacements: one step at a time
+        with an open boundary, then the inlet boundary between steps (the
+        reference applies it after each step, tracker.py:238-239)."""
+        lib = L.load()
+        dev = state.device
+        open_bc = BoundaryCondition("_open_bc", self.boundary.bounds).descriptor(dev)
+        o = L.TrackOpts()
+        o.sort_every = 0
+        kind = getattr(self.field, "kind", "grid")
+        wsf = {"octree": lambda: lib.jt_track_octree_workspace_size(n, 1, C.byref(o)),
+               "nodes": lambda: lib.jt_track_nodes_workspace_size(n, 1)}.get(
+            kind, lambda: lib.jt_track_grid_workspace_size(n, 1, C.byref(o)))
+        ws_bytes = int(wsf())
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        traj[0] = state
+        for s in range(n_steps):
+            L.check(entry(C.byref(desc), C.byref(open_bc), _INTEGRATORS[self.integrator_name], L.ptr(state), n,
+                          L.ptr(step_times[s:s + 1]), C.c_float(F32(dt)), 1, 1, None, C.byref(o), L.ptr(ws),
+                          ws_bytes, L.stream_handle()), name)
+            self.boundary._apply_host_inlet(state)
+            if (s + 1) % rec == 0:
+                traj[(s + 1) // rec] = state
 
 
 def create_tracker(integrator_name: str, field, boundary_condition, **options) -> ParticleTracker:

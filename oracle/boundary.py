@@ -106,7 +106,8 @@ def per_axis(x, bounds, kinds):
 
 def continuous_inlet(x, bounds, flow_axis="x", flow_direction="positive",
                      inlet_distribution="grid", concentrations=None):
-    """boundary.py:1515-1599 (grid replacement :1249-1318)."""
+    """boundary.py:1515-1599 (grid replacement :1249-1318, random
+    replacement :1552-1571)."""
     b = ensure_bounds(bounds)
     a = {"x": 0, "y": 1, "z": 2}[flow_axis]
     res = np.array(x, dtype=F32, copy=True)
@@ -130,7 +131,21 @@ def continuous_inlet(x, bounds, flow_axis="x", flow_direction="positive",
                 res[p, others[0]] = g[0]
                 res[p, others[1]] = g[1]
         elif inlet_distribution in ("uniform", "gaussian"):
-            raise NotImplementedError("random inlet distributions draw from np.random")
+            # :1552-1571: draws from the global np.random state, one off-axis
+            # at a time in axis order, n_exited values each
+            n_ex = int(np.sum(exit_mask))
+            newp = np.zeros((n_ex, 3), dtype=F32)
+            newp[:, a] = inlet
+            for i in range(3):
+                if i != a:
+                    dim_min, dim_max = b[0, i], b[1, i]
+                    if inlet_distribution == "uniform":
+                        newp[:, i] = np.random.uniform(dim_min, dim_max, n_ex).astype(F32)
+                    else:
+                        center = (dim_min + dim_max) / 2
+                        sigma = (dim_max - dim_min) / 6
+                        newp[:, i] = np.clip(np.random.normal(center, sigma, n_ex), dim_min, dim_max).astype(F32)
+            res[exit_mask] = newp
         else:
             # non-grid fallback with an unknown distribution: zeros off-axis
             newp = np.zeros((int(exit_mask.sum()), 3), dtype=F32)

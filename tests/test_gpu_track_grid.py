@@ -307,3 +307,31 @@ def test_fast_path_leaves_snapshot_range(gpu, storage, extrapolation):
     tt, pos = OT.track(x0, (0.9, 1.15), 26, field_fn,
                        lambda p: OB.per_axis(p, [[0, 0, 0], [1, 1, 1]], ["periodic"] * 3), "rk4")
     np.testing.assert_array_equal(traj.positions, pos)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "gaussian"])
+@pytest.mark.parametrize("direction", ["positive", "negative"])
+def test_continuous_inlet_random_bitwise(gpu, dist, direction):
+    """'uniform'/'gaussian' inlets draw their replacements from the host
+    np.random state: the boundary call and a tracked run (step with an open
+    boundary, then jt_inlet_exits / np.random / jt_inlet_place) reproduce the
+    oracle under the same seed, step by step."""
+    b = [[0, 0, 0], [1, 1, 1]]
+    x = np.random.default_rng(9).uniform(-0.3, 1.3, (5000, 3)).astype(F32)
+    bc = jt.continuous_inlet_boundary_factory(b, "z", direction, dist)
+    np.random.seed(21)
+    got = bc(x)
+    np.random.seed(21)
+    np.testing.assert_array_equal(got, OB.continuous_inlet(x, b, "z", direction, dist))
+    data, times, g, x0 = _cfg1(4000)
+    f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic")
+    bcx = jt.continuous_inlet_boundary_factory(b, "x", direction, dist)
+    np.random.seed(5)
+    traj = jt.create_tracker("rk4", f, bcx).track_particles(x0, (0.0, 0.3), 31)
+    meta = OF.GridMeta(g, g, g)
+    field_fn = lambda p, t: OF.structured_series_sample_corners(data, times, meta, p, t, "periodic")
+    np.random.seed(5)
+    tt, pos = OT.track(x0, (0.0, 0.3), 31, field_fn,
+                       lambda p: OB.continuous_inlet(p, b, "x", direction, dist), "rk4")
+    np.testing.assert_array_equal(traj.positions, pos)
+    assert (pos[1:, :, 0] == (0.0 if direction == "positive" else 1.0)).any()   # replacements happened

@@ -46,8 +46,10 @@ def test_continuous_inlet_descriptor():
     f0, f1 = np.float32(0), np.float32(1)
     np.testing.assert_array_equal(b.inlet["coords"][1], np.linspace(f0, f1, 32, dtype=np.float32))
     assert b.inlet["backstop"] == np.float32(0.001)
-    with pytest.raises(NotImplementedError):
-        jt.continuous_inlet_boundary_factory([[0, 0, 0], [1, 1, 1]], inlet_distribution="uniform")
+    for dist in ("uniform", "gaussian"):
+        r = jt.continuous_inlet_boundary_factory([[0, 0, 0], [1, 1, 1]], inlet_distribution=dist)
+        assert r.inlet["mode"] == L.JT_INLET_HOST and r.host_inlet and r.inlet["distribution"] == dist
+    assert not b.host_inlet
 
 
 def test_tracker_options_defaults_match_reference():

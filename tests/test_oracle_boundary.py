@@ -66,3 +66,46 @@ def test_continuous_inlet_grid_snap():
     # backstop for particles behind the inlet
     assert y[2, 0] == F32(0.001)
     np.testing.assert_array_equal(y[3], x[3])
+
+
+def test_oracle_random_inlet_invariants():
+    """The random replacements of the oracle's continuous inlet
+    (boundary.py:1552-1571): exited particles land on the inlet face with
+    off-axis coordinates inside the face ([lo, hi) for 'uniform', clipped to
+    [lo, hi] for 'gaussian'); the others are untouched; exactly 2 * n_exited
+    values are drawn from the global np.random state, and the lower off-axis
+    receives the first n_exited of them (checked against a second draw from
+    the same seed); with no exit nothing is drawn."""
+    b = [[0.0, -1.0, 0.5], [1.0, 1.0, 2.5]]
+    rng = np.random.default_rng(3)
+    x = rng.uniform([0.1, -0.9, 0.6], [0.9, 0.9, 2.4], (400, 3)).astype(np.float32)
+    x[::7, 0] = 1.25                                    # 58 exit through x > 1
+    n_ex = int((x[:, 0] > 1.0).sum())
+    for dist in ("uniform", "gaussian"):
+        np.random.seed(11)
+        got = B.continuous_inlet(x, b, "x", "positive", dist)
+        after = np.random.random()
+        np.random.seed(11)
+        if dist == "uniform":
+            first = np.random.uniform(0.0, 1.0, 2 * n_ex)
+        else:
+            first = np.random.normal(0.0, 1.0, 2 * n_ex)
+        assert np.random.random() == after              # 2 * n_ex draws, no more
+        ex = x[:, 0] > 1.0
+        np.testing.assert_array_equal(got[~ex], x[~ex])
+        assert np.all(got[ex, 0] == 0.0)
+        lo, hi = np.array(b[0][1:]), np.array(b[1][1:])
+        assert np.all(got[ex, 1:] >= lo) and np.all(got[ex, 1:] <= hi)
+        # the y column comes from the first n_ex standard draws, z from the rest
+        if dist == "uniform":
+            np.testing.assert_allclose(got[ex, 1], -1.0 + 2.0 * first[:n_ex], rtol=0, atol=1e-6)
+            np.testing.assert_allclose(got[ex, 2], 0.5 + 2.0 * first[n_ex:], rtol=0, atol=1e-6)
+        else:
+            np.testing.assert_allclose(got[ex, 1], np.clip(0.0 + (2.0 / 6) * first[:n_ex], -1, 1), atol=1e-6)
+            np.testing.assert_allclose(got[ex, 2], np.clip(1.5 + (2.0 / 6) * first[n_ex:], 0.5, 2.5), atol=1e-6)
+    np.random.seed(5)
+    before = np.random.get_state()[1].copy()
+    inside = x.copy()
+    inside[:, 0] = 0.5
+    B.continuous_inlet(inside, b, "x", "positive", "uniform")
+    np.testing.assert_array_equal(np.random.get_state()[1], before)

@@ -518,11 +518,24 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
         }                                                                                                 \
     } while (0)
 
-// Hilbert index of each particle's cell for the backward sort: a block of
-// consecutive particles then starts in a compact box.
+// ---- bucket ordering for the backward --------------------------------------
+// The reverse kernel needs a block of consecutive threads to start in a
+// compact region (its LDS box), not a total order, so particles are ordered
+// by a counting sort over super-cells (2^ADJ_BUCKET_SHIFT cells a side) taken
+// along the Hilbert curve: three small kernels and a scan instead of a radix
+// sort.  The order within a super-cell is that of the atomic slot grabs.
+#ifndef ADJ_BUCKET_SHIFT
+#define ADJ_BUCKET_SHIFT 1
+#endif
+
+__global__ void __launch_bounds__(256) k_fill_zero(int* __restrict__ a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
 __global__ void __launch_bounds__(256)
-k_adj_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int bits, unsigned* __restrict__ keys,
-           int* __restrict__ idx) {
+k_bucket_count(GridDev g, const float* __restrict__ xs, int n, int shift, int cbits, unsigned* __restrict__ keys,
+               int* __restrict__ counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned c[3];
@@ -532,8 +545,47 @@ k_adj_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int bits, 
         int ci = (f != f) ? 0 : (int)fminf(fmaxf(floorf(f), 0.0f), (float)(nn[a] - 1));
         c[a] = (unsigned)ci >> shift;
     }
-    keys[i] = hilbert3(c[0], c[1], c[2], bits);
-    idx[i] = i;
+    const unsigned k = hilbert3(c[0], c[1], c[2], cbits);
+    keys[i] = k;
+    atomicAdd(&counts[k], 1);
+}
+
+__global__ void __launch_bounds__(256)
+k_bucket_scatter(const unsigned* __restrict__ keys, int n, int* __restrict__ offs, int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[atomicAdd(&offs[keys[i]], 1)] = i;
+}
+
+// Bits per axis of the super-cell lattice: the grid's Hilbert bits less
+// ADJ_BUCKET_SHIFT, and at most log2(n)/3 (about one super-cell per
+// particle at most).
+static int bucket_bits(int bits, int64_t n) {
+    int cb = bits - ADJ_BUCKET_SHIFT;
+    int lg = 0;
+    while ((2LL << lg) <= n) ++lg;
+    cb = cb < lg / 3 ? cb : lg / 3;
+    return cb < 1 ? 1 : cb;
+}
+
+static size_t bucket_scan_bytes(int nb) {
+    size_t temp = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const int*)nullptr, (int*)nullptr, nb);
+    return temp;
+}
+
+// out[j] = the index i of the j-th particle of xs in super-cell order.
+static int bucket_order(hipStream_t s, const GridDev& g, const float* xs, int n, int shift, int bits, int cb,
+                        unsigned* keys, int* counts, int* offs, char* tmp, size_t temp, int* out) {
+    const int nb = 1 << (3 * cb);
+    const dim3 b(256), gn((unsigned)((n + 255) / 256)), gb((unsigned)((nb + 255) / 256));
+    hipLaunchKernelGGL(k_fill_zero, gb, b, 0, s, counts, nb);
+    hipLaunchKernelGGL(k_bucket_count, gn, b, 0, s, g, xs, n, shift + (bits - cb), cb, keys, counts);
+    JT_HIP(hipGetLastError());
+    JT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, temp, counts, offs, nb, s));
+    hipLaunchKernelGGL(k_bucket_scatter, gn, b, 0, s, keys, n, offs, out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
 }
 
 }  // namespace jt
@@ -546,11 +598,10 @@ extern "C" size_t jt_track_grid_adjoint_workspace_size(int64_t n, int32_t n_step
     a.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
     a.take<float>(3 * nn);                                          // adjoint state
     a.take<float>(3 * nn * (ckpt_every > 0 ? ckpt_every : 1));      // segment states
-    a.take<unsigned>(nn); a.take<unsigned>(nn); a.take<int>(nn); a.take<int>(nn); a.take<int>(nn);   // cell sorts
-    size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (const int*)nullptr, (int*)nullptr, (int)nn, 0, 30);
-    a.take<char>(temp);
+    a.take<unsigned>(nn); a.take<int>(nn); a.take<int>(nn);       // bucket keys, the two orders
+    const int nb = 1 << (3 * bucket_bits(10, (int64_t)nn));         // the largest bucket lattice for n
+    a.take<int>(nb); a.take<int>(nb);                               // counts, offsets
+    a.take<char>(bucket_scan_bytes(nb));
     return a.used + 256;
 }
 
@@ -581,13 +632,12 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     float* lam = arena.take<float>((size_t)3 * n);
     float* seg = arena.take<float>((size_t)3 * n * ckpt_every);
     unsigned* keys = arena.take<unsigned>(n);
-    unsigned* skeys = arena.take<unsigned>(n);
-    int* idx = arena.take<int>(n);
     int* perm = arena.take<int>(n);                   // segment slot -> particle
     int* order = arena.take<int>(n);                  // reverse thread -> segment slot
-    size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (const int*)nullptr, (int*)nullptr, (int)n, 0, 30);
+    const int nb_max = 1 << (3 * bucket_bits(10, n));
+    int* counts = arena.take<int>(nb_max);
+    int* offs = arena.take<int>(nb_max);
+    const size_t temp = bucket_scan_bytes(nb_max);
     char* tmp = arena.take<char>(temp);
     if (!tab || !lam || !seg || !order || !tmp || !arena.ok())
         return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu",
@@ -609,7 +659,7 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     while ((maxn - 1) >> shift >= 1024) ++shift;
     int bits = 1;                                      // bits per axis of the shifted cell index
     while (((maxn - 1) >> shift) >> bits) ++bits;
-    const int end_bit = 3 * bits;
+    const int cb = bucket_bits(bits, n);
     // the blend table on the host, to cut sub-segments where the snapshot
     // pair changes (the LDS box holds two snapshot slots)
     std::vector<Blend> htab((size_t)3 * n_steps);
@@ -624,8 +674,8 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
         const int sa = c * ckpt_every;
         const int sb = sa + ckpt_every < n_steps ? sa + ckpt_every : n_steps;
         const float* xc = ckpt + (size_t)c * n * 3;
-        hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, xc, (int)n, shift, bits, keys, idx);
-        JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, perm, (int)n, 0, end_bit, s));
+        rc = bucket_order(s, g, xc, (int)n, shift, bits, cb, keys, counts, offs, tmp, temp, perm);
+        if (rc) return rc;
         if (pf) {
             switch (st) {
             case ST_F32x3: hipLaunchKernelGGL((k_adj_recompute<ST_F32x3, true, true>), g256x, b256, 0, s, g, d, xc,
@@ -661,9 +711,9 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
             fits(s_lo);
             while (s_lo > sa && s_hi - s_lo < R && fits(s_lo - 1)) --s_lo;
             // order the segment slots by their cell at the start of this sub-segment
-            hipLaunchKernelGGL(k_adj_keys, g256, b256, 0, s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift,
-                               bits, keys, idx);
-            JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, order, (int)n, 0, end_bit, s));
+            rc = bucket_order(s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift, bits, cb, keys, counts, offs,
+                              tmp, temp, order);
+            if (rc) return rc;
             if (pf) {
                 switch (st) {
                 case ST_F32x3: hipLaunchKernelGGL((k_adj_reverse<ST_F32x3, true, true>), gadj, badj, 0, s, g, d, (int)n,

@@ -62,7 +62,7 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
 #define ADJ_RESORT 8
 #endif
 #ifndef ADJ_MARGIN
-#define ADJ_MARGIN 0
+#define ADJ_MARGIN 1               // the widest box margin tried, in cells each way
 #endif
 #ifndef ADJ_REC_WAVES
 #define ADJ_REC_WAVES 4
@@ -336,15 +336,23 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     __syncthreads();
     {
         long long vol = 1;
-        int ext[3];
+        int ext[3], l3[3], h3[3];
         for (int d = 0; d < 3; ++d) {
             int l = sred[d], h = sred[3 + d];
             for (int w = 1; w < NW; ++w) { l = min(l, sred[w * 6 + d]); h = max(h, sred[w * 6 + 3 + d]); }
-            l -= ADJ_MARGIN;
-            h += 1 + ADJ_MARGIN;
-            tl.b[d] = l;
-            ext[d] = h - l + 1;
-            vol *= (ext[d] > 0 ? ext[d] : 0);
+            l3[d] = l;
+            h3[d] = h + 1;
+        }
+        // the widest margin (ADJ_MARGIN down to 0 cells each way) whose box
+        // still fits: particles drift out of a tight box during the sub-segment
+        for (int mg = ADJ_MARGIN; mg >= 0; --mg) {
+            vol = 1;
+            for (int d = 0; d < 3; ++d) {
+                tl.b[d] = l3[d] - mg;
+                ext[d] = h3[d] + mg - tl.b[d] + 1;
+                vol *= (ext[d] > 0 ? ext[d] : 0);
+            }
+            if (vol <= TILE_NODES) break;
         }
         if (vol > TILE_NODES) {                        // a cube of the largest fitting side around the
             int cap = 1;                               // block's middle particle (offset 0)

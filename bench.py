@@ -454,10 +454,10 @@ def main():
                                "x periodic / y,z reflective, FP32 field",
                    "particles": n, "grid": [GRID_N] * 3, "snapshots": N_SNAP, "dt": DT,
                    "steps_per_launch": spl, "cell_sort_every": args.sort_every,
-                   "snapshot_layout": "z-halo"},
+                   "snapshot_layout": "z-halo, 4 components per node"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_track<RK4,f32x3,z-halo,fast path>", "launch_ms": avg_launch_ms,
+                     "kernel": "k_track<RK4,f32x4,z-halo,fast path>", "launch_ms": avg_launch_ms,
                      "call_ms_with_cell_sort": avg_call_ms,
                      "particle_steps_per_launch": n * steps_per,
                      "bytes_per_particle_step": bpps32},

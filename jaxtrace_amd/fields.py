@@ -12,8 +12,10 @@ errors) while holding torch tensors on the ROCm device:
 Snapshots may be stored in float32 or in bfloat16/float16 (padded to 4
 components, 8 bytes per node) to halve the gather traffic; all arithmetic
 stays float32.  The default 'halo' layout adds one z-plane so the two
-z-corners of a cell are one contiguous load; 'packed' keeps the caller's
-float32 (T,Nx,Ny,Nz,3) array without a copy.
+z-corners of a cell are one contiguous load, with 4 components per node (16
+bytes in float32: 16-byte aligned loads, measured 6 % faster than 12-byte
+nodes); 'halo3' is the same with 3 float32 components; 'packed' keeps the
+caller's float32 (T,Nx,Ny,Nz,3) array without a copy.
 """
 from __future__ import annotations
 
@@ -70,8 +72,8 @@ class _GridSeries:
             raise ValueError(f"Unsupported extrapolation: {extrapolation}")
         if storage not in _STORAGE:
             raise ValueError(f"storage must be one of {sorted(_STORAGE)}, got {storage!r}")
-        if layout not in ("packed", "padded", "halo", "halo4"):
-            raise ValueError("layout must be 'packed', 'padded', 'halo' or 'halo4'")
+        if layout not in ("packed", "padded", "halo", "halo3", "halo4"):
+            raise ValueError("layout must be 'packed', 'padded', 'halo', 'halo3' or 'halo4'")
         gx = np.asarray(grid_x.cpu() if isinstance(grid_x, torch.Tensor) else grid_x, dtype=F32)
         gy = np.asarray(grid_y.cpu() if isinstance(grid_y, torch.Tensor) else grid_y, dtype=F32)
         gz = np.asarray(grid_z.cpu() if isinstance(grid_z, torch.Tensor) else grid_z, dtype=F32)
@@ -97,12 +99,12 @@ class _GridSeries:
         self.storage, self.layout = storage, layout
         dt_code, tdt = _STORAGE[storage]
         # packed: (T,Nx,Ny,Nz,3) f32 as given; padded: 4 components per node;
-        # halo/halo4: one extra z-plane (plane Nz repeats plane 0 for periodic
+        # halo/halo3/halo4: one extra z-plane (plane Nz repeats plane 0 for periodic
         # bounds, plane Nz-1 otherwise) so both z-corners of a cell are one load
         # 16-bit nodes are (x, y, 0, z): z in the high half of the second
         # dword over a zero low half, which the kernels read as float32 bits
-        ncomp = 3 if (storage == "float32" and layout in ("packed", "halo")) else 4
-        z_halo = 1 if layout in ("halo", "halo4") else 0
+        ncomp = 3 if (storage == "float32" and layout in ("packed", "halo3")) else 4
+        z_halo = 1 if layout in ("halo", "halo3", "halo4") else 0
         self._slots = [0, 1, 2] if storage == "float32" else [0, 1, 3]
         if ncomp == 3 and not z_halo:
             snaps = data.to(device=self.device, dtype=torch.float32).contiguous()

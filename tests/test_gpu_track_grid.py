@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 
 STORAGES = [("float32", "packed"), ("float32", "padded"), ("bfloat16", "packed"), ("float16", "packed"),
-            ("float32", "halo"), ("float32", "halo4"), ("bfloat16", "halo"), ("float16", "halo")]
+            ("float32", "halo"), ("float32", "halo3"), ("float32", "halo4"), ("bfloat16", "halo"),
+            ("float16", "halo")]
 
 
 def _rounded(data, storage):
@@ -207,7 +208,7 @@ def test_large_mixed_bc_properties(gpu):
         np.testing.assert_array_equal(p[s + 1].cpu().numpy()[idx], nxt)
 
 
-@pytest.mark.parametrize("layout", ["packed", "halo"])
+@pytest.mark.parametrize("layout", ["packed", "halo", "halo3"])
 def test_cell_sorting_is_invisible(gpu, layout):
     """Re-ordering particles by cell every k steps (and writing trajectories
     back through the permutation) leaves every recorded position bitwise
@@ -280,7 +281,7 @@ def test_periodic_fast_path_edges(gpu, storage):
     field_fn = lambda p, t: OF.structured_series_sample_corners(ref, times, meta, p, t, "periodic")
     open_bc = BoundaryCondition("_open_bc", np.array([[0, 0, 0], [1, 1, 1]], F32))
     tt, pos = OT.track(x0, (0.0, 0.1), 11, field_fn, lambda p: p, "rk4")
-    for layout in ("halo", "packed"):
+    for layout in ("halo", "halo3", "packed"):
         f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", storage=storage,
                                          layout=layout)
         traj = jt.create_tracker("rk4", f, open_bc).track_particles(x0, (0.0, 0.1), 11)

@@ -536,14 +536,9 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
 #define ADJ_BUCKET_SHIFT 1
 #endif
 
-__global__ void __launch_bounds__(256) k_fill_zero(int* __restrict__ a, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0;
-}
-
+// Hilbert index of each particle's super-cell.
 __global__ void __launch_bounds__(256)
-k_bucket_count(GridDev g, const float* __restrict__ xs, int n, int shift, int cbits, unsigned* __restrict__ keys,
-               int* __restrict__ counts) {
+k_bucket_keys(GridDev g, const float* __restrict__ xs, int n, int shift, int cbits, unsigned* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned c[3];
@@ -553,16 +548,7 @@ k_bucket_count(GridDev g, const float* __restrict__ xs, int n, int shift, int cb
         int ci = (f != f) ? 0 : (int)fminf(fmaxf(floorf(f), 0.0f), (float)(nn[a] - 1));
         c[a] = (unsigned)ci >> shift;
     }
-    const unsigned k = hilbert3(c[0], c[1], c[2], cbits);
-    keys[i] = k;
-    atomicAdd(&counts[k], 1);
-}
-
-__global__ void __launch_bounds__(256)
-k_bucket_scatter(const unsigned* __restrict__ keys, int n, int* __restrict__ offs, int* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[atomicAdd(&offs[keys[i]], 1)] = i;
+    keys[i] = hilbert3(c[0], c[1], c[2], cbits);
 }
 
 // Bits per axis of the super-cell lattice: the grid's Hilbert bits less
@@ -576,23 +562,14 @@ static int bucket_bits(int bits, int64_t n) {
     return cb < 1 ? 1 : cb;
 }
 
-static size_t bucket_scan_bytes(int nb) {
-    size_t temp = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const int*)nullptr, (int*)nullptr, nb);
-    return temp;
-}
-
 // out[j] = the index i of the j-th particle of xs in super-cell order.
 static int bucket_order(hipStream_t s, const GridDev& g, const float* xs, int n, int shift, int bits, int cb,
                         unsigned* keys, int* counts, int* offs, char* tmp, size_t temp, int* out) {
     const int nb = 1 << (3 * cb);
-    const dim3 b(256), gn((unsigned)((n + 255) / 256)), gb((unsigned)((nb + 255) / 256));
-    hipLaunchKernelGGL(k_fill_zero, gb, b, 0, s, counts, nb);
-    hipLaunchKernelGGL(k_bucket_count, gn, b, 0, s, g, xs, n, shift + (bits - cb), cb, keys, counts);
+    hipLaunchKernelGGL(k_bucket_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, xs, n,
+                       shift + (bits - cb), cb, keys);
     JT_HIP(hipGetLastError());
-    JT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, temp, counts, offs, nb, s));
-    hipLaunchKernelGGL(k_bucket_scatter, gn, b, 0, s, keys, n, offs, out);
-    JT_HIP(hipGetLastError());
+    JT_HIP(bucket_sort(s, keys, n, 0, nb, counts, offs, tmp, temp, out));
     return JT_OK;
 }
 

@@ -16,6 +16,11 @@
 //   inside the 1e-5 density bar).  Each dropped particle is farther than R
 //   from the point, so each dropped term is below norm*exp(-R^2/(2h^2)).
 #include "jt_common.h"
+#include <algorithm>
+
+#ifndef KDE_VALU
+#define KDE_VALU 0                 // 1: the 8 x 8 x 16-tile VALU kernel instead of the MFMA one
+#endif
 
 namespace jt {
 
@@ -147,6 +152,191 @@ k_kde_grid_cutoff(const float* __restrict__ gx, int nx, const float* __restrict_
             if (iz0 + q < nz) out[((size_t)iy * nx + ix) * nz + iz0 + q] = (float)(acc[q] * scale);
 }
 
+// k_kde_tile_mfma: the same sum with 16 x 16 x 16 tiles on the matrix cores.
+// For a tile and a batch of particles p the densities are a matrix product,
+//   D[(x, y), z] += sum_p (ex[x, p] * ey[y, p]) * ez[p, z],
+// with the separable factors e*[., p] = exp(-0.5 d^2 / h^2) per tile
+// coordinate.  The larger tile shares the 2R halo of its box among 4x the
+// points of the 8 x 8 x 16 tile above.  Each of the 4 waves owns 4 of the
+// tile's 16 y rows: per 4 particles it issues one v_mfma_f32_16x16x4_f32 per
+// row (A: 16 x by 4 particles, B: 4 particles by 16 z), so a pair costs an
+// MFMA quarter-slot instead of a VALU multiply-add.  The fp32 MFMA sums run
+// over one batch of 256 particles and are then added in float64, as in the
+// VALU kernel.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+#define KDE_T 16
+#define KDE_BATCH 256
+
+__global__ void __launch_bounds__(256) k_kde_zero(float* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = 0.0f;
+}
+
+// nsplit blocks share a tile: block b takes tile b / nsplit and, of the
+// tile's cell columns (cx, cy), those with index = b % nsplit modulo nsplit,
+// adding its partial densities into out (zeroed first).  The particle
+// counts of the tiles' boxes differ widely on clustered data, so one block
+// per tile leaves a tail of heavy blocks running alone.
+__global__ void __launch_bounds__(256)
+k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
+                const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
+                int nsplit, float* __restrict__ out) {
+    __shared__ float ex[KDE_BATCH][KDE_T];
+    __shared__ float4 ey[KDE_BATCH][KDE_T / 4];
+    __shared__ float ez[KDE_BATCH][KDE_T];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tiles_z = (nz + KDE_T - 1) / KDE_T, tiles_x = (nx + KDE_T - 1) / KDE_T;
+    const int part = blockIdx.x % nsplit;
+    int b = blockIdx.x / nsplit;
+    const int bz = b % tiles_z; b /= tiles_z;
+    const int bx = b % tiles_x; b /= tiles_x;
+    const int by = b;
+    // the tile's coordinates (clamped at the grid's end; clamped points are not written)
+    float xs[KDE_T], ys[KDE_T], zs[KDE_T];
+#pragma unroll
+    for (int i = 0; i < KDE_T; ++i) {
+        xs[i] = gx[min(bx * KDE_T + i, nx - 1)];
+        ys[i] = gy[min(by * KDE_T + i, ny - 1)];
+        zs[i] = gz[min(bz * KDE_T + i, nz - 1)];
+    }
+    float lo[3] = {xs[0], ys[0], zs[0]}, hi[3] = {xs[0], ys[0], zs[0]};
+#pragma unroll
+    for (int i = 1; i < KDE_T; ++i) {
+        lo[0] = fminf(lo[0], xs[i]); hi[0] = fmaxf(hi[0], xs[i]);
+        lo[1] = fminf(lo[1], ys[i]); hi[1] = fmaxf(hi[1], ys[i]);
+        lo[2] = fminf(lo[2], zs[i]); hi[2] = fmaxf(hi[2], zs[i]);
+    }
+    int c0[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+        c0[a] = clampi((int)floorf((lo[a] - R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
+        c1[a] = clampi((int)floorf((hi[a] + R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
+    }
+    const float nh = -0.5f * inv_h2;
+    const int col = lane & 15, kq = lane >> 4;   // MFMA operand lane: A row / B column, and k
+    __shared__ int wcount[4];
+    double accd[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) accd[r][q] = 0.0;
+    const int ncy = c1[1] - c0[1] + 1, ncol = (c1[0] - c0[0] + 1) * ncy;
+    for (int k = part; k < ncol; k += nsplit) {
+        {
+            const int cx = c0[0] + k / ncy, cy = c0[1] + k % ncy;
+            // cells (cx, cy, c0z..c1z) are consecutive ids: their points are one run
+            const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + c0[2];
+            const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + c1[2];
+            int s = 0x7fffffff, e = 0;
+            for (int id = id0; id <= id1; ++id) {
+                const int cs = g.cstart[id], ce = g.cend[id];
+                if (ce > cs) { s = min(s, cs); e = max(e, ce); }
+            }
+            if (s >= e) continue;                         // no particles in these cells (s is still INT_MAX)
+            // this thread's particle of the next batch, loaded one batch ahead
+            float qx = 0.f, qy = 0.f, qz = 0.f;
+            if (s + tid < e) {
+                qx = g.spos[(size_t)(s + tid) * 3]; qy = g.spos[(size_t)(s + tid) * 3 + 1];
+                qz = g.spos[(size_t)(s + tid) * 3 + 2];
+            }
+            for (int base = s; base < e; base += KDE_BATCH) {
+                const int j = base + tid;
+                const float px = qx, py = qy, pz = qz;
+                if ((long long)j + KDE_BATCH < (long long)e) {
+                    qx = g.spos[(size_t)(j + KDE_BATCH) * 3]; qy = g.spos[(size_t)(j + KDE_BATCH) * 3 + 1];
+                    qz = g.spos[(size_t)(j + KDE_BATCH) * 3 + 2];
+                }
+                // keep the particles within R of the tile's box on every axis
+                // (the hash cells overshoot it); each one dropped is farther
+                // than R from every tile point
+                const bool keep = j < e && px >= lo[0] - R && px <= hi[0] + R && py >= lo[1] - R &&
+                                  py <= hi[1] + R && pz >= lo[2] - R && pz <= hi[2] + R;
+                // batch slots in thread order (deterministic): wave counts, then offsets
+                const unsigned long long kb = __ballot(keep);
+                if (lane == 0) wcount[wv] = __popcll(kb);
+                __syncthreads();
+                int off = 0, cnt = 0;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) { off += w < wv ? wcount[w] : 0; cnt += wcount[w]; }
+                const int cnt4 = (cnt + 3) & ~3;          // padded with zero factors to whole MFMA k-steps
+                if (tid < cnt4 - cnt) {                   // the padding slots cnt .. cnt4-1
+                    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int i = 0; i < KDE_T; i += 4) {
+                        *reinterpret_cast<float4*>(&ex[cnt + tid][i]) = z;
+                        ey[cnt + tid][i / 4] = z;
+                        *reinterpret_cast<float4*>(&ez[cnt + tid][i]) = z;
+                    }
+                }
+                if (keep) {
+                    const int slot = off + __popcll(kb & ((1ull << lane) - 1ull));
+                    float fx[KDE_T], fy[KDE_T], fz[KDE_T];
+#pragma unroll
+                    for (int i = 0; i < KDE_T; ++i) {
+                        const float dx = xs[i] - px, dy = ys[i] - py, dz = zs[i] - pz;
+                        fx[i] = __expf(nh * (dx * dx));
+                        fy[i] = __expf(nh * (dy * dy));
+                        fz[i] = __expf(nh * (dz * dz));
+                    }
+#pragma unroll
+                    for (int i = 0; i < KDE_T; i += 4) {
+                        *reinterpret_cast<float4*>(&ex[slot][i]) = make_float4(fx[i], fx[i + 1], fx[i + 2], fx[i + 3]);
+                        ey[slot][i / 4] = make_float4(fy[i], fy[i + 1], fy[i + 2], fy[i + 3]);
+                        *reinterpret_cast<float4*>(&ez[slot][i]) = make_float4(fz[i], fz[i + 1], fz[i + 2], fz[i + 3]);
+                    }
+                }
+                __syncthreads();
+                floatx4 acc[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                int p0 = 0;
+                for (; p0 + 16 <= cnt4; p0 += 16) {        // 4 k-steps: their LDS reads first
+                    float ax[4], bz[4];
+                    float4 y4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int p = p0 + 4 * u + kq;
+                        ax[u] = ex[p][col];
+                        bz[u] = ez[p][col];
+                        y4[u] = ey[p][wv];                 // this wave's rows 4 wv .. 4 wv + 3
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].x, bz[u], acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].y, bz[u], acc[1], 0, 0, 0);
+                        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].z, bz[u], acc[2], 0, 0, 0);
+                        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].w, bz[u], acc[3], 0, 0, 0);
+                    }
+                }
+                for (; p0 < cnt4; p0 += 4) {
+                    const int p = p0 + kq;
+                    const float ax = ex[p][col], bz4 = ez[p][col];
+                    const float4 y4 = ey[p][wv];
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.x, bz4, acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.y, bz4, acc[1], 0, 0, 0);
+                    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.z, bz4, acc[2], 0, 0, 0);
+                    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.w, bz4, acc[3], 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) accd[r][q] += (double)acc[r][q];
+                __syncthreads();
+            }
+        }
+    }
+    // D of row r (y = 4 wv + r): lane holds x = 4 (lane / 16) + q, z = lane % 16
+    const int iz = bz * KDE_T + col;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int iy = by * KDE_T + 4 * wv + r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ix = bx * KDE_T + 4 * kq + q;
+            if (ix < nx && iy < ny && iz < nz && accd[r][q] != 0.0)
+                unsafeAtomicAdd(&out[((size_t)iy * nx + ix) * nz + iz], (float)(accd[r][q] * scale));
+        }
+    }
+}
+
 }  // namespace jt
 
 using namespace jt;
@@ -182,10 +372,23 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
     const double hh = (double)h;
     const float inv_h2 = (float)(1.0 / (hh * hh));
     const double norm = pow(2.0 * M_PI, -1.5);
+#if KDE_VALU
     long long blocks = (long long)((ny + 7) / 8) * ((nx + 7) / 8) * ((nz + 15) / 16);
     JT_REQUIRE(blocks < (1LL << 31), "grid too large");
     hipLaunchKernelGGL(k_kde_grid_cutoff, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, out);
+#else
+    const long long tiles = (long long)((ny + KDE_T - 1) / KDE_T) * ((nx + KDE_T - 1) / KDE_T) * ((nz + KDE_T - 1) / KDE_T);
+    // about 16k blocks (the chip holds 768 at a time), at most 16 per tile
+    const int nsplit = (int)std::max(1LL, std::min(16LL, 16384 / tiles));
+    JT_REQUIRE(tiles * nsplit < (1LL << 31), "grid too large");
+    const size_t npts = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_kde_zero, dim3((unsigned)std::min<size_t>((npts + 255) / 256, 4096)), dim3(256), 0,
+                       (hipStream_t)stream, out, npts);
+    JT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_kde_tile_mfma, dim3((unsigned)(tiles * nsplit)), dim3(256), 0, (hipStream_t)stream,
+                       gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, nsplit, out);
+#endif
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

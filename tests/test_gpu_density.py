@@ -97,6 +97,22 @@ def test_kde_exhaustive_and_cutoff(gpu):
                                rtol=2e-5, atol=1e-6)
 
 
+def test_kde_cutoff_tiles_sampled(gpu):
+    """Cut-off KDE on a 40^3 grid (16^3 tiles, the last one 8 points deep on
+    each axis) over 2^16 mixture particles: 4096 sampled grid points against
+    the float64 exhaustive sum (kde.py:243-253) at the 1e-5 bar."""
+    P = W.gaussian_mixture(1 << 16, seed=11)
+    cut = jt.KDEEstimator(P, resolution=40, bounds=((0, 1),) * 3, cutoff_sigmas=7.0)
+    d = cut.density_grid_3d().cpu().numpy().ravel()
+    X, Y, Z = OD.make_grid_3d(P, 40, ((0, 1),) * 3)
+    G = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    idx = np.random.default_rng(12).choice(len(G), 4096, replace=False)
+    want = OD.kde_eval(G[idx], P, cut.h, dtype=np.float64)
+    m = want >= 1e-3 * want.max()
+    assert m.sum() > 1000
+    np.testing.assert_allclose(d[idx][m], want[m], rtol=1e-5)
+
+
 def test_full_size_sph_sampled_queries(gpu):
     """2^24 Gaussian-mixture particles, h = 2/256: neighbour lists of 2048
     sampled grid queries bit-exact with the oracle in both key modes."""

@@ -246,10 +246,12 @@ int jt_sph_eval(const jt_hashgrid* g, const float* Q, int64_t m, float radius,
 int jt_kde_eval(const float* P, int64_t n, int32_t dim, const float* Q, int64_t m,
                 float h, float* out, void* stream);
 /* The same density on the structured grid of KDEEstimator.evaluate_3d
- * (kde.py:188-240; out is (ny, nx, nz), 'xy' meshgrid order), summing only
- * points within `cutoff` of each grid point, found through a 3-D JT_KEY_X64
- * hash grid of the points whose cell size is >= cutoff.  n_total is the
- * number of points (the mean's divisor). */
+ * (kde.py:188-240; out is (ny, nx, nz), 'xy' meshgrid order), summing every
+ * point within `cutoff` of each grid point (and some farther ones), found
+ * through a 3-D JT_KEY_X64 hash grid of the points whose cell size is
+ * >= cutoff / 8.  n_total is the number of points (the mean's divisor).
+ * Partial sums are added with float atomics, so the last float32 bits may
+ * differ between calls. */
 int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
                        const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
                        float cutoff, float* out, void* stream);

@@ -4,7 +4,10 @@
 // k_kde_exhaustive: the reference's all-pairs algorithm.  One thread per
 //   query; particles stream through LDS in tiles of 256 and are reused by all
 //   256 queries of the block.  Terms are accumulated in float64.
-// k_kde_grid_cutoff: for the structured evaluation grid of evaluate_3d
+// k_kde_tile_mfma (below; jt_kde_grid_cutoff runs it): the cut-off sum on
+//   the structured grid on the matrix cores, 16 x 16 x 16 tiles.
+// k_kde_grid_cutoff (built with KDE_VALU=1 instead; 5x slower at 256^3): for
+//   the structured evaluation grid of evaluate_3d
 //   (meshgrid 'xy' order, index (iy*nx + ix)*nz + iz).  A block owns an
 //   8 x 8 x 16 tile of grid points (a thread: one (x, y) column, 4 z points)
 //   and streams, through LDS, the particles of the hash-grid cells that

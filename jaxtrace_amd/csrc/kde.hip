@@ -168,7 +168,15 @@ k_kde_grid_cutoff(const float* __restrict__ gx, int nx, const float* __restrict_
 // VALU kernel.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 #define KDE_T 16
-#define KDE_BATCH 256
+#define KDE_BATCH 256                                 // particles read per round (one per thread)
+#define KDE_SLOTS 240                                 // particles per MFMA pass (60 k-steps)
+#define KDE_STAGE (KDE_SLOTS + KDE_BATCH - 1)         // staged positions at most
+#ifndef KDE_TARGET_BLOCKS
+#define KDE_TARGET_BLOCKS 32768
+#endif
+#ifndef KDE_MAX_SPLIT
+#define KDE_MAX_SPLIT 16
+#endif
 
 __global__ void __launch_bounds__(256) k_kde_zero(float* __restrict__ out, size_t n) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = 0.0f;
@@ -183,9 +191,9 @@ __global__ void __launch_bounds__(256)
 k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
                 const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
                 int nsplit, float* __restrict__ out) {
-    __shared__ float ex[KDE_BATCH][KDE_T];
-    __shared__ float4 ey[KDE_BATCH][KDE_T / 4];
-    __shared__ float ez[KDE_BATCH][KDE_T];
+    __shared__ float ex[KDE_SLOTS][KDE_T];
+    __shared__ float4 ey[KDE_SLOTS][KDE_T / 4];
+    __shared__ float ez[KDE_SLOTS][KDE_T];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tiles_z = (nz + KDE_T - 1) / KDE_T, tiles_x = (nx + KDE_T - 1) / KDE_T;
     const int part = blockIdx.x % nsplit;
@@ -215,116 +223,139 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
     }
     const float nh = -0.5f * inv_h2;
     const int col = lane & 15, kq = lane >> 4;   // MFMA operand lane: A row / B column, and k
-    __shared__ int wcount[4];
+    __shared__ int wcount[2][4];
+    __shared__ float sx[KDE_STAGE], sy[KDE_STAGE], sz[KDE_STAGE];
     double accd[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) accd[r][q] = 0.0;
-    const int ncy = c1[1] - c0[1] + 1, ncol = (c1[0] - c0[0] + 1) * ncy;
-    for (int k = part; k < ncol; k += nsplit) {
-        {
-            const int cx = c0[0] + k / ncy, cy = c0[1] + k % ncy;
-            // cells (cx, cy, c0z..c1z) are consecutive ids: their points are one run
-            const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + c0[2];
-            const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + c1[2];
-            int s = 0x7fffffff, e = 0;
-            for (int id = id0; id <= id1; ++id) {
-                const int cs = g.cstart[id], ce = g.cend[id];
-                if (ce > cs) { s = min(s, cs); e = max(e, ce); }
+    // Sum the first n (<= KDE_SLOTS) staged particles: their factors (one
+    // particle per thread), then the MFMA k-steps over them.  Block-uniform.
+    auto consume = [&](int n) {
+        const int n4 = (n + 3) & ~3;              // padded with zero factors to whole k-steps
+        if (tid < n4) {
+            const bool real = tid < n;
+            const float px = real ? sx[tid] : 0.0f, py = real ? sy[tid] : 0.0f, pz = real ? sz[tid] : 0.0f;
+            float fx[KDE_T], fy[KDE_T], fz[KDE_T];
+#pragma unroll
+            for (int i = 0; i < KDE_T; ++i) {
+                const float dx = xs[i] - px, dy = ys[i] - py, dz = zs[i] - pz;
+                fx[i] = real ? __expf(nh * (dx * dx)) : 0.0f;
+                fy[i] = __expf(nh * (dy * dy));
+                fz[i] = __expf(nh * (dz * dz));
             }
-            if (s >= e) continue;                         // no particles in these cells (s is still INT_MAX)
-            // this thread's particle of the next batch, loaded one batch ahead
-            float qx = 0.f, qy = 0.f, qz = 0.f;
-            if (s + tid < e) {
-                qx = g.spos[(size_t)(s + tid) * 3]; qy = g.spos[(size_t)(s + tid) * 3 + 1];
-                qz = g.spos[(size_t)(s + tid) * 3 + 2];
-            }
-            for (int base = s; base < e; base += KDE_BATCH) {
-                const int j = base + tid;
-                const float px = qx, py = qy, pz = qz;
-                if ((long long)j + KDE_BATCH < (long long)e) {
-                    qx = g.spos[(size_t)(j + KDE_BATCH) * 3]; qy = g.spos[(size_t)(j + KDE_BATCH) * 3 + 1];
-                    qz = g.spos[(size_t)(j + KDE_BATCH) * 3 + 2];
-                }
-                // keep the particles within R of the tile's box on every axis
-                // (the hash cells overshoot it); each one dropped is farther
-                // than R from every tile point
-                const bool keep = j < e && px >= lo[0] - R && px <= hi[0] + R && py >= lo[1] - R &&
-                                  py <= hi[1] + R && pz >= lo[2] - R && pz <= hi[2] + R;
-                // batch slots in thread order (deterministic): wave counts, then offsets
-                const unsigned long long kb = __ballot(keep);
-                if (lane == 0) wcount[wv] = __popcll(kb);
-                __syncthreads();
-                int off = 0, cnt = 0;
 #pragma unroll
-                for (int w = 0; w < 4; ++w) { off += w < wv ? wcount[w] : 0; cnt += wcount[w]; }
-                const int cnt4 = (cnt + 3) & ~3;          // padded with zero factors to whole MFMA k-steps
-                if (tid < cnt4 - cnt) {                   // the padding slots cnt .. cnt4-1
-                    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                    for (int i = 0; i < KDE_T; i += 4) {
-                        *reinterpret_cast<float4*>(&ex[cnt + tid][i]) = z;
-                        ey[cnt + tid][i / 4] = z;
-                        *reinterpret_cast<float4*>(&ez[cnt + tid][i]) = z;
-                    }
-                }
-                if (keep) {
-                    const int slot = off + __popcll(kb & ((1ull << lane) - 1ull));
-                    float fx[KDE_T], fy[KDE_T], fz[KDE_T];
-#pragma unroll
-                    for (int i = 0; i < KDE_T; ++i) {
-                        const float dx = xs[i] - px, dy = ys[i] - py, dz = zs[i] - pz;
-                        fx[i] = __expf(nh * (dx * dx));
-                        fy[i] = __expf(nh * (dy * dy));
-                        fz[i] = __expf(nh * (dz * dz));
-                    }
-#pragma unroll
-                    for (int i = 0; i < KDE_T; i += 4) {
-                        *reinterpret_cast<float4*>(&ex[slot][i]) = make_float4(fx[i], fx[i + 1], fx[i + 2], fx[i + 3]);
-                        ey[slot][i / 4] = make_float4(fy[i], fy[i + 1], fy[i + 2], fy[i + 3]);
-                        *reinterpret_cast<float4*>(&ez[slot][i]) = make_float4(fz[i], fz[i + 1], fz[i + 2], fz[i + 3]);
-                    }
-                }
-                __syncthreads();
-                floatx4 acc[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-                int p0 = 0;
-                for (; p0 + 16 <= cnt4; p0 += 16) {        // 4 k-steps: their LDS reads first
-                    float ax[4], bz[4];
-                    float4 y4[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int p = p0 + 4 * u + kq;
-                        ax[u] = ex[p][col];
-                        bz[u] = ez[p][col];
-                        y4[u] = ey[p][wv];                 // this wave's rows 4 wv .. 4 wv + 3
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].x, bz[u], acc[0], 0, 0, 0);
-                        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].y, bz[u], acc[1], 0, 0, 0);
-                        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].z, bz[u], acc[2], 0, 0, 0);
-                        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].w, bz[u], acc[3], 0, 0, 0);
-                    }
-                }
-                for (; p0 < cnt4; p0 += 4) {
-                    const int p = p0 + kq;
-                    const float ax = ex[p][col], bz4 = ez[p][col];
-                    const float4 y4 = ey[p][wv];
-                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.x, bz4, acc[0], 0, 0, 0);
-                    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.y, bz4, acc[1], 0, 0, 0);
-                    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.z, bz4, acc[2], 0, 0, 0);
-                    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.w, bz4, acc[3], 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) accd[r][q] += (double)acc[r][q];
-                __syncthreads();
+            for (int i = 0; i < KDE_T; i += 4) {
+                *reinterpret_cast<float4*>(&ex[tid][i]) = make_float4(fx[i], fx[i + 1], fx[i + 2], fx[i + 3]);
+                ey[tid][i / 4] = make_float4(fy[i], fy[i + 1], fy[i + 2], fy[i + 3]);
+                *reinterpret_cast<float4*>(&ez[tid][i]) = make_float4(fz[i], fz[i + 1], fz[i + 2], fz[i + 3]);
             }
         }
+        __syncthreads();
+        floatx4 acc[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        int p0 = 0;
+        for (; p0 + 16 <= n4; p0 += 16) {             // 4 k-steps: their LDS reads first
+            float ax[4], bz[4];
+            float4 y4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int p = p0 + 4 * u + kq;
+                ax[u] = ex[p][col];
+                bz[u] = ez[p][col];
+                y4[u] = ey[p][wv];                     // this wave's rows 4 wv .. 4 wv + 3
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].x, bz[u], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].y, bz[u], acc[1], 0, 0, 0);
+                acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].z, bz[u], acc[2], 0, 0, 0);
+                acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].w, bz[u], acc[3], 0, 0, 0);
+            }
+        }
+        for (; p0 < n4; p0 += 4) {
+            const int p = p0 + kq;
+            const float ax = ex[p][col], bz4 = ez[p][col];
+            const float4 y4 = ey[p][wv];
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.x, bz4, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.y, bz4, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.z, bz4, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.w, bz4, acc[3], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) accd[r][q] += (double)acc[r][q];
+        __syncthreads();                               // factors and staged positions are rewritten next
+    };
+    int staged = 0, round = 0;                         // block-uniform
+    const int ncy = c1[1] - c0[1] + 1, ncol = (c1[0] - c0[0] + 1) * ncy;
+    for (int k = part; k < ncol; k += nsplit) {
+        const int cx = c0[0] + k / ncy, cy = c0[1] + k % ncy;
+        // cells (cx, cy, c0z..c1z) are consecutive ids: their points are one run
+        const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + c0[2];
+        const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + c1[2];
+        int s = 0x7fffffff, e = 0;
+        for (int id = id0; id <= id1; ++id) {
+            const int cs = g.cstart[id], ce = g.cend[id];
+            if (ce > cs) { s = min(s, cs); e = max(e, ce); }
+        }
+        if (s >= e) continue;                          // no particles in these cells (s is still INT_MAX)
+        // this thread's particle of the next round, loaded one round ahead
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        if ((long long)s + tid < (long long)e) {
+            qx = g.spos[(size_t)(s + tid) * 3]; qy = g.spos[(size_t)(s + tid) * 3 + 1];
+            qz = g.spos[(size_t)(s + tid) * 3 + 2];
+        }
+        for (int base = s; base < e; base += KDE_BATCH) {
+            const int j = base + tid;
+            const float px = qx, py = qy, pz = qz;
+            if ((long long)j + KDE_BATCH < (long long)e) {
+                qx = g.spos[(size_t)(j + KDE_BATCH) * 3]; qy = g.spos[(size_t)(j + KDE_BATCH) * 3 + 1];
+                qz = g.spos[(size_t)(j + KDE_BATCH) * 3 + 2];
+            }
+            // keep the particles within R of the tile's box on every axis
+            // (the hash cells overshoot it); each one dropped is farther
+            // than R from every tile point
+            const bool keep = j < e && px >= lo[0] - R && px <= hi[0] + R && py >= lo[1] - R &&
+                              py <= hi[1] + R && pz >= lo[2] - R && pz <= hi[2] + R;
+            // staging slots in thread order (deterministic): wave counts, then
+            // offsets; the counts alternate between two rows, as a wave may
+            // write the next round's count while another still reads these
+            const unsigned long long kb = __ballot(keep);
+            if (lane == 0) wcount[round & 1][wv] = __popcll(kb);
+            __syncthreads();
+            int off = 0, cnt = 0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int c = wcount[round & 1][w];
+                off += w < wv ? c : 0;
+                cnt += c;
+            }
+            ++round;
+            if (keep) {
+                const int slot = staged + off + __popcll(kb & ((1ull << lane) - 1ull));
+                sx[slot] = px; sy[slot] = py; sz[slot] = pz;
+            }
+            staged += cnt;
+            while (staged >= KDE_SLOTS) {
+                __syncthreads();                       // the staged positions are visible
+                consume(KDE_SLOTS);
+                // move the rest to the front
+                const int rest = staged - KDE_SLOTS;
+                float a = 0.f, b2 = 0.f, c2 = 0.f;
+                if (tid < rest) { a = sx[KDE_SLOTS + tid]; b2 = sy[KDE_SLOTS + tid]; c2 = sz[KDE_SLOTS + tid]; }
+                __syncthreads();
+                if (tid < rest) { sx[tid] = a; sy[tid] = b2; sz[tid] = c2; }
+                staged = rest;
+            }
+        }
+    }
+    if (staged > 0) {
+        __syncthreads();
+        consume(staged);
     }
     // D of row r (y = 4 wv + r): lane holds x = 4 (lane / 16) + q, z = lane % 16
     const int iz = bz * KDE_T + col;
@@ -382,8 +413,8 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
                        gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, out);
 #else
     const long long tiles = (long long)((ny + KDE_T - 1) / KDE_T) * ((nx + KDE_T - 1) / KDE_T) * ((nz + KDE_T - 1) / KDE_T);
-    // about 16k blocks (the chip holds 768 at a time), at most 16 per tile
-    const int nsplit = (int)std::max(1LL, std::min(16LL, 16384 / tiles));
+    // about KDE_TARGET_BLOCKS blocks (the chip holds 768 at a time), at most KDE_MAX_SPLIT per tile
+    const int nsplit = (int)std::max(1LL, std::min((long long)KDE_MAX_SPLIT, (long long)KDE_TARGET_BLOCKS / tiles));
     JT_REQUIRE(tiles * nsplit < (1LL << 31), "grid too large");
     const size_t npts = (size_t)nx * ny * nz;
     hipLaunchKernelGGL(k_kde_zero, dim3((unsigned)std::min<size_t>((npts + 255) / 256, 4096)), dim3(256), 0,

@@ -264,6 +264,68 @@ k_query(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K, in
         if (lane == 0) out_rho[qi] = acc;
         return;
     }
+    if (MODE == 1 && masses == nullptr && hs == nullptr) {
+        // The K smallest keys without sorting: a radix select, 8 bits a pass,
+        // over the distance bits (non-negative floats order as their bits)
+        // finds the K-th smallest distance Td and the rank kr of that key
+        // among the keys of distance Td.  The K nearest are then every
+        // candidate nearer than Td and kr + 1 candidates at Td, and with
+        // uniform masses and smoothing the sum needs no more (nvalid > K, so
+        // all of them are within r).  Each wave has 256 counters after the
+        // candidate slices.
+        unsigned* hist = (unsigned*)((int*)(lds + (size_t)waves * nc_pad) + (size_t)waves * nc_pad) + wv * 256;
+        unsigned prefix = 0u, pmask = 0u;
+        int kr = K - 1;                                 // rank sought among the keys under the prefix
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            for (int b = lane; b < 256; b += 64) hist[b] = 0u;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            for (int c = lane; c < nc; c += 64) {
+                const unsigned db = (unsigned)(keys[c] >> 32);
+                if ((db & pmask) == prefix) atomicAdd(&hist[(db >> shift) & 255u], 1u);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                           h3 = hist[4 * lane + 3];
+            const unsigned tot = h0 + h1 + h2 + h3;
+            unsigned incl = tot;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            const unsigned excl = incl - tot;
+            const bool mine = (unsigned)kr >= excl && (unsigned)kr < incl;   // exactly one lane
+            const int src = __builtin_ctzll(__ballot(mine));
+            int bin = 4 * lane;
+            unsigned below = excl, r2 = (unsigned)kr - excl;
+            if (r2 >= h0) {
+                r2 -= h0; below += h0; ++bin;
+                if (r2 >= h1) {
+                    r2 -= h1; below += h1; ++bin;
+                    if (r2 >= h2) { below += h2; ++bin; }
+                }
+            }
+            bin = __shfl(bin, src);
+            below = __shfl(below, src);
+            kr -= (int)below;
+            prefix |= (unsigned)bin << shift;
+            pmask |= 255u << shift;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // the counters are read before
+            __builtin_amdgcn_wave_barrier();                           // the next pass clears them
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        float s = 0.0f;
+        for (int c = lane; c < nc; c += 64) {
+            const unsigned db = (unsigned)(keys[c] >> 32);
+            if (db < prefix) s += sph_w(kern, D, __uint_as_float(db), hmean);
+        }
+        s = wave_sum(s);
+        if (lane == 0) out_rho[qi] = s + (float)(kr + 1) * sph_w(kern, D, __uint_as_float(prefix), hmean);
+        return;
+    }
     // Bitonic sort of this wave's nc_pad keys in LDS (ascending).
     for (int k = 2; k <= nc_pad; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -462,7 +524,7 @@ static int query_common(const jt_hashgrid* h, HGDev& g, int64_t m, int32_t cap, 
     int nc = probes * cap;
     nc_pad = 64;
     while (nc_pad < nc) nc_pad <<= 1;
-    size_t per_wave = (size_t)nc_pad * 12;
+    size_t per_wave = (size_t)nc_pad * 12 + 256 * 4;      // keys, candidates, select counters
     waves = (int)(65536 / per_wave);
     waves = waves < 1 ? 1 : (waves > 4 ? 4 : waves);
     JT_REQUIRE(per_wave * waves <= 160 * 1024, "per_cell_cap too large for LDS");
@@ -554,7 +616,7 @@ extern "C" int jt_sph_eval(const jt_hashgrid* h, const float* Q, int64_t m, floa
     hipLaunchKernelGGL(k_sph_direct, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, (hipStream_t)stream, g, Q,
                        (int)m, radius, per_cell_cap, max_neighbors, masses, smoothing, h_mean, kernel, out_rho);
     JT_HIP(hipGetLastError());
-    size_t lds = (size_t)nc_pad * 12 * waves;
+    size_t lds = ((size_t)nc_pad * 12 + 256 * 4) * waves;      // + k_query<1>'s radix-select counters
     hipLaunchKernelGGL(k_query<1>, dim3((unsigned)((m + waves - 1) / waves)), dim3(64 * waves), lds,
                        (hipStream_t)stream, g, Q, (int)m, radius, per_cell_cap, max_neighbors, nc_pad,
                        nullptr, nullptr, masses, smoothing, h_mean, kernel, out_rho, 1);

@@ -462,51 +462,68 @@ extern "C" int jt_hashgrid_build(const jt_hashgrid* h, const float* P, void* ws,
 }
 
 // SPH density with no neighbour list: one wavefront per query sums m W(d, h)
-// over the valid candidates in the same per-lane order and with the same
-// wave reduction as k_query<1>, so where the valid candidates number at most
-// K (every one of them is among the K nearest) the value is k_query<1>'s.
-// Otherwise it writes JT_SPH_REDO for the top-K pass.  No LDS: occupancy is
-// bounded by registers alone.
+// over its valid candidates.  The 27 (2-D: 9) probe cells' capped slot
+// ranges are laid end to end (a wave scan of their counts, kept in LDS) and
+// the lanes take consecutive candidates of that list, so cells of a few
+// points do not leave most lanes idle.  Where the valid candidates number at
+// most K (every one of them is among the K nearest) that sum is the density;
+// otherwise it writes JT_SPH_REDO for the top-K pass.
 __global__ void __launch_bounds__(256)
 k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K,
              const float* __restrict__ masses, const float* __restrict__ hs, float hmean, int kern,
              float* __restrict__ out_rho) {
-    const int lane = threadIdx.x & 63;
-    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (qi >= m) return;                       // whole wave exits together
+    __shared__ int toff[4][32], tst[4][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int qi = blockIdx.x * 4 + wv;
+    if (qi >= m) return;                       // whole wave exits together (no block barrier below)
     const int D = g.dim;
     float q[3] = {0.f, 0.f, 0.f};
     for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
     int cq[3] = {0, 0, 0};
     for (int a = 0; a < D; ++a) cq[a] = cell_coord(q[a], g.origin[a], g.inv);
     const int probes = (D == 3) ? 27 : 9;
+    int st = 0, cnt = 0;
+    if (lane < probes) {                       // lane p: probe p's cell, in the reference's offset order
+        const int p = lane;
+        const int di = (D == 3) ? p / 9 - 1 : p / 3 - 1;
+        const int dj = (D == 3) ? (p / 3) % 3 - 1 : p % 3 - 1;
+        const int dk = (D == 3) ? p % 3 - 1 : 0;
+        const int cell = (D == 3) ? probe_cell(g, cq[0] + di, cq[1] + dj, cq[2] + dk)
+                                  : probe_cell(g, cq[0] + di, cq[1] + dj, 0);
+        if (cell >= 0) { st = g.cstart[cell]; cnt = min(g.cend[cell] - st, cap); }
+    }
+    int incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    if (lane < probes) { toff[wv][lane] = incl - cnt; tst[wv][lane] = st; }
+    if (lane == 0) toff[wv][probes] = total;   // sentinel: the list's end
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     const bool per_point = masses != nullptr || hs != nullptr;
     int nvalid = 0;
     float acc = 0.0f;
-    for (int p = 0; p < probes; ++p) {
-        int di = (D == 3) ? p / 9 - 1 : p / 3 - 1;
-        int dj = (D == 3) ? (p / 3) % 3 - 1 : p % 3 - 1;
-        int dk = (D == 3) ? p % 3 - 1 : 0;
-        int cell = (D == 3) ? probe_cell(g, cq[0] + di, cq[1] + dj, cq[2] + dk)
-                            : probe_cell(g, cq[0] + di, cq[1] + dj, 0);
-        if (cell < 0) continue;
-        const int st = g.cstart[cell], cnt = min(g.cend[cell] - st, cap);
-        for (int s = lane; s < cnt; s += 64) {
-            const float* pp = g.spos + (size_t)(st + s) * D;
-            float dx = pp[0] - q[0], dy = pp[1] - q[1];
-            float d2 = dx * dx + dy * dy;
-            if (D == 3) { float dz = pp[2] - q[2]; d2 = d2 + dz * dz; }
-            const float d = sqrtf(d2);
-            if (d <= r) {
-                ++nvalid;
-                float mass = 1.0f, h = hmean;
-                if (per_point) {
-                    const int pt = g.order[st + s];
-                    if (masses) mass = masses[pt];
-                    if (hs) h = hs[pt];
-                }
-                acc += mass * sph_w(kern, D, d, h);
+    int p = 0;
+    for (int f = lane; f < total; f += 64) {
+        while (f >= toff[wv][p + 1]) ++p;      // stops at the sentinel at the latest
+        const int slot = tst[wv][p] + (f - toff[wv][p]);
+        const float* pp = g.spos + (size_t)slot * D;
+        float dx = pp[0] - q[0], dy = pp[1] - q[1];
+        float d2 = dx * dx + dy * dy;
+        if (D == 3) { float dz = pp[2] - q[2]; d2 = d2 + dz * dz; }
+        const float d = sqrtf(d2);
+        if (d <= r) {
+            ++nvalid;
+            float mass = 1.0f, h = hmean;
+            if (per_point) {
+                const int pt = g.order[slot];
+                if (masses) mass = masses[pt];
+                if (hs) h = hs[pt];
             }
+            acc += mass * sph_w(kern, D, d, h);
         }
     }
     nvalid = wave_sum_i(nvalid);

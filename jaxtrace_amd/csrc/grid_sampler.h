@@ -572,10 +572,10 @@ inline bool periodic_fast(const GridDev& g, int node_bytes) {
 
 // Waves per SIMD the tracking kernels' register budget is sized for (128
 // VGPRs at 4).  The per-lane corner cache of the fast path holds 48 floats
-// across stages; at 4 waves that spills 92 B (float32) / 108 B (16-bit) per
-// lane and still runs faster than the spill-free 3-wave build (9.8 against
-// 10.4 ms and 7.7 against 8.9 ms per 25 steps of 2^24 particles); 5 waves is
-// far slower.
+// across stages; at 4 waves the RK4 kernels spill (a private segment of
+// 112 B float32 / 108 B 16-bit per lane) and still run faster than the
+// spill-free 3-wave build (9.8 against 10.4 ms and 7.7 against 8.9 ms per 25
+// steps of 2^24 particles); 5 waves is far slower.
 #ifndef JT_WAVES_F32
 #define JT_WAVES_F32 4
 #endif

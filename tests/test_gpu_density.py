@@ -76,6 +76,25 @@ def test_sph_truncation_path(gpu):
     np.testing.assert_allclose(est.evaluate(Q), want, rtol=1e-5)
 
 
+def test_sph_truncation_ties(gpu):
+    """The K-th nearest inside a group of equal distances.  Points on a
+    lattice of spacing 2^-9 around lattice-point queries have bit-identical
+    float32 distances per shell (|k|^2 = 9 holds 30 points, and the 100th
+    nearest of the centre query is the 7th of them), so the first-K cut
+    (stable order, neighbors.py:153-161) splits a tie group."""
+    d = F32(2.0 ** -9)
+    k = np.arange(-5, 6, dtype=F32)
+    X, Y, Z = np.meshgrid(k, k, k, indexing="ij")
+    P = (F32(0.5) + d * np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)).astype(F32)
+    Q = (F32(0.5) + d * np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, -1, 2], [2, 2, 2]], F32)).astype(F32)
+    h = float(4 * d)
+    for K in (100, 57, 123):
+        est = jt.SPHDensityEstimator(P, dimensions="3d", smoothing_length=h, key_mode="x64", max_neighbors=K,
+                                     normalize=False)
+        want, _, _ = OD.sph_density(P, Q, h, 64, K, "x64", False)
+        np.testing.assert_allclose(est.evaluate(Q), want, rtol=1e-5)
+
+
 def test_kde_exhaustive_and_cutoff(gpu):
     P = W.gaussian_mixture(1 << 14, seed=7)
     kde = jt.KDEEstimator(P, resolution=24, bounds=((0, 1),) * 3)

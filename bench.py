@@ -79,7 +79,8 @@ def launch_ranges(n_steps, spl, sort_every):
     return out
 
 
-def run_tracking(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events, sort_every=0):
+def run_tracking(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torch, record_events, sort_every=0,
+                 integ=None):
     """One jt_track_grid call over [0, n_steps), as track_particles makes it
     (launches of up to spl steps, a cell sort every sort_every steps).  With
     record_events, returns (call_ms / launch, launch_ms, steps) per launch: the
@@ -99,7 +100,8 @@ def run_tracking(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torc
         o.n_launch_events = len(kev)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-    rc = lib.jt_track_grid(C.byref(desc), C.byref(bc), L.JT_RK4, L.ptr(x), x.shape[0], L.ptr(step_times),
+    rc = lib.jt_track_grid(C.byref(desc), C.byref(bc), L.JT_RK4 if integ is None else integ, L.ptr(x), x.shape[0],
+                           L.ptr(step_times),
                            C.c_float(np.float32(DT)), n_steps, 1, None, C.byref(o), L.ptr(ws), ws.numel(), stream)
     L.check(rc, "jt_track_grid")
     if not record_events:
@@ -110,12 +112,14 @@ def run_tracking(lib, L, desc, bc, x, step_times, n_steps, spl, ws, stream, torc
     return [(per_call, kev[2 * k].elapsed_time(kev[2 * k + 1]), b - a) for k, (a, b) in enumerate(ranges)]
 
 
-def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
+def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0, integ=None, steps=None, warmup=None):
     """Warm-up, then the timed steps, each one jt_track_grid call (launches of
     spl steps: the steps per launch, capped at the sort interval)."""
     dev = field.device
     n = x0.shape[0]
-    total = args.warmup + args.steps
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    total = warmup + steps
     times = (np.arange(total + 1, dtype=np.float64) * DT).astype(np.float32)
     step_times = torch.from_numpy(times).to(dev)
     desc = field.descriptor()
@@ -127,12 +131,12 @@ def measure(field, bc_obj, x0, args, torch, L, lib, sort_every=0):
     stream = L.stream_handle()
     spl = launch_steps(args, sort_every)
     x = x0.clone()
-    run_tracking(lib, L, desc, bc, x, step_times, args.warmup, spl, ws, stream, torch, False, sort_every)
+    run_tracking(lib, L, desc, bc, x, step_times, warmup, spl, ws, stream, torch, False, sort_every, integ)
     torch.cuda.synchronize()
     x = x0.clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    durs = run_tracking(lib, L, desc, bc, x, step_times, args.steps, spl, ws, stream, torch, True, sort_every)
+    durs = run_tracking(lib, L, desc, bc, x, step_times, steps, spl, ws, stream, torch, True, sort_every, integ)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, durs, x
@@ -240,6 +244,59 @@ MFMA_F32_TFLOPS = 157.3                    # dense FP32 MFMA peak of MI355X
 
 def _events(torch):
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_integrators(torch, L, lib, field, bc, x0, args, steps=200, warmup=25):
+    """RK2 and Euler (rk2.py:117-180, euler.py:111-157) through the same fused
+    kernel on the headline workload: FP32 field, 2^24 particles, mixed
+    boundary, launches of `steps_per_launch` steps with the cell sort.  Timed:
+    `steps` steps after `warmup`.  Byte model: 24 B of state + per stage 2
+    snapshots x 8 corners x 12 B (RK2 408 B, Euler 216 B per particle-step)."""
+    n = x0.shape[0]
+    spl = launch_steps(args, args.sort_every)
+    out = {}
+    for name, integ, stages in (("rk2", L.JT_RK2, 2), ("euler", L.JT_EULER, 1)):
+        wall, durs, _ = measure(field, bc, x0, args, torch, L, lib, args.sort_every, integ=integ, steps=steps,
+                                warmup=warmup)
+        full = [r for r in durs if r[2] == spl] or durs
+        launch_ms = float(np.mean([r[1] for r in full]))
+        bpps = 24 + stages * 2 * 8 * 12
+        ach = bpps * n * full[0][2] / (launch_ms * 1e-3) / 1e9
+        out[name] = {"value": n * steps / wall, "unit": "particle-steps/s", "launch_ms": launch_ms,
+                     "steps_per_launch": full[0][2],
+                     "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "bytes_per_particle_step": bpps}}
+    out["workload"] = f"the headline FP32 workload, {steps} steps after {warmup}"
+    return out
+
+
+def bench_nearest(torch, jt, W, L, lib, dev, m=1 << 20):
+    """Nearest-node sampling of TimeSeriesField (time_series.py:249-259,
+    391-412): the 32^3 lattice nodes of the ABC flow (4 snapshots) as the
+    node set, 2^20 uniform queries at t = 0.5.  Timed: one
+    sample_at_positions call (jt_nodes_sample, exhaustive as the reference).
+    Rate: query-node pairs per second, against a VALU bound of 13 lane-ops
+    per pair (the squared distance 8, the first-minimum test 5)."""
+    data, times, g = W.abc_flow()
+    T = data.shape[0]
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    f = jt.TimeSeriesField(data.reshape(T, -1, 3), times, pos, device=dev)
+    q = torch.from_numpy(W.uniform_particles(m, 9)).to(dev)
+    f.sample_at_positions(q, 0.5)
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch)
+    e0.record()
+    f.sample_at_positions(q, 0.5)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    pairs = float(m) * pos.shape[0]
+    rate = pairs / (ms * 1e-3)
+    peak = VALU_LANE_OPS / 13.0
+    return {"value": rate, "unit": "pairs/s", "ms": ms, "queries": m, "nodes": int(pos.shape[0]),
+            "roofline": {"bound": "valu", "achieved": rate, "peak": peak, "unit": "pairs/s", "frac": rate / peak},
+            "workload": "ABC flow 32^3 nodes x 4 snapshots, 2^20 uniform queries, nearest node (exhaustive)"}
 
 
 def bench_octree(torch, jt, W, L, lib, dev, steps=20):
@@ -510,6 +567,12 @@ def main():
                           "bytes_per_particle_step": algorithmic_bytes_per_particle_step(2)}
         del field16
 
+    if not args.no_extras:
+        try:
+            result["rk2_euler"] = bench_integrators(torch, L, lib, field32, bc, x0, args)
+        except Exception as e:  # reported, never fatal
+            result["rk2_euler"] = {"error": str(e)}
+
     # one-step error vs a float64 step on a 4096-particle sample of the final state
     try:
         host = data.cpu().numpy()
@@ -544,7 +607,7 @@ def main():
         result["adjoint"] = {"error": str(e)}
         result["adjoint_fwd_ratio"] = None
     if not args.no_extras:
-        for key, fn in (("octree", bench_octree), ("density", bench_density)):
+        for key, fn in (("octree", bench_octree), ("density", bench_density), ("nearest_node", bench_nearest)):
             try:
                 result[key] = fn(torch, jt, W, L, lib, dev)
             except Exception as e:  # reported, never fatal

@@ -351,9 +351,11 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
     hash grid of cell h with 64 slots per cell, 512 neighbours, x32 keys (the
     reference as shipped); timed: the fused query + kernel-sum launch
-    (jt_sph_eval) and the hash-grid build.  Roofline: 27 x 8 + C_q x 16 + 4 B
-    per query, C_q (candidates read) from jt_hashgrid_query_stats on a
-    sample of queries.
+    (jt_sph_eval); the hash-grid construction (HashGridNeighbors on device
+    points: bounds with their read-back, keys, radix sort, cell table, sorted
+    positions; byte model 36 B per point); and the estimator's whole setup.
+    Roofline: 27 x 8 + C_q x 16 + 4 B per query, C_q (candidates read) from
+    jt_hashgrid_query_stats on a sample of queries.
     KDE: Scott bandwidth, evaluate_3d grid of kde_res^3 points, cut-off 5.3 h
     (each dropped term < 1e-6 of the kernel's peak); timed: the
     jt_kde_grid_cutoff launch.  Rate: pairs within the cut-off per second
@@ -372,14 +374,21 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     del X, Y, Z
     M = int(Qd.shape[0])
     out = {}
+    # the hash grid alone (x32 keys, cell h): bounds, keys, radix sort, cell
+    # table, sorted positions; after one warm-up build
+    jt.HashGridNeighbors(Pd, h, 64, "x32", dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    jt.HashGridNeighbors(Pd, h, 64, "x32", dev)
+    torch.cuda.synchronize()
+    hg_ms = (time.perf_counter() - t0) * 1e3
     e0, e1 = _events(torch)
     torch.cuda.synchronize()
-    e0.record()
+    t0 = time.perf_counter()
     est = jt.SPHDensityEstimator(Pd, dimensions="3d", smoothing_length=h, kernel_type="cubic_spline",
                                  per_cell_cap=64, max_neighbors=512, normalize=True, key_mode="x32")
-    e1.record()
     torch.cuda.synchronize()
-    build_ms = e0.elapsed_time(e1)
+    build_ms = (time.perf_counter() - t0) * 1e3
     rho = torch.empty(M, dtype=torch.float32, device=dev)
     radius = F(2.0 * est._mean_h)
 
@@ -403,7 +412,11 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     bpq = 27 * 8 + cq * 16 + 4
     ach = bpq * M / (sph_ms * 1e-3) / 1e9
     out["sph"] = {"value": M / (sph_ms * 1e-3), "unit": "queries/s", "ms": sph_ms, "queries": M,
-                  "hashgrid_build_ms": build_ms,
+                  "estimator_setup_ms": build_ms,
+                  "hashgrid_build": {"ms": hg_ms, "bytes_per_point": 36,
+                                     "roofline": {"bound": "hbm", "achieved": 36.0 * len(P) / (hg_ms * 1e-3) / 1e9,
+                                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                                  "frac": 36.0 * len(P) / (hg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
                   "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": ach / HBM_PEAK_GBS, "bytes_per_query": bpq, "mean_candidates": cq,
                                "mean_within_radius": float(st[:, 1].double().mean())},

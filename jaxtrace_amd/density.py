@@ -127,18 +127,30 @@ class SPHDensityEstimator:
             raise ValueError(f"Unknown kernel type: {kernel_type}")
         if neighbor_search != "hashgrid":
             raise TypeError("neighbor_search must be 'hashgrid' on the GPU path")
-        P3 = positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else np.asarray(positions)
-        if self.dimensions.lower() == "2d":                                   # :112-117
-            if P3.shape[1] != 3:
-                raise ValueError("2D SPH requires (N,3) positions for slicing")
-            self.P = self._extract_2d(P3)
-            self.D = 2
-        else:
-            if P3.shape[1] != 3:
+        self._P_host = None
+        if (isinstance(positions, torch.Tensor) and positions.device.type == "cuda"
+                and self.dimensions.lower() != "2d"):
+            # device positions stay on the device (the host copy is made on
+            # first use of .P)
+            if positions.ndim != 2 or positions.shape[1] != 3:
                 raise ValueError("3D SPH requires (N,3) positions")
-            self.P = np.asarray(P3, dtype=F32)
+            self._P_dev = positions.detach().to(device=self.device, dtype=torch.float32).contiguous()
             self.D = 3
-        self.N = int(self.P.shape[0])
+            self.N = int(self._P_dev.shape[0])
+        else:
+            P3 = positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else np.asarray(positions)
+            if self.dimensions.lower() == "2d":                               # :112-117
+                if P3.shape[1] != 3:
+                    raise ValueError("2D SPH requires (N,3) positions for slicing")
+                self._P_host = self._extract_2d(P3)
+                self.D = 2
+            else:
+                if P3.shape[1] != 3:
+                    raise ValueError("3D SPH requires (N,3) positions")
+                self._P_host = np.asarray(P3, dtype=F32)
+                self.D = 3
+            self.N = int(self._P_host.shape[0])
+            self._P_dev = None
         if self.N < 2:
             raise ValueError("At least two particles are required")
         if masses is None:
@@ -147,7 +159,8 @@ class SPHDensityEstimator:
             self.masses = np.asarray(masses, dtype=F32)
             if self.masses.shape[0] != self.N:
                 raise ValueError("masses must be length N")
-        self._P_dev = _to_dev(self.P, self.device)
+        if self._P_dev is None:
+            self._P_dev = _to_dev(self._P_host, self.device)
         if smoothing_length is not None:                                      # :137-140
             self.smoothing_lengths = np.full(self.N, F32(float(smoothing_length)), F32)
         else:
@@ -156,9 +169,19 @@ class SPHDensityEstimator:
         self._hash = HashGridNeighbors(self._P_dev, max(self._mean_h, 1e-6), self.per_cell_cap,
                                        key_mode, self.device)                 # :148-152
         self._neighbor_backend = "hashgrid"
-        uniform_h = bool(np.all(self.smoothing_lengths == F32(self._mean_h)))
+        if smoothing_length is not None:                                      # every h_i is that value
+            uniform_h = bool(F32(float(smoothing_length)) == F32(self._mean_h))
+        else:
+            uniform_h = bool(np.all(self.smoothing_lengths == F32(self._mean_h)))
         self._h_dev = None if uniform_h else _to_dev(self.smoothing_lengths, self.device)
         self._m_dev = None if self.masses is None else _to_dev(self.masses, self.device)
+
+    @property
+    def P(self) -> np.ndarray:
+        """The particle positions on the host, (N, D) float32."""
+        if self._P_host is None:
+            self._P_host = self._P_dev.cpu().numpy()
+        return self._P_host
 
     def _extract_2d(self, P3):
         """sph.py:164-177."""

@@ -270,6 +270,38 @@ def bench_integrators(torch, L, lib, field, bc, x0, args, steps=200, warmup=25):
     return out
 
 
+def bench_stream(torch, jt, field, bc_obj, x0, steps=100, every=25):
+    """Streamed trajectory output (TrackerOptions.output_path) on the headline
+    workload: frames recorded every `every` steps go to a .npy file in the
+    temp directory while tracking runs.  Timed beside the same run kept on
+    the device and the in-memory run with its one host copy at the end."""
+    import tempfile
+    path = os.path.join(tempfile.gettempdir(), "jt_stream_bench.npy")
+    span = (0.0, steps * DT)
+
+    def run(**kw):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        tr = jt.create_tracker("rk4", field, bc_obj, record_every=every, **kw)
+        tr = tr.track_particles(x0, span, steps + 1, dt=DT)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, tr
+
+    run(return_device=True)
+    t_dev, _ = run(return_device=True)
+    t_mem, mem = run()
+    t_st, st = run(output_path=path)
+    same = bool(np.array_equal(np.asarray(st.positions), mem.positions))
+    nbytes = int(mem.positions.nbytes)
+    del mem, st
+    os.remove(path)
+    return {"device_only_ms": t_dev * 1e3, "in_memory_ms": t_mem * 1e3, "streamed_ms": t_st * 1e3,
+            "frames": 1 + steps // every, "bytes": nbytes, "streamed_GBs": nbytes / t_st / 1e9,
+            "streamed_equals_in_memory": same,
+            "workload": f"the headline FP32 workload, {x0.shape[0]} particles, {steps} steps, "
+                        f"a frame every {every} steps"}
+
+
 def bench_nearest(torch, jt, W, L, lib, dev, m=1 << 20):
     """Nearest-node sampling of TimeSeriesField (time_series.py:249-259,
     391-412): the 32^3 lattice nodes of the ABC flow (4 snapshots) as the
@@ -585,6 +617,10 @@ def main():
             result["rk2_euler"] = bench_integrators(torch, L, lib, field32, bc, x0, args)
         except Exception as e:  # reported, never fatal
             result["rk2_euler"] = {"error": str(e)}
+        try:
+            result["trajectory_stream"] = bench_stream(torch, jt, field32, bc, x0)
+        except Exception as e:  # reported, never fatal
+            result["trajectory_stream"] = {"error": str(e)}
 
     # one-step error vs a float64 step on a 4096-particle sample of the final state
     try:

@@ -12,6 +12,7 @@ second, host-side code path).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 from dataclasses import dataclass, field
 from typing import Any, Callable, Dict, Optional, Tuple
@@ -41,6 +42,13 @@ class TrackerOptions:
       sort_every         re-sort particles by grid cell every k steps (0 = off;
                          None = from 65536 particles, every 25 steps on
                          structured fields and every 10 on octree fields)
+      output_path        stream the recorded frames into this .npy file,
+                         (frames, N, 3) float32, while tracking runs; the
+                         Trajectory's positions are then a read-only memmap of
+                         it (velocities, when recorded, go to
+                         '<stem>_velocities.npy').  Host memory holds two
+                         chunks, not the whole trajectory.
+      frames_per_chunk   recorded frames per streamed chunk (0 = about 1 GiB)
     """
     max_memory_gb: float = 8.0
     batch_size: Optional[int] = None
@@ -66,6 +74,8 @@ class TrackerOptions:
     return_device: bool = False
     steps_per_launch: int = 0
     sort_every: Optional[int] = None
+    output_path: Optional[str] = None
+    frames_per_chunk: int = 0
 
 
 class Trajectory:
@@ -160,47 +170,159 @@ class ParticleTracker:
         out_times = times[rec_idx] if n_steps > 0 else times[:1]
 
         step_times = torch.from_numpy(times[:-1].copy() if n_steps > 0 else np.zeros(1, F32)).to(dev)
+        meta = {"integrator": self.integrator_name, "batch_processing": "single_batch",
+                "n_particles": n, "n_timesteps": int(n_timesteps), "dt": dt,
+                "record_every": rec, "backend": "hip-gfx950", "field_storage": fld.storage}
+        state = x.clone()
+        t0 = time.perf_counter()
+        if opt.output_path is not None:
+            if opt.return_device:
+                raise ValueError("output_path streams frames to a file; it cannot be combined with return_device")
+            traj, velocities = self._track_streamed(state, n, step_times, dt, n_steps, rec, out_times)
+            meta["output_path"] = str(opt.output_path)
+        else:
+            launch = self._launcher(state, n, dt, n_steps)
+            traj = torch.empty((n_rec, n, 3), dtype=torch.float32, device=dev)
+            launch(step_times, n_steps, rec, traj)
+            velocities = None
+            if opt.record_velocities:
+                velocities = torch.empty_like(traj)
+                for r in range(n_rec):
+                    velocities[r] = fld._sample(traj[r], float(out_times[r]))
+            if not opt.return_device:
+                traj = traj.cpu().numpy()
+                if velocities is not None:
+                    velocities = velocities.cpu().numpy()
+        if opt.progress_callback is not None:
+            opt.progress_callback(1.0)
+        meta["wall_s"] = time.perf_counter() - t0
+        return Trajectory(traj, out_times, velocities, meta)
+
+    def _launcher(self, state, n, dt, max_steps):
+        """The tracking call for this field and boundary, with its workspace
+        sized for up to max_steps steps: launch(step_times, n_steps, rec, traj)
+        advances state by n_steps steps from step_times[0] and writes state
+        before the first step and after every rec-th step into traj."""
+        lib = L.load()
+        fld = self.field
+        dev = fld.device
         desc = fld.descriptor()
         bc = self.boundary.descriptor(dev)
         o = self._opts(n)
         kind = getattr(fld, "kind", "grid")
         if kind == "octree":
-            ws_bytes = int(lib.jt_track_octree_workspace_size(n, n_steps, C.byref(o)))
+            ws_bytes = int(lib.jt_track_octree_workspace_size(n, max_steps, C.byref(o)))
             entry, name = lib.jt_track_octree, "jt_track_octree"
         elif kind == "nodes":
-            ws_bytes = int(lib.jt_track_nodes_workspace_size(n, n_steps))
+            ws_bytes = int(lib.jt_track_nodes_workspace_size(n, max_steps))
             entry, name = lib.jt_track_nodes, "jt_track_nodes"
         else:
-            ws_bytes = int(lib.jt_track_grid_workspace_size(n, n_steps, C.byref(o)))
+            ws_bytes = int(lib.jt_track_grid_workspace_size(n, max_steps, C.byref(o)))
             entry, name = lib.jt_track_grid, "jt_track_grid"
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
-        traj = torch.empty((n_rec, n, 3), dtype=torch.float32, device=dev)
-        state = x.clone()
-        t0 = time.perf_counter()
-        if self.boundary.host_inlet:
-            self._track_host_inlet(entry, name, desc, state, n, step_times, dt, n_steps, rec, traj)
-        else:
-            L.check(entry(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
-                          L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
-                          rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
-                          L.stream_handle()), name)
-        velocities = None
-        if opt.record_velocities:
-            velocities = torch.empty_like(traj)
-            for r in range(n_rec):
-                velocities[r] = fld._sample(traj[r], float(out_times[r]))
-        if opt.progress_callback is not None:
-            opt.progress_callback(1.0)
-        meta = {"integrator": self.integrator_name, "batch_processing": "single_batch",
-                "n_particles": n, "n_timesteps": int(n_timesteps), "dt": dt,
-                "record_every": rec, "backend": "hip-gfx950", "field_storage": fld.storage}
-        if not opt.return_device:
-            traj = traj.cpu().numpy()
-            if velocities is not None:
-                velocities = velocities.cpu().numpy()
-        meta["wall_s"] = time.perf_counter() - t0
-        return Trajectory(traj, out_times, velocities, meta)
 
+        def launch(step_times, n_steps, rec, traj):
+            if self.boundary.host_inlet:
+                self._track_host_inlet(entry, name, desc, state, n, step_times, dt, n_steps, rec, traj)
+            else:
+                L.check(entry(C.byref(desc), C.byref(bc), _INTEGRATORS[self.integrator_name],
+                              L.ptr(state), n, L.ptr(step_times), C.c_float(F32(dt)), n_steps,
+                              rec, L.ptr(traj), C.byref(o), L.ptr(ws), ws_bytes,
+                              L.stream_handle()), name)
+        return launch
+
+    def _track_streamed(self, state, n, step_times, dt, n_steps, rec, out_times):
+        """Chunked tracking with the recorded frames streamed to .npy files.
+
+        Chunk k advances the state by (frames in the chunk) x rec steps into
+        device buffer k % 2 (about 1 GiB each by default) on the current
+        stream.  While chunk k + 1 computes, a second stream (default
+        priority) copies chunk k to the host in 64 MiB pieces through two
+        pinned staging slots, and the host writes each landed piece into the
+        file at its byte offset while the next piece copies.  Pinned memory
+        stays at 128 MiB whatever the particle count.  Frames are the same
+        values as one whole-run call: each particle's arithmetic does not
+        depend on where launches split."""
+        opt = self.options
+        fld = self.field
+        dev = fld.device
+        n_rec = len(out_times)
+        path = str(opt.output_path)
+        vpath = path[:-4] + "_velocities.npy" if path.endswith(".npy") else path + "_velocities.npy"
+        frame_bytes = max(n * 12, 1)
+        fpc = int(opt.frames_per_chunk) if opt.frames_per_chunk > 0 else max(1, (1 << 30) // frame_bytes)
+        fpc = max(1, min(fpc, max(n_rec - 1, 1)))
+        pos_f = _NpyFrames(path, n_rec, n)
+        vel_f = _NpyFrames(vpath, n_rec, n) if opt.record_velocities else None
+        pos_f.write(0, state.cpu().numpy())
+        if vel_f is not None:
+            vel_f.write(0, fld._sample(state, float(out_times[0])).cpu().numpy())
+        launch = self._launcher(state, n, dt, fpc * rec)
+        comp = torch.cuda.current_stream(dev)
+        copy = torch.cuda.Stream(dev)
+        nb = 2 if n_rec - 1 > fpc else 1
+        dbuf = [torch.empty((fpc + 1, n, 3), dtype=torch.float32, device=dev) for _ in range(nb)]
+        vdbuf = None
+        if vel_f is not None:
+            vdbuf = [torch.empty((fpc, n, 3), dtype=torch.float32, device=dev) for _ in range(nb)]
+        # two pinned staging slots: one copies on the copy stream while the
+        # host writes the other
+        piece = _STREAM_PIECE_BYTES // 4
+        slots = [torch.empty(piece, dtype=torch.float32, pin_memory=True) for _ in range(2)]
+        landed = [torch.cuda.Event() for _ in range(2)]
+        computed = [torch.cuda.Event() for _ in range(nb)]
+        drained = [torch.cuda.Event() for _ in range(nb)]
+
+        def drain(b, f0, nf):
+            """Copy chunk b's frames to the host piece by piece and write them."""
+            srcs = [(dbuf[b][1:nf + 1].reshape(-1), pos_f)]
+            if vdbuf is not None:
+                srcs.append((vdbuf[b][:nf].reshape(-1), vel_f))
+            base = (f0 + 1) * n * 3
+            j, prev = 0, None
+            with torch.cuda.stream(copy):
+                copy.wait_event(computed[b])
+                for src, out in srcs:
+                    for lo in range(0, src.numel(), piece):
+                        hi = min(src.numel(), lo + piece)
+                        sl = j % 2
+                        slots[sl][:hi - lo].copy_(src[lo:hi], non_blocking=True)
+                        landed[sl].record(copy)
+                        if prev is not None:
+                            _write_piece(*prev)
+                        prev = (slots[sl], landed[sl], out, base + lo, hi - lo)
+                        j += 1
+                drained[b].record(copy)
+            if prev is not None:
+                _write_piece(*prev)
+
+        pending = None
+        f0, k = 0, 0
+        while f0 < n_rec - 1:
+            nf = min(fpc, n_rec - 1 - f0)
+            b = k % nb
+            if k >= nb:
+                comp.wait_event(drained[b])           # device buffer b is free again
+            launch(step_times[f0 * rec:(f0 + nf) * rec], nf * rec, rec, dbuf[b][:nf + 1])
+            if vdbuf is not None:
+                for i in range(nf):
+                    vdbuf[b][i] = fld._sample(dbuf[b][i + 1], float(out_times[f0 + 1 + i]))
+            computed[b].record(comp)
+            if pending is not None:
+                drain(*pending)                         # the previous chunk, while this one computes
+            pending = (b, f0, nf)
+            f0 += nf
+            k += 1
+        if pending is not None:
+            drain(*pending)
+        torch.cuda.synchronize(dev)
+        pos_f.close()
+        positions = np.load(path, mmap_mode="r")
+        velocities = None
+        if vel_f is not None:
+            vel_f.close()
+            velocities = np.load(vpath, mmap_mode="r")
+        return positions, velocities
 
     def _track_host_inlet(self, entry, name, desc, state, n, step_times, dt, n_steps, rec, traj):
         """A continuous inlet with host-drawn replacements: one step at a time
@@ -241,3 +363,41 @@ def create_tracker(integrator_name: str, field, boundary_condition, **options) -
                             "(periodic/reflective/clamping/absorbing/mixed/continuous_inlet)")
         raise TypeError("boundary_condition must be callable")
     return ParticleTracker(integrator_name, field, boundary_condition, TrackerOptions(**options))
+
+
+class _NpyFrames:
+    """A (frames, n, 3) float32 .npy file written frame range by frame range
+    with pwrite at each range's byte offset (no page faults on a mapping)."""
+
+    def __init__(self, path: str, n_frames: int, n: int):
+        mm = np.lib.format.open_memmap(path, mode="w+", dtype=np.float32, shape=(n_frames, n, 3))
+        self.offset = int(mm.offset)
+        del mm
+        self.frame_bytes = n * 12
+        self.fd = os.open(path, os.O_WRONLY)
+
+    def write(self, first: int, frames: np.ndarray) -> None:
+        self.write_values(first * (self.frame_bytes // 4), frames)
+
+    def write_values(self, first_value: int, values: np.ndarray) -> None:
+        """Write float32 values starting at flat value index first_value."""
+        flat = np.ascontiguousarray(values, dtype=np.float32).reshape(-1).view(np.uint8)
+        if flat.size == 0:
+            return
+        buf = memoryview(flat)
+        pos = self.offset + first_value * 4
+        while len(buf):
+            k = os.pwrite(self.fd, buf, pos)
+            buf, pos = buf[k:], pos + k
+
+    def close(self) -> None:
+        os.close(self.fd)
+
+
+# pinned staging per slot for streamed output (two slots)
+_STREAM_PIECE_BYTES = 64 << 20
+
+
+def _write_piece(slot, landed, out, first_value, count):
+    landed.synchronize()
+    out.write_values(first_value, slot[:count].numpy())

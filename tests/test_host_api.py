@@ -56,3 +56,24 @@ def test_tracker_options_defaults_match_reference():
     o = jt.TrackerOptions()
     assert o.max_batch_size == 100_000 and o.min_batch_size == 100 and o.max_memory_gb == 8.0
     assert o.record_every == 1 and not o.return_device
+
+
+def test_streamed_frame_writer_ranges_and_layout(tmp_path):
+    """The streamed-output writer puts frame ranges at their byte offsets of a
+    standard (frames, n, 3) float32 .npy file, in any order."""
+    from jaxtrace_amd.tracking import _NpyFrames
+    rng = np.random.default_rng(0)
+    want = rng.standard_normal((7, 5, 3)).astype(np.float32)
+    w = _NpyFrames(str(tmp_path / "t.npy"), 7, 5)
+    w.write(3, want[3:7])
+    w.write(0, want[0])
+    w.write(1, want[1:3])
+    w.close()
+    got = np.load(tmp_path / "t.npy")
+    assert got.dtype == np.float32 and got.shape == (7, 5, 3)
+    np.testing.assert_array_equal(got, want)
+    e = _NpyFrames(str(tmp_path / "e.npy"), 4, 0)
+    e.write(0, np.zeros((1, 0, 3), np.float32))
+    e.write(1, np.zeros((3, 0, 3), np.float32))
+    e.close()
+    assert np.load(tmp_path / "e.npy").shape == (4, 0, 3)

@@ -302,6 +302,34 @@ def bench_stream(torch, jt, field, bc_obj, x0, steps=100, every=25):
                         f"a frame every {every} steps"}
 
 
+def bench_seeding(torch, jt, W, L, lib, dev, res=256, reps=20):
+    """uniform_grid_seeds' meshgrid expansion (jt_grid_seeds) for res^3 seeds
+    in the unit box, coordinate arrays resident: 12 B written per seed."""
+    from jaxtrace_amd import seeding as S
+    ax = [torch.from_numpy(a).to(dev) for a in S.grid_axes(res, [[0, 0, 0], [1, 1, 1]])]
+    n = res ** 3
+    out = torch.empty((n, 3), dtype=torch.float32, device=dev)
+
+    def call():
+        L.check(lib.jt_grid_seeds(L.ptr(ax[0]), res, L.ptr(ax[1]), res, L.ptr(ax[2]), res, L.ptr(out),
+                                  L.stream_handle()), "jt_grid_seeds")
+
+    call()
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch)
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = n * 12 / (ms * 1e-3) / 1e9
+    return {"value": n / (ms * 1e-3), "unit": "seeds/s", "ms": ms, "seeds": n,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_seed": 12},
+            "workload": f"uniform_grid_seeds {res}^3 in the unit box, mean of {reps} calls"}
+
+
 def bench_nearest(torch, jt, W, L, lib, dev, m=1 << 20):
     """Nearest-node sampling of TimeSeriesField (time_series.py:249-259,
     391-412): the 32^3 lattice nodes of the ABC flow (4 snapshots) as the
@@ -656,7 +684,8 @@ def main():
         result["adjoint"] = {"error": str(e)}
         result["adjoint_fwd_ratio"] = None
     if not args.no_extras:
-        for key, fn in (("octree", bench_octree), ("density", bench_density), ("nearest_node", bench_nearest)):
+        for key, fn in (("octree", bench_octree), ("density", bench_density), ("nearest_node", bench_nearest),
+                        ("seeding", bench_seeding)):
             try:
                 result[key] = fn(torch, jt, W, L, lib, dev)
             except Exception as e:  # reported, never fatal

@@ -346,6 +346,14 @@ int jt_track_grid_adjoint(const jt_grid_field* field, const jt_boundary* bc, con
                           int32_t ckpt_every, const float* grad_xT, float* grad_x0, float* grad_U,
                           void* ws, size_t ws_bytes, void* stream);
 
+/* Replaces the meshgrid expansion of uniform_grid_seeds
+ * (jaxtrace/tracking/seeding.py:185-247): out (nx*ny*nz, 3) float32 with row
+ * (i*ny + j)*nz + k = (xc[i], yc[j], zc[k]), np.meshgrid(..., indexing='ij')
+ * order.  xc/yc/zc are the coordinate arrays the reference builds with
+ * np.linspace (device pointers, nx/ny/nz floats). */
+int jt_grid_seeds(const float* xc, int32_t nx, const float* yc, int32_t ny, const float* zc, int32_t nz,
+                  float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

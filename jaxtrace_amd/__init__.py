@@ -22,6 +22,7 @@ from .boundary import (
 )
 from .tracking import TrackerOptions, Trajectory, ParticleTracker, create_tracker
 from .adjoint import track_rk4
+from .seeding import uniform_grid_seeds
 from .density import HashGridNeighbors, SPHDensityEstimator, KDEEstimator, scott_bandwidth, silverman_bandwidth
 
 __version__ = "0.1.0"
@@ -33,4 +34,5 @@ __all__ = [
     "apply_periodic", "clamp_to_bounds", "reflect_boundary", "absorbing_boundary",
     "TrackerOptions", "Trajectory", "ParticleTracker", "create_tracker",
     "track_rk4", "HashGridNeighbors", "SPHDensityEstimator", "KDEEstimator", "scott_bandwidth", "silverman_bandwidth",
+    "uniform_grid_seeds",
 ]

@@ -145,13 +145,38 @@ def cubic_spline(r, h, D=3):
     return np.where(q <= 1.0, w1, np.where(q <= 2.0, w2, F32(0.0))).astype(F32)
 
 
+PLANE_AXES = {"xy": (0, 1, 2), "xz": (0, 2, 1), "yz": (1, 2, 0)}
+
+
+def extract_2d(P3, plane="xy", position=0.0, slab_thickness=0.1):
+    """sph.py:164-177 / kde.py:112-124: the points of the slab
+    |p_k - position| <= thickness/2, projected on the plane's two axes (all
+    points projected when the slab is empty)."""
+    P3 = np.asarray(P3)
+    i, j, k = PLANE_AXES[plane]
+    mask = np.abs(P3[:, k] - float(position)) <= (float(slab_thickness) * 0.5)
+    pts2 = P3[mask][:, [i, j]]
+    if pts2.size == 0:
+        pts2 = P3[:, [i, j]]
+    return pts2.astype(F32, copy=False)
+
+
 def sph_density(P, Q, h, per_cell_cap=64, max_neighbors=512, key_mode="x32", normalize=True,
-                masses=None, kernel="cubic_spline"):
-    """SPHDensityEstimator(dimensions='3d', smoothing_length=h,
-    kernel_type='cubic_spline', neighbor_search='hashgrid').evaluate(Q)
-    (sph.py:107-160, 269-279, 333-380)."""
+                masses=None, kernel="cubic_spline", dimensions="3d", plane="xy", position=0.0,
+                slab_thickness=0.1):
+    """SPHDensityEstimator(dimensions, smoothing_length=h, kernel_type,
+    neighbor_search='hashgrid').evaluate(Q) (sph.py:107-160, 269-279,
+    333-380).  dimensions='2d' takes the slab of (N,3) positions
+    (sph.py:112-117) and projects (M,3) queries on the plane (:354-361)."""
     P = np.asarray(P, F32)
     Q = np.asarray(Q, F32)
+    Dm = 3
+    if dimensions.lower() == "2d":
+        P = extract_2d(P, plane, position, slab_thickness)
+        Dm = 2
+        if Q.shape[1] == 3:
+            i, j, _ = PLANE_AXES[plane]
+            Q = Q[:, [i, j]]
     N = P.shape[0]
     m = np.ones(N, F32) if masses is None else np.asarray(masses, F32)
     hs = np.full(N, F32(h), F32)
@@ -161,14 +186,18 @@ def sph_density(P, Q, h, per_cell_cap=64, max_neighbors=512, key_mode="x32", nor
     valid = idx >= 0
     mass = np.where(valid, m[np.maximum(idx, 0)], F32(0))
     hi = np.where(valid, hs[np.maximum(idx, 0)], F32(mean_h))
-    w = SPH_KERNELS[kernel](dist, hi, 3)
+    w = SPH_KERNELS[kernel](dist, hi, Dm)
     rho = (mass * w).sum(axis=1, dtype=F32)
     if normalize:                                                          # :365-378
         xmin, xmax = float(Q[:, 0].min()), float(Q[:, 0].max())
         ymin, ymax = float(Q[:, 1].min()), float(Q[:, 1].max())
-        zmin, zmax = float(Q[:, 2].min()), float(Q[:, 2].max())
-        vol = max((xmax - xmin) * (ymax - ymin) * (zmax - zmin), 1e-12)
-        rho = (rho / F32(N / vol)).astype(F32)
+        if Dm == 2:
+            area = max((xmax - xmin) * (ymax - ymin), 1e-12)
+            rho = (rho / F32(N / area)).astype(F32)
+        else:
+            zmin, zmax = float(Q[:, 2].min()), float(Q[:, 2].max())
+            vol = max((xmax - xmin) * (ymax - ymin) * (zmax - zmin), 1e-12)
+            rho = (rho / F32(N / vol)).astype(F32)
     return rho.astype(F32), idx, dist
 
 
@@ -240,6 +269,34 @@ def kde_normalize_3d(X, Y, Z, D, N):
     domain_vol = float((X.max() - X.min()) * (Y.max() - Y.min()) * (Z.max() - Z.min()))
     avg = N / max(domain_vol, 1e-12)
     return D / max(avg, 1e-12)
+
+
+def make_grid_2d(pts, resolution, bounds=None):
+    """kde.py:173-186 on the 2-D points: f32 linspace axes, meshgrid 'xy' -> (ny, nx)."""
+    pts = np.asarray(pts, F32)
+    if bounds is None:
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+    else:
+        lo = np.array([bounds[0][0], bounds[1][0]], dtype=float)
+        hi = np.array([bounds[0][1], bounds[1][1]], dtype=float)
+    res = resolution if isinstance(resolution, int) else resolution[:2]
+    nx = ny = int(res)
+    x = np.linspace(lo[0], hi[0], nx, dtype=np.float32)
+    y = np.linspace(lo[1], hi[1], ny, dtype=np.float32)
+    return np.meshgrid(x, y, indexing="xy")
+
+
+def kde_normalize_2d(X, Y, Z, N):
+    """kde.py:210-220."""
+    dx = (X.max() - X.min()) / max(X.shape[1] - 1, 1)
+    dy = (Y.max() - Y.min()) / max(Y.shape[0] - 1, 1)
+    cell_area = float(dx * dy)
+    total = float((Z * cell_area).sum())
+    if total > 0:
+        Z = Z * (N / total)
+    domain_area = float((X.max() - X.min()) * (Y.max() - Y.min()))
+    avg = N / max(domain_area, 1e-12)
+    return Z / max(avg, 1e-12)
 
 
 def wendland(r, h, D=3):

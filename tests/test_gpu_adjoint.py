@@ -89,3 +89,110 @@ def test_adjoint_against_fp64_finite_differences(gpu):
         Um[idx] -= 1e-4
         fd = (L(Up, x64) - L(Um, x64)) / 2e-4
         np.testing.assert_allclose(float(U.grad[idx]), fd, rtol=2e-3, atol=1e-7)
+
+
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_adjoint_at_scale_against_fp64_finite_differences(gpu, storage):
+    """The adjoint workload's field (random Fourier 128^3 x 8 snapshots at
+    t_j = j/7, seed 2), 2^16 particles (seed 7), targets (seed 8), RK4
+    dt = 2e-3, periodic box, checkpoints every K = 32 steps, 40 steps from
+    t = 0.1: two checkpoint segments, a snapshot change at t = 1/7 (step 21)
+    inside the first, 128 blocks of particles spread over a grid far larger
+    than one block's LDS box (so the reverse kernel flushes to global
+    atomics).  32 entries of dL/dx0 and 32 of dL/dU are checked against
+    float64 central differences (eps 1e-6 for x0, 1e-4 for U) of the float64
+    map on the stored snapshot values.  Entries are drawn (seed 9) among
+    particles, and nodes of particles, whose float64 stage points all stay
+    1e-3 cells away from cell faces (so the perturbed tracks see the same
+    cells); each difference re-tracks only the particles the entry reaches.
+    Tolerance: 2e-3 relative plus 1e-4 of the largest sampled entry."""
+    dev = torch.device("cuda")
+    n_part, steps, K, dt, t0 = 1 << 16, 40, 32, 2e-3, 0.1
+    data, times, g = W.random_fourier(n=128, n_snap=8, t_end=1.0, seed=2, device=dev)
+    x0 = np.random.default_rng(7).random((n_part, 3)).astype(np.float32)
+    target = np.random.default_rng(8).random((n_part, 3)).astype(np.float32)
+    bc = jt.periodic_boundary([[0, 0, 0], [1, 1, 1]])
+    U = data.clone().requires_grad_(True)
+    X = torch.tensor(x0, device=dev, requires_grad=True)
+    xT = ADJ.track_rk4(U, times, g, g, g, X, (t0, t0 + steps * dt), steps + 1, dt=dt, boundary=bc,
+                       storage=storage, checkpoint_every=K)
+    ((xT - torch.tensor(target, device=dev)) ** 2).sum(1).mean().backward()
+    gx = X.grad.cpu().numpy()
+    gU = U.grad
+    Us = data.to(torch.bfloat16).to(torch.float32) if storage == "bfloat16" else data
+    U64 = Us.double().cpu().numpy()
+    del U, data, Us
+    meta = OF.GridMeta(g, g, g)
+    tg = np.linspace(t0, t0 + steps * dt, steps + 1, dtype=np.float32)
+    st, dt32 = tg[:-1], float(tg[1] - tg[0])
+    kinds = ["periodic"] * 3
+    x64 = x0.astype(np.float64)
+    t64 = target.astype(np.float64)
+    _, rec = A.track_np(U64, times, meta, x64, st, dt32, kinds, record=True)
+    assert {r[1] for r in rec} == {(0, 1), (1, 2)}
+    margin = np.min(np.stack([r[3] for r in rec]), axis=0)
+    ok = np.nonzero(margin >= 1e-3)[0]
+    assert ok.size > 1000
+
+    def loss_of(Uv, parts, xs):
+        xT = A.track_np(Uv, times, meta, xs, st, dt32, kinds)
+        return float(((xT - t64[parts]) ** 2).sum() / n_part)
+
+    rng = np.random.default_rng(9)
+    got, fd = [], []
+    for i in rng.choice(ok, 32, replace=False):
+        a = int(rng.integers(0, 3))
+        xp, xm = x64[[i]].copy(), x64[[i]].copy()
+        xp[0, a] += 1e-6
+        xm[0, a] -= 1e-6
+        got.append(float(gx[i, a]))
+        fd.append((loss_of(U64, [i], xp) - loss_of(U64, [i], xm)) / 2e-6)
+    got, fd = np.array(got), np.array(fd)
+    np.testing.assert_allclose(got, fd, rtol=2e-3, atol=1e-4 * np.abs(fd).max())
+
+    # nodes of the eligible particles' stage cells; the particles an entry
+    # reaches are those with a stage point in one of the node's 8 cells
+    # whose blend pair weights that snapshot: a sorted (cell, particle) key
+    # list per snapshot answers that
+    n = 128
+    cells = np.stack([r[0] for r in rec])                 # (stages, N, 3)
+    pairs = [r[1] for r in rec]
+    wts = [r[2] for r in rec]
+    lin = (cells[..., 0] * n + cells[..., 1]) * n + cells[..., 2]
+    keys = {}
+    for snap in {q for pr in pairs for q in pr}:
+        use = [s for s in range(len(rec)) if snap in pairs[s] and wts[s][pairs[s].index(snap)] != 0.0]
+        keys[snap] = np.unique((lin[use] * n_part + np.arange(n_part)[None, :]).ravel())
+    corner = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)])
+    got, fd = [], []
+    tries = 0
+    while len(got) < 32:
+        tries += 1
+        assert tries < 5000
+        i = int(rng.choice(ok))
+        s_i = int(rng.integers(0, len(rec)))
+        slot = int(rng.integers(0, 2))
+        snap, wsnap = pairs[s_i][slot], wts[s_i][slot]
+        if wsnap == 0.0:
+            continue
+        node = (cells[s_i, i] + rng.integers(0, 2, 3)) % n
+        adj = (node[None, :] - corner) % n
+        cl = (adj[:, 0] * n + adj[:, 1]) * n + adj[:, 2]
+        kk = keys[snap]
+        lo_, hi_ = np.searchsorted(kk, cl * n_part), np.searchsorted(kk, (cl + 1) * n_part)
+        parts = np.unique(np.concatenate([kk[l:h] % n_part for l, h in zip(lo_, hi_)]))
+        assert i in parts
+        if (margin[parts] < 1e-3).any():
+            continue
+        c = int(rng.integers(0, 3))
+        idx = (snap, int(node[0]), int(node[1]), int(node[2]), c)
+        u = U64[idx]
+        U64[idx] = u + 1e-4
+        lp = loss_of(U64, parts, x64[parts])
+        U64[idx] = u - 1e-4
+        lm = loss_of(U64, parts, x64[parts])
+        U64[idx] = u
+        got.append(float(gU[idx]))
+        fd.append((lp - lm) / 2e-4)
+    got, fd = np.array(got), np.array(fd)
+    np.testing.assert_allclose(got, fd, rtol=2e-3, atol=1e-4 * np.abs(fd).max())

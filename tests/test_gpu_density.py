@@ -149,3 +149,88 @@ def test_full_size_sph_sampled_queries(gpu):
         np.testing.assert_array_equal(gi, oi)
         np.testing.assert_array_equal(gd, od)
         del g
+
+
+# ---- 2-D mode: the reference's default SPH dimensionality (sph.py:90, 112-117),
+# 2-D hash keys (neighbors.py:78-81) and KDEEstimator.evaluate_2d (kde.py:205-221)
+
+@pytest.mark.parametrize("key_mode", ["x32", "x64"])
+def test_hashgrid_2d_build_and_queries_bitwise(gpu, key_mode):
+    P = np.ascontiguousarray(W.gaussian_mixture(30_000, seed=21)[:, :2])
+    h = 1.0 / 64
+    g = jt.HashGridNeighbors(P, h, 16, key_mode)
+    o = OD.HashGrid(P, h, 16, key_mode)
+    np.testing.assert_array_equal(g.pt_indices_sorted.cpu().numpy(), o.order)
+    rng = np.random.default_rng(22)
+    Q = np.concatenate([rng.uniform(-0.05, 1.05, (200, 2)), P[rng.choice(len(P), 200)]]).astype(F32)
+    for K in (64, 300):
+        gi, gd = g.query_many(Q, 2 * h, K)
+        oi, od = o.query_many(Q, 2 * h, K)
+        np.testing.assert_array_equal(gd, od)
+        np.testing.assert_array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("key_mode", ["x32", "x64"])
+@pytest.mark.parametrize("kernel", ["cubic_spline", "wendland", "gaussian"])
+def test_sph_density_2d(gpu, key_mode, kernel):
+    """dimensions='2d' on (N,3) positions: the slab |y - 0.5| <= 0.05 in the
+    'xz' plane, queries given with three columns (projected), area
+    normalisation.  In x32 mode the 2-D key keeps only the second axis, so
+    cells are whole columns and the 64-point cap and the 512-neighbour cut
+    both bite."""
+    P3 = W.gaussian_mixture(200_000, seed=23)
+    h = 2.0 / 256
+    rng = np.random.default_rng(24)
+    Q = np.concatenate([rng.uniform(0, 1, (300, 3)), P3[rng.choice(len(P3), 300)]]).astype(F32)
+    kw = dict(plane="xz", position=0.5, slab_thickness=0.1)
+    est = jt.SPHDensityEstimator(P3, dimensions="2d", smoothing_length=h, kernel_type=kernel, key_mode=key_mode,
+                                 **kw)
+    got = est.evaluate(Q)
+    want, _, _ = OD.sph_density(P3, Q, h, 64, 512, key_mode, True, kernel=kernel, dimensions="2d", **kw)
+    floor = 1e-3 * np.abs(want).max()
+    m = np.abs(want) >= floor
+    assert m.sum() > 100                             # Wendland's support is h: fewer queries reach the floor
+    np.testing.assert_allclose(got[m], want[m], rtol=1e-5)
+    np.testing.assert_allclose(got[~m], want[~m], rtol=1e-5, atol=1e-5 * floor)
+
+
+def test_sph_density_2d_reference_defaults(gpu):
+    """The reference's default estimator (dimensions='2d', plane 'xy',
+    position 0, slab 0.1, cubic spline), given a smoothing length, on
+    (M,2) queries."""
+    P3 = W.gaussian_mixture(100_000, seed=25)
+    P3[:, 2] *= 0.1                                  # put points in the default slab |z| <= 0.05
+    h = 1.0 / 128
+    Q = np.random.default_rng(26).uniform(0, 1, (500, 2)).astype(F32)
+    got = jt.SPHDensityEstimator(P3, smoothing_length=h).evaluate(Q)
+    want, _, _ = OD.sph_density(P3, Q, h, dimensions="2d")
+    m = np.abs(want) >= 1e-3 * np.abs(want).max()
+    np.testing.assert_allclose(got[m], want[m], rtol=1e-5)
+
+
+def test_kde_evaluate_2d_and_projections(gpu):
+    """KDEEstimator.evaluate_2d on 2-D points and on the 'yz' slab of 3-D
+    points (kde.py:112-124, 173-186, 205-221), and evaluate() with 2-D
+    queries on a 3-D estimator (kde.py:160-166), against the float64
+    exhaustive sum and the oracle's normalisation (pinned by its integral)."""
+    P3 = W.gaussian_mixture(1 << 14, seed=27)
+    P2 = np.ascontiguousarray(P3[:, :2])
+    for pts_in, kw in ((P2, {}), (P3, dict(plane="yz", slab_position=0.5, slab_thickness=0.2))):
+        kde = jt.KDEEstimator(pts_in, resolution=32, **kw)
+        np.testing.assert_allclose(kde.h, OD.scott_bandwidth(pts_in), rtol=0)
+        X, Y, Z = kde.evaluate_2d()
+        pts = pts_in if pts_in.shape[1] == 2 else OD.extract_2d(pts_in, kw["plane"], kw["slab_position"],
+                                                                  kw["slab_thickness"])
+        Xo, Yo = OD.make_grid_2d(pts, 32)
+        np.testing.assert_array_equal(X, Xo)
+        np.testing.assert_array_equal(Y, Yo)
+        G = np.stack([Xo.ravel(), Yo.ravel()], 1)
+        dens = OD.kde_eval(G, pts, kde.h, dtype=np.float64).reshape(Xo.shape)
+        want = OD.kde_normalize_2d(Xo, Yo, dens, len(pts_in))
+        m = want >= 1e-3 * want.max()
+        np.testing.assert_allclose(Z[m], want[m], rtol=2e-5)
+    kde3 = jt.KDEEstimator(P3, plane="xy", slab_position=0.4, slab_thickness=0.1)
+    Q2 = np.random.default_rng(28).uniform(0, 1, (400, 2)).astype(F32)
+    want = OD.kde_eval(Q2, OD.extract_2d(P3, "xy", 0.4, 0.1), kde3.h, dtype=np.float64)
+    got = kde3.evaluate(Q2)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5 * want.max())

@@ -120,8 +120,8 @@ def _cfg1(n=10_000):
     return data, times, g, x0
 
 
-@pytest.mark.parametrize("storage,layout", [("float32", "packed"), ("bfloat16", "packed"),
-                                            ("float32", "halo"), ("bfloat16", "halo")])
+@pytest.mark.parametrize("storage,layout", [("float32", "packed"), ("bfloat16", "packed"), ("float16", "packed"),
+                                            ("float32", "halo"), ("bfloat16", "halo"), ("float16", "halo")])
 def test_cfg1_rk4_periodic_per_step_parity(gpu, storage, layout):
     """Config: ABC flow 32^3 x 4 snapshots, 10,000 particles (seed 42), RK4,
     dt = 0.01, 100 steps, periodic unit box."""

@@ -50,3 +50,25 @@ def test_fd_pins_autograd_restatement():
         Um[idx] -= e
         fd = (_loss(Up, times, meta, x0, target, st, dt, kinds) - _loss(Um, times, meta, x0, target, st, dt, kinds)) / (2 * e)
         np.testing.assert_allclose(float(Ur.grad[idx]), float(fd), rtol=1e-4, atol=1e-10)
+
+
+def test_corner_tracker_equals_autograd_restatement():
+    """track_np (float64 NumPy, corners only) is the same map as the torch
+    restatement: equal to rounding over a run that crosses a snapshot change
+    (t = 1/3) with mixed walls, and its recorded stage cells are the floor of
+    the stage points."""
+    data, times, g = W.random_fourier(n=10, n_snap=4, t_end=1.0, n_modes=8, kmax=3, seed=4)
+    meta = OF.GridMeta(g, g, g)
+    rng = np.random.default_rng(6)
+    x0 = rng.uniform(0, 1, (200, 3))
+    kinds = ["periodic", "reflective", "periodic"]
+    tg = np.linspace(0.3, 0.38, 41, dtype=np.float32)
+    dt = float(tg[1] - tg[0])
+    want = A.track(data.double(), times, meta, torch.tensor(x0), tg[:-1], dt, kinds).numpy()
+    got, rec = A.track_np(data.double().numpy(), times, meta, x0, tg[:-1], dt, kinds, record=True)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+    assert len(rec) == 4 * 40
+    assert {r[1] for r in rec} == {(0, 1), (1, 2)}
+    cells0, _, _, margin0 = rec[0]
+    np.testing.assert_array_equal(cells0, np.floor(x0 * 10).astype(np.int64) % 10)
+    assert (margin0 >= 0).all() and (margin0 <= 0.5).all()

@@ -17,6 +17,7 @@ import argparse
 import ctypes as C
 import hashlib
 import json
+import math
 import os
 import sys
 import time
@@ -31,6 +32,8 @@ N_PARTICLES = 1 << 24
 GRID_N = 256
 N_SNAP = 16
 DT = 2e-3
+# KDE cut-off in bandwidths: the one the sampled cut-off parity test runs at
+KDE_CUTOFF_SIGMAS = 7.0
 
 
 def kernel_source_sha(files) -> str:
@@ -164,44 +167,59 @@ def cpu_baseline(host_data, times, grid, x0_host, n_part, n_steps):
     return n_part * n_steps / el, el
 
 
-def step_error_vs_fp64(host_data_f32, host_data_rounded, times, grid, states, t):
-    """Relative one-step position error of a float32 step on ``host_data_rounded``
-    against a float64 step on the float32 field, from identical states."""
+def trajectory_error_vs_fp64(torch, jt, data, times, grid, x0, bc, n_steps=1000, every=10, sample=1024,
+                             report=(10, 20, 50, 100, 200, 500, 1000)):
+    """Trajectory error of the GPU's own output on the headline workload over
+    the full run: all particles tracked on the GPU for n_steps RK4 steps from
+    t = 0 with float32 and bfloat16 snapshot storage (a frame every `every`
+    steps).  For `sample` particles (seed 11), |x_gpu - x_fp64| per frame in
+    units of the domain size (1), against float64 tracks (float64 arithmetic
+    on the stored values, oracle.adjoint_ref.track_np) of the same particles
+    through (a) the float32 field and (b) the field as stored in bfloat16.
+    (a) is storage rounding and float32 arithmetic together; (b) isolates the
+    arithmetic.  The field is chaotic, so any difference grows exponentially:
+    the line reports the max and median at the `report` steps and the last
+    frame at which the max over the sample is still within 1e-4."""
+    from oracle import adjoint_ref as A
     from oracle import fields as OF
-    from oracle import integrators as OI
     meta = OF.GridMeta(grid, grid, grid)
-    f32 = lambda p, tt: OF.structured_series_sample_corners(host_data_rounded, times, meta, p, tt, "periodic")
-    d64 = host_data_f32.astype(np.float64) if host_data_f32.size < (1 << 26) else None
-
-    def f64(p, tt):
-        # float64 twin: blend + trilinear evaluated in float64 on the stored values
-        from oracle.fields import blend_params
-        pb = blend_params(times, np.float32(tt))
-        V0 = host_data_f32[pb[1]]
-        V1 = host_data_f32[pb[2]]
-        a = float(pb[3])
-        n = grid.shape[0]
-        f = np.mod(p * n, n)
-        i0 = np.floor(f).astype(np.int64) % n
-        w = f - np.floor(f)
-        i1 = (i0 + 1) % n
-        out = np.zeros_like(p)
-        for dx in (0, 1):
-            for dy in (0, 1):
-                for dz in (0, 1):
-                    ix = i1[:, 0] if dx else i0[:, 0]
-                    iy = i1[:, 1] if dy else i0[:, 1]
-                    iz = i1[:, 2] if dz else i0[:, 2]
-                    wt = ((w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1])
-                          * (w[:, 2] if dz else 1 - w[:, 2]))
-                    v = (1 - a) * V0[ix, iy, iz].astype(np.float64) + a * V1[ix, iy, iz].astype(np.float64)
-                    out += wt[:, None] * v
-        return out
-    x32 = OI.rk4_step(states, t, DT, f32)
-    x64 = OI.rk4_step(states.astype(np.float64), float(np.float32(t)), float(np.float32(DT)), f64,
-                      dtype=np.float64)
-    # error relative to the domain size (1): positions near 0 make |x|-relative errors meaningless
-    return float(np.abs(x32 - x64).max())
+    kinds = ["periodic", "reflective", "reflective"]
+    n = x0.shape[0]
+    idx = np.sort(np.random.default_rng(11).choice(n, sample, replace=False))
+    tidx = torch.from_numpy(idx).to(x0.device)
+    tg = np.linspace(0.0, n_steps * DT, n_steps + 1, dtype=np.float32)
+    gpu = {}
+    for storage in ("float32", "bfloat16"):
+        f = jt.StructuredTimeSeriesField(data, times, grid, grid, grid, bounds_handling="periodic",
+                                         storage=storage, layout="halo")
+        tr = jt.create_tracker("rk4", f, bc, record_every=every, return_device=True)
+        traj = tr.track_particles(x0, (0.0, n_steps * DT), n_steps + 1, dt=DT)
+        gpu[storage] = traj.positions[:, tidx].cpu().numpy().astype(np.float64)
+        del traj, tr, f
+        torch.cuda.empty_cache()
+    host32 = data.cpu().numpy()
+    hostbf = data.to(torch.bfloat16).to(torch.float32).cpu().numpy()
+    dt = float(np.float32(DT))
+    steps = np.arange(0, n_steps + 1, every)
+    out = {"report_steps": list(report)}
+    for ref_name, host in (("fp64_on_f32_field", host32), ("fp64_on_bf16_stored_field", hostbf)):
+        x = x0[tidx].cpu().numpy().astype(np.float64)
+        ref = [x]
+        for k in range(0, n_steps, every):
+            x = A.track_np(host, times, meta, x, tg[k:k + every], dt, kinds)
+            ref.append(x)
+        ref = np.stack(ref)
+        for storage in ("float32", "bfloat16"):
+            e = np.abs(gpu[storage] - ref).max(axis=2)          # (frames, sample)
+            emax, emed = e.max(axis=1), np.median(e, axis=1)
+            first_out = steps[emax > 1e-4]
+            out[f"{storage}_vs_{ref_name}"] = {
+                "max": [float(emax[r // every]) for r in report],
+                "median": [float(emed[r // every]) for r in report],
+                "max_within_1e-4_up_to_step": int(first_out[0] - every) if first_out.size else int(n_steps)}
+    out["sample"] = (f"{sample} particles of the headline workload, {n_steps} RK4 steps (t = 0 .. "
+                     f"{n_steps * DT:g}), |x_gpu - x_fp64| per particle (max over axes), domain size 1")
+    return out
 
 
 def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32):
@@ -414,10 +432,16 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     (jt_sph_eval); the hash-grid construction (HashGridNeighbors on device
     points: bounds with their read-back, keys, radix sort, cell table, sorted
     positions; byte model 36 B per point); and the estimator's whole setup.
-    Roofline: 27 x 8 + C_q x 16 + 4 B per query, C_q (candidates read) from
-    jt_hashgrid_query_stats on a sample of queries.
-    KDE: Scott bandwidth, evaluate_3d grid of kde_res^3 points, cut-off 5.3 h
-    (each dropped term < 1e-6 of the kernel's peak); timed: the
+    Roofline: 9 x 8 + C_q/3 x 16 + 4 B per query: C_q is the reference's
+    candidate count (27 probes, from jt_hashgrid_query_stats on a sample of
+    queries), of which x32 keys make 9 distinct cells, each read once; queries
+    over 512 neighbours are redone by the top-K pass and are not counted.
+    KDE: Scott bandwidth, evaluate_3d grid of kde_res^3 points, cut-off
+    KDE_CUTOFF_SIGMAS h, the cut-off the sampled GPU parity test
+    (tests/test_gpu_density.py::test_kde_cutoff_tiles_sampled) holds to 1e-5
+    of the float64 exhaustive sum; every dropped term is below
+    (2 pi)^(-3/2) exp(-c^2/2), so their mean is too, and the line reports
+    that bound relative to the density floor (1e-3 of the maximum); timed: the
     jt_kde_grid_cutoff launch.  Rate: pairs within the cut-off per second
     (counted by jt_hashgrid_query_stats on a sample).  Roofline: the FP32
     MFMA peak against 2 flops per (grid point, particle) pair of each 16^3
@@ -469,7 +493,9 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     L.check(lib.jt_hashgrid_query_stats(C.byref(est._hash._desc), L.ptr(Qs), sample, C.c_float(radius), 64,
                                         L.ptr(st), L.stream_handle()), "jt_hashgrid_query_stats")
     cq = float(st[:, 0].double().mean())
-    bpq = 27 * 8 + cq * 16 + 4
+    # x32 keys: the direct pass reads the 9 distinct probe cells of the
+    # 27-probe stencil once (the other 18 repeat them)
+    bpq = 9 * 8 + cq / 3 * 16 + 4
     ach = bpq * M / (sph_ms * 1e-3) / 1e9
     out["sph"] = {"value": M / (sph_ms * 1e-3), "unit": "queries/s", "ms": sph_ms, "queries": M,
                   "estimator_setup_ms": build_ms,
@@ -483,9 +509,9 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
                   "workload": "2^24 Gaussian-mixture particles, 256^3 query grid, cubic spline h=2/256, x32 keys"}
     del est, rho
     kde = jt.KDEEstimator(Pd, bandwidth_rule="scott", normalize=True, resolution=kde_res,
-                          bounds=((0, 1), (0, 1), (0, 1)), cutoff_sigmas=5.3)
+                          bounds=((0, 1), (0, 1), (0, 1)), cutoff_sigmas=KDE_CUTOFF_SIGMAS)
     x, y, z = kde._axes_3d()
-    R = F(5.3 * kde.h)
+    R = F(KDE_CUTOFF_SIGMAS * kde.h)
     hg = jt.HashGridNeighbors(Pd, float(R) / 2, 1, "x64", dev)
     gx, gy, gz = (torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (x, y, z))
     D = torch.empty((len(y), len(x), len(z)), dtype=torch.float32, device=dev)
@@ -501,6 +527,8 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     e1.record()
     torch.cuda.synchronize()
     kde_ms = e0.elapsed_time(e1)
+    d_max = float(D.max())
+    tail = (2 * math.pi) ** -1.5 * math.exp(-0.5 * KDE_CUTOFF_SIGMAS ** 2) / (1e-3 * d_max)
     Xk, Yk, Zk = np.meshgrid(x, y, z, indexing="xy")
     Qk = np.stack([Xk.ravel(), Yk.ravel(), Zk.ravel()], 1).astype(F)
     mk = len(Qk)
@@ -526,11 +554,14 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     tile_pairs = float(wsum.sum())
     ach = 2.0 * tile_pairs / (kde_ms * 1e-3) / 1e12
     out["kde"] = {"value": rate, "unit": "pairs/s", "ms": kde_ms, "grid": [kde_res] * 3, "pairs_within_cutoff": pairs,
-                  "bandwidth": float(kde.h), "cutoff": float(R),
+                  "bandwidth": float(kde.h), "cutoff": float(R), "cutoff_sigmas": KDE_CUTOFF_SIGMAS,
+                  "cutoff_parity_test": "tests/test_gpu_density.py::test_kde_cutoff_tiles_sampled",
+                  "dropped_tail_bound_rel_at_floor": tail,
                   "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_F32_TFLOPS, "unit": "TFLOP/s",
                                "frac": ach / MFMA_F32_TFLOPS, "tile_pairs": tile_pairs,
                                "flops_per_tile_pair": 2},
-                  "workload": f"2^24 Gaussian-mixture particles, {kde_res}^3 grid, Scott bandwidth, cut-off 5.3 h"}
+                  "workload": f"2^24 Gaussian-mixture particles, {kde_res}^3 grid, Scott bandwidth, "
+                              f"cut-off {KDE_CUTOFF_SIGMAS} h"}
     return out
 
 
@@ -650,19 +681,13 @@ def main():
         except Exception as e:  # reported, never fatal
             result["trajectory_stream"] = {"error": str(e)}
 
-    # one-step error vs a float64 step on a 4096-particle sample of the final state
+    # trajectory error of the GPU output against float64 over the whole run
     try:
-        host = data.cpu().numpy()
-        hb = data.to(torch.bfloat16).to(torch.float32).cpu().numpy()
-        idx = np.random.default_rng(11).choice(n, min(4096, n), replace=False)
-        st = xf32[idx].cpu().numpy()
-        result["step_rel_err_vs_fp64"] = {
-            "f32": step_error_vs_fp64(host, host, times, grid, st, np.float32(0.5)),
-            "bf16": step_error_vs_fp64(host, hb, times, grid, st, np.float32(0.5)),
-            "sample": "max |x_step - x_step_fp64| / domain size over 4096 particles, one RK4 step "
-                      "at t=0.5 from identical states"}
+        if args.no_extras:
+            raise RuntimeError("skipped (--no-extras)")
+        result["trajectory_err_vs_fp64"] = trajectory_error_vs_fp64(torch, jt, data, times, grid, x0, bc)
     except Exception as e:  # reported, never fatal
-        result["step_rel_err_vs_fp64"] = {"error": str(e)}
+        result["trajectory_err_vs_fp64"] = {"error": str(e)}
 
     if not args.no_cpu_baseline:
         host = data.cpu().numpy()

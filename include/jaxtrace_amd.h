@@ -340,9 +340,14 @@ size_t jt_track_grid_adjoint_workspace_size(int64_t n, int32_t n_steps, int32_t 
  * record_every = K), (n_steps/K + 1, n, 3).  Given grad_xT = dL/dx_T, writes
  * grad_x0 = dL/dx_0 and ADDS dL/dU into grad_U, a float32
  * (n_snap, nx, ny, nz, 3) array the caller zeroes.  Atomic accumulation:
- * grad_U is reproducible to rounding, not bitwise. */
+ * grad_U is reproducible to rounding, not bitwise.
+ * host_step_times (n_steps) and host_field_times (n_snap) are HOST copies of
+ * step_times and field->times with the same values: the library cuts its
+ * reverse sweep where the snapshot pair changes from them, so the call never
+ * reads the device and stays asynchronous on `stream`. */
 int jt_track_grid_adjoint(const jt_grid_field* field, const jt_boundary* bc, const float* ckpt,
-                          int64_t n, const float* step_times, float dt, int32_t n_steps,
+                          int64_t n, const float* step_times, const float* host_step_times,
+                          const float* host_field_times, float dt, int32_t n_steps,
                           int32_t ckpt_every, const float* grad_xT, float* grad_x0, float* grad_U,
                           void* ws, size_t ws_bytes, void* stream);
 

@@ -144,8 +144,8 @@ _SIGS = {
                                  C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
     "jt_track_grid_adjoint_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.c_int32]),
     "jt_track_grid_adjoint": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_void_p, C.c_int64,
-                                        C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
-                                        C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_int32,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
 }
 
 

@@ -72,8 +72,11 @@ class _GridTrackRK4(torch.autograd.Function):
         gU = torch.zeros(ctx.U_shape, dtype=torch.float32, device=dev)
         ws_bytes = int(lib.jt_track_grid_adjoint_workspace_size(n, spec.n_steps, spec.K))
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        hst = np.ascontiguousarray(spec.step_times_host, dtype=F32)
+        hft = np.ascontiguousarray(field.times, dtype=F32)
         L.check(lib.jt_track_grid_adjoint(C.byref(field.descriptor()), C.byref(spec.bc.descriptor(dev)), L.ptr(ckpt),
-                                          n, L.ptr(spec.step_times), C.c_float(spec.dt), spec.n_steps, spec.K,
+                                          n, L.ptr(spec.step_times), hst.ctypes.data, hft.ctypes.data,
+                                          C.c_float(spec.dt), spec.n_steps, spec.K,
                                           L.ptr(g), L.ptr(gx0), L.ptr(gU), L.ptr(ws), ws_bytes, L.stream_handle()),
                 "jt_track_grid_adjoint")
         return gx0, gU, None
@@ -97,7 +100,8 @@ def track_rk4(U: torch.Tensor, times, grid_x, grid_y, grid_z, x0: torch.Tensor, 
     tg = np.linspace(float(t0), float(t1), int(n_timesteps), dtype=np.float32)
     spec.dt = F32(float(tg[1] - tg[0]) if (dt is None and n_timesteps > 1) else (0.0 if dt is None else dt))
     spec.n_steps = max(int(n_timesteps) - 1, 0)
-    spec.step_times = torch.from_numpy(tg[:-1].copy() if spec.n_steps else np.zeros(1, F32)).to(x0.device)
+    spec.step_times_host = tg[:-1].copy() if spec.n_steps else np.zeros(1, F32)
+    spec.step_times = torch.from_numpy(spec.step_times_host).to(x0.device)
     spec.times = np.asarray(times, F32)
     spec.gx, spec.gy, spec.gz = grid_x, grid_y, grid_z
     spec.bounds = bounds_handling

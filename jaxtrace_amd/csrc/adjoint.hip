@@ -70,9 +70,6 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
 #ifndef ADJ_MINW
 #define ADJ_MINW 1
 #endif
-#ifndef JT_ADJ_REUSE
-#define JT_ADJ_REUSE 0             // the corner cache costs the reverse kernel its second wave per SIMD
-#endif
 
 __device__ __forceinline__ int wave_min_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -109,13 +106,12 @@ struct Stencil {
 // trilinear weights' derivatives are +-1/d per axis (0 where a non-periodic
 // axis is clipped).  depends = false where u is the constant 0 of the 'zero'
 // bounds mode.
-// PF: the fast path's gather through the per-lane corner cache cc.
+// PF: the periodic fast path's index math.
 template <int ST, bool ZH, bool PF = false>
 __device__ __forceinline__ void sample_jac(const GridDev& g, const Blend& bl, const float q[3], float u[3],
-                                           float J[9], bool& depends, CornerCache<ST>& cc) {
+                                           float J[9], bool& depends) {
     Corners C;
-    if constexpr (PF && JT_ADJ_REUSE) gather_cached<ST>(g, bl, q[0], q[1], q[2], C, cc);
-    else gather_corners<ST, ZH, PF>(g, bl, q[0], q[1], q[2], C);
+    gather_corners<ST, ZH, PF>(g, bl, q[0], q[1], q[2], C);
     lerp_corners(g, C, u);
     depends = !(!C.valid && g.bmode == JT_BOUNDS_ZERO);
     float gd[3];
@@ -382,8 +378,6 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     if (live) { lam[0] = lam_io[3 * (size_t)pid]; lam[1] = lam_io[3 * (size_t)pid + 1]; lam[2] = lam_io[3 * (size_t)pid + 2]; }
     RunAcc run;
     run.live = false;
-    CornerCache<ST> cc;
-    cache_reset(cc);
     if (live) {
         for (int s = s_hi - 1; s >= s_lo; --s) {
             const float* xi = seg + ((size_t)(s - sa) * n + slot) * 3;
@@ -401,7 +395,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
                 const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
                 float k[3], Jn[9];
                 bool dn;
-                sample_jac<ST, ZH, PF>(g, b, q, k, Jn, dn, cc);
+                sample_jac<ST, ZH, PF>(g, b, q, k, Jn, dn);
                 if (r == 3) {
                     float xp[3];
 #pragma unroll
@@ -591,7 +585,8 @@ extern "C" size_t jt_track_grid_adjoint_workspace_size(int64_t n, int32_t n_step
 }
 
 extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_boundary* bc, const float* ckpt,
-                                     int64_t n, const float* step_times, float dt, int32_t n_steps,
+                                     int64_t n, const float* step_times, const float* host_step_times,
+                                     const float* host_field_times, float dt, int32_t n_steps,
                                      int32_t ckpt_every, const float* grad_xT, float* grad_x0, float* grad_U,
                                      void* ws, size_t ws_bytes, void* stream) {
     GridDev g;
@@ -610,7 +605,8 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
     JT_REQUIRE(n_steps >= 0 && ckpt_every >= 1, "bad n_steps / ckpt_every");
     if (n == 0) return JT_OK;
-    JT_REQUIRE(ckpt && grad_xT && grad_U && (n_steps == 0 || step_times), "NULL pointer");
+    JT_REQUIRE(ckpt && grad_xT && grad_U && (n_steps == 0 || (step_times && host_step_times && host_field_times)),
+               "NULL pointer");
     hipStream_t s = (hipStream_t)stream;
     Arena arena(ws, ws_bytes);
     Blend* tab = arena.take<Blend>((size_t)3 * (n_steps > 0 ? n_steps : 1));
@@ -645,11 +641,18 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     int bits = 1;                                      // bits per axis of the shifted cell index
     while (((maxn - 1) >> shift) >> bits) ++bits;
     const int cb = bucket_bits(bits, n);
-    // the blend table on the host, to cut sub-segments where the snapshot
-    // pair changes (the LDS box holds two snapshot slots)
+    // the stage blends on the host, from the caller's host copies of the
+    // step times and snapshot times (k_blend_table's float32 operations, so
+    // the same snapshot pairs), to cut sub-segments where the pair changes
+    // (the LDS box holds two snapshot slots) without reading the device
     std::vector<Blend> htab((size_t)3 * n_steps);
-    JT_HIP(hipMemcpyAsync(htab.data(), tab, htab.size() * sizeof(Blend), hipMemcpyDeviceToHost, s));
-    JT_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < n_steps; ++k) {
+        const float t = host_step_times[k];
+        const float t1 = t + dth, t2 = t + dt;
+        htab[3 * (size_t)k + 0] = time_blend(host_field_times, g.T, t, g.interp, g.extrap);
+        htab[3 * (size_t)k + 1] = time_blend(host_field_times, g.T, t1, g.interp, g.extrap);
+        htab[3 * (size_t)k + 2] = time_blend(host_field_times, g.T, t2, g.interp, g.extrap);
+    }
     dim3 b256(256), g256((unsigned)((n + 255) / 256));
     dim3 g256x((unsigned)((((n + 255) / 256) + 7) / 8 * 8));   // multiple of 8 for the XCD remap
     dim3 badj(ADJ_BLOCK), gadj((unsigned)((((n + ADJ_BLOCK - 1) / ADJ_BLOCK) + 7) / 8 * 8));

@@ -468,6 +468,11 @@ extern "C" int jt_hashgrid_build(const jt_hashgrid* h, const float* P, void* ws,
 // points do not leave most lanes idle.  Where the valid candidates number at
 // most K (every one of them is among the K nearest) that sum is the density;
 // otherwise it writes JT_SPH_REDO for the top-K pass.
+// x32 keys drop the cell's x index (probe_cell), so the three x offsets of
+// the reference's stencil probe the same cells: the candidate list is the
+// 9 (2-D: 3) distinct cells' list three times over.  Those cells are read
+// once and every candidate counts three times (3 nvalid against K, 3 times
+// the sum: the same multiset of terms the reference adds).
 __global__ void __launch_bounds__(256)
 k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K,
              const float* __restrict__ masses, const float* __restrict__ hs, float hmean, int kern,
@@ -481,10 +486,12 @@ k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int 
     for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
     int cq[3] = {0, 0, 0};
     for (int a = 0; a < D; ++a) cq[a] = cell_coord(q[a], g.origin[a], g.inv);
-    const int probes = (D == 3) ? 27 : 9;
+    const bool x32 = g.key_mode == JT_KEY_X32;
+    const int mult = x32 ? 3 : 1;              // copies of each distinct probe cell in the stencil
+    const int probes = ((D == 3) ? 27 : 9) / mult;
     int st = 0, cnt = 0;
     if (lane < probes) {                       // lane p: probe p's cell, in the reference's offset order
-        const int p = lane;
+        const int p = x32 ? lane + probes : lane;   // x32: the di = 0 third of the stencil
         const int di = (D == 3) ? p / 9 - 1 : p / 3 - 1;
         const int dj = (D == 3) ? (p / 3) % 3 - 1 : p % 3 - 1;
         const int dk = (D == 3) ? p % 3 - 1 : 0;
@@ -526,8 +533,9 @@ k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int 
             acc += mass * sph_w(kern, D, d, h);
         }
     }
-    nvalid = wave_sum_i(nvalid);
+    nvalid = wave_sum_i(nvalid) * mult;
     acc = wave_sum(acc);
+    if (x32) acc = acc * 3.0f;
     if (lane == 0) out_rho[qi] = nvalid <= K ? acc : __uint_as_float(JT_SPH_REDO);
 }
 

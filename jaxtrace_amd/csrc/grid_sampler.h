@@ -37,7 +37,7 @@ struct GridDev {
 };
 
 // searchsorted(times, t, side='right') for non-decreasing times.
-__device__ __forceinline__ int search_right(const float* times, int T, float t) {
+__host__ __device__ __forceinline__ int search_right(const float* times, int T, float t) {
     int lo = 0, hi = T;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -47,7 +47,9 @@ __device__ __forceinline__ int search_right(const float* times, int T, float t) 
 }
 
 // time_series.py:155-245, scalar part (which snapshots and which weight).
-__device__ inline Blend time_blend(const float* times, int T, float t, int interp, int extrap) {
+// Also run on the host (float32 operations with the same values) where the
+// host needs a step's snapshot pair without reading the device.
+__host__ __device__ inline Blend time_blend(const float* times, int T, float t, int interp, int extrap) {
     Blend b;
     b.i0 = 0; b.i1 = 0; b.a = 0.0f; b.mode = BL_COPY;
     if (interp == JT_TIME_NEAREST) {                  // :171-174, first minimum
@@ -61,7 +63,8 @@ __device__ inline Blend time_blend(const float* times, int T, float t, int inter
         return b;
     }
     int ir = search_right(times, T, t);               // :162-164
-    int il = min(max(ir - 1, 0), T - 1);
+    int il = ir - 1 < 0 ? 0 : ir - 1;
+    il = il > T - 1 ? T - 1 : il;
     float tmin = times[0], tmax = times[T - 1];
     bool left = t < tmin, right = t > tmax;
     if (left || right) {
@@ -85,7 +88,7 @@ __device__ inline Blend time_blend(const float* times, int T, float t, int inter
         }
         return b;
     }
-    int irc = min(max(ir, 0), T - 1);                  // :178-188
+    int irc = ir > T - 1 ? T - 1 : ir;                 // :178-188 (ir >= 0)
     float t0 = times[il], t1 = times[irc];
     float dd = t1 - t0;
     dd = dd < 1e-12f ? 1e-12f : dd;

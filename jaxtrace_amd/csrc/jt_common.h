@@ -75,7 +75,7 @@ __device__ __forceinline__ float jmodf(float a, float n) {
     return r;
 }
 // jnp.clip (minimum(maximum(x, lo), hi)) keeping NaN.
-__device__ __forceinline__ float clipf(float x, float lo, float hi) {
+__host__ __device__ __forceinline__ float clipf(float x, float lo, float hi) {
     float y = x < lo ? lo : x;
     return y > hi ? hi : y;
 }

@@ -6,24 +6,11 @@
 //   256 queries of the block.  Terms are accumulated in float64.
 // k_kde_tile_mfma (below; jt_kde_grid_cutoff runs it): the cut-off sum on
 //   the structured grid on the matrix cores, 16 x 16 x 16 tiles.
-// k_kde_grid_cutoff (built with KDE_VALU=1 instead; 5x slower at 256^3): for
-//   the structured evaluation grid of evaluate_3d
-//   (meshgrid 'xy' order, index (iy*nx + ix)*nz + iz).  A block owns an
-//   8 x 8 x 16 tile of grid points (a thread: one (x, y) column, 4 z points)
-//   and streams, through LDS, the particles of the hash-grid cells that
-//   overlap the tile's box grown by R; every particle within R of any tile
-//   point is summed (and some farther ones).  The Gaussian is separable on
-//   the lattice: per particle the block evaluates exp(-0.5 d^2 / h^2) once
-//   for each of the tile's 8 x, 8 y and 16 z coordinates, and a pair's term
-//   is the product of its three factors (a few ulp from exp of the sum, far
-//   inside the 1e-5 density bar).  Each dropped particle is farther than R
-//   from the point, so each dropped term is below norm*exp(-R^2/(2h^2)).
+//   Each dropped particle is farther than R from the grid point, so each
+//   dropped term is below norm*exp(-R^2/(2h^2)).
 #include "jt_common.h"
 #include <algorithm>
 
-#ifndef KDE_VALU
-#define KDE_VALU 0                 // 1: the 8 x 8 x 16-tile VALU kernel instead of the MFMA one
-#endif
 
 namespace jt {
 
@@ -64,108 +51,16 @@ struct KGrid {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-__global__ void __launch_bounds__(256)
-k_kde_grid_cutoff(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
-                  const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
-                  float* __restrict__ out) {
-    __shared__ float ex[256][8];
-    __shared__ float ey[256][8];
-    __shared__ float4 ez[256][4];
-    const int tid = threadIdx.x;
-    const int tiles_z = (nz + 15) / 16, tiles_x = (nx + 7) / 8;
-    int b = blockIdx.x;
-    const int bz = b % tiles_z; b /= tiles_z;
-    const int bx = b % tiles_x; b /= tiles_x;
-    const int by = b;
-    const int zq = tid & 3, txl = (tid >> 2) & 7, tyl = tid >> 5;
-    const int ix = bx * 8 + txl, iy = by * 8 + tyl, iz0 = bz * 16 + zq * 4;
-    // the tile's coordinates (clamped at the grid's end; clamped points are not written)
-    float xs[8], ys[8], zs[16];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) { xs[i] = gx[min(bx * 8 + i, nx - 1)]; ys[i] = gy[min(by * 8 + i, ny - 1)]; }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) zs[k] = gz[min(bz * 16 + k, nz - 1)];
-    float lo[3] = {xs[0], ys[0], zs[0]}, hi[3] = {xs[0], ys[0], zs[0]};
-#pragma unroll
-    for (int i = 1; i < 8; ++i) {
-        lo[0] = fminf(lo[0], xs[i]); hi[0] = fmaxf(hi[0], xs[i]);
-        lo[1] = fminf(lo[1], ys[i]); hi[1] = fmaxf(hi[1], ys[i]);
-    }
-#pragma unroll
-    for (int k = 1; k < 16; ++k) { lo[2] = fminf(lo[2], zs[k]); hi[2] = fmaxf(hi[2], zs[k]); }
-    int c0[3], c1[3];
-    for (int a = 0; a < 3; ++a) {
-        c0[a] = clampi((int)floorf((lo[a] - R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
-        c1[a] = clampi((int)floorf((hi[a] + R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
-    }
-    const float nh = -0.5f * inv_h2;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int cx = c0[0]; cx <= c1[0]; ++cx)
-        for (int cy = c0[1]; cy <= c1[1]; ++cy) {
-            // cells (cx, cy, c0z..c1z) are consecutive ids: their points are one run
-            const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + c0[2];
-            const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + c1[2];
-            int s = 0x7fffffff, e = 0;
-            for (int id = id0; id <= id1; ++id) {
-                int cs = g.cstart[id], ce = g.cend[id];
-                if (ce > cs) { s = min(s, cs); e = max(e, ce); }
-            }
-            for (int base = s; base < e; base += 256) {
-                const int j = base + tid;
-                // this thread's particle: its factor for each tile coordinate
-                if (j < e) {
-                    const float px = g.spos[(size_t)j * 3], py = g.spos[(size_t)j * 3 + 1];
-                    const float pz = g.spos[(size_t)j * 3 + 2];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const float dx = xs[i] - px, dy = ys[i] - py;
-                        ex[tid][i] = __expf(nh * (dx * dx));
-                        ey[tid][i] = __expf(nh * (dy * dy));
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        float4 v;
-                        float dz;
-                        dz = zs[4 * q] - pz;     v.x = __expf(nh * (dz * dz));
-                        dz = zs[4 * q + 1] - pz; v.y = __expf(nh * (dz * dz));
-                        dz = zs[4 * q + 2] - pz; v.z = __expf(nh * (dz * dz));
-                        dz = zs[4 * q + 3] - pz; v.w = __expf(nh * (dz * dz));
-                        ez[tid][q] = v;
-                    }
-                }
-                __syncthreads();
-                const int cnt = min(256, e - base);
-                float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int k = 0; k < cnt; ++k) {
-                    const float exy = ex[k][txl] * ey[k][tyl];
-                    const float4 z4 = ez[k][zq];
-                    part[0] = fmaf(exy, z4.x, part[0]);
-                    part[1] = fmaf(exy, z4.y, part[1]);
-                    part[2] = fmaf(exy, z4.z, part[2]);
-                    part[3] = fmaf(exy, z4.w, part[3]);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] += (double)part[q];
-                __syncthreads();
-            }
-        }
-    if (ix < nx && iy < ny)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (iz0 + q < nz) out[((size_t)iy * nx + ix) * nz + iz0 + q] = (float)(acc[q] * scale);
-}
-
 // k_kde_tile_mfma: the same sum with 16 x 16 x 16 tiles on the matrix cores.
 // For a tile and a batch of particles p the densities are a matrix product,
 //   D[(x, y), z] += sum_p (ex[x, p] * ey[y, p]) * ez[p, z],
 // with the separable factors e*[., p] = exp(-0.5 d^2 / h^2) per tile
-// coordinate.  The larger tile shares the 2R halo of its box among 4x the
-// points of the 8 x 8 x 16 tile above.  Each of the 4 waves owns 4 of the
+// coordinate.  A 16^3 tile shares the 2R halo of its box among 4x the
+// points of an 8 x 8 x 16 tile (the VALU kernel this replaced).  Each of the 4 waves owns 4 of the
 // tile's 16 y rows: per 4 particles it issues one v_mfma_f32_16x16x4_f32 per
 // row (A: 16 x by 4 particles, B: 4 particles by 16 z), so a pair costs an
 // MFMA quarter-slot instead of a VALU multiply-add.  The fp32 MFMA sums run
-// over one batch of 256 particles and are then added in float64, as in the
-// VALU kernel.
+// over one pass of staged particles and are then added in float64.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 #define KDE_T 16
 #define KDE_BATCH 256                                 // particles read per round (one per thread)
@@ -406,12 +301,6 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
     const double hh = (double)h;
     const float inv_h2 = (float)(1.0 / (hh * hh));
     const double norm = pow(2.0 * M_PI, -1.5);
-#if KDE_VALU
-    long long blocks = (long long)((ny + 7) / 8) * ((nx + 7) / 8) * ((nz + 15) / 16);
-    JT_REQUIRE(blocks < (1LL << 31), "grid too large");
-    hipLaunchKernelGGL(k_kde_grid_cutoff, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                       gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, out);
-#else
     const long long tiles = (long long)((ny + KDE_T - 1) / KDE_T) * ((nx + KDE_T - 1) / KDE_T) * ((nz + KDE_T - 1) / KDE_T);
     // about KDE_TARGET_BLOCKS blocks (the chip holds 768 at a time), at most KDE_MAX_SPLIT per tile
     const int nsplit = (int)std::max(1LL, std::min((long long)KDE_MAX_SPLIT, (long long)KDE_TARGET_BLOCKS / tiles));
@@ -422,7 +311,6 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
     JT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_kde_tile_mfma, dim3((unsigned)(tiles * nsplit)), dim3(256), 0, (hipStream_t)stream,
                        gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, nsplit, out);
-#endif
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

@@ -337,12 +337,7 @@ static void launch_track(hipStream_t s, const GridDev& g, const BcDev& bc, const
                          const Blend* tab, int s0, int s1, float dt, float dth, float dt6, int rec, float* traj) {
     const unsigned B = TrackBlock<ST>::v;
     const unsigned nb = ((unsigned)n + B - 1) / B;
-#ifdef JT_F32_LDS
-    const size_t lds = (ST == ST_F32x3 || ST == ST_F32x4) ? JT_F32_LDS : 0;
-#else
-    const size_t lds = 0;
-#endif
-    hipLaunchKernelGGL((k_track<INTEG, ST, ZH, PF>), dim3((nb + 7) / 8 * 8), dim3(B), lds, s, g, bc, ro, n, tab, s0, s1,
+    hipLaunchKernelGGL((k_track<INTEG, ST, ZH, PF>), dim3((nb + 7) / 8 * 8), dim3(B), 0, s, g, bc, ro, n, tab, s0, s1,
                        dt, dth, dt6, rec, traj);
 }
 

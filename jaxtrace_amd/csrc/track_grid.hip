@@ -496,15 +496,21 @@ extern "C" size_t jt_track_grid_workspace_size(int64_t n, int32_t n_steps, const
     return a.used + 256;
 }
 
-extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, int32_t integrator,
-                             float* x, int64_t n, const float* step_times, float dt, int32_t n_steps,
-                             int32_t record_every, float* traj, const jt_track_opts* opts,
-                             void* ws, size_t ws_bytes, void* stream) {
-    GridDev g;
-    int st;
+// The cell-sort buffers of one tracking call (all NULL: no sorting).
+struct SortBufs {
+    unsigned *keys = nullptr, *skeys = nullptr;
+    int *perm = nullptr, *sperm = nullptr, *ids = nullptr, *ids2 = nullptr, *bcount = nullptr, *boffs = nullptr;
+    float *xa = nullptr, *xb = nullptr;
+    char* tmp = nullptr;
+    size_t temp = 0;
+};
+
+// Checks shared by the tracking entry points; fills g/st/d.
+static int track_grid_args(const jt_grid_field* field, const jt_boundary* bc, int32_t integrator, const float* x,
+                           int64_t n, const float* step_times, int32_t n_steps, int32_t record_every,
+                           const jt_track_opts* opts, GridDev& g, int& st, BcDev& d) {
     int rc = make_grid_dev(field, g, st);
     if (rc) return rc;
-    BcDev d;
     rc = make_bc_dev(bc, d);
     if (rc) return rc;
     JT_REQUIRE(integrator >= JT_EULER && integrator <= JT_RK4, "bad integrator %d", integrator);
@@ -517,36 +523,22 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
     JT_REQUIRE(n == 0 || x != nullptr, "NULL x");
     JT_REQUIRE(n_steps == 0 || step_times != nullptr, "NULL step_times");
     JT_REQUIRE(opts == nullptr || (opts->sort_every >= 0 && opts->steps_per_launch >= 0), "bad opts");
-    hipStream_t s = (hipStream_t)stream;
+    return JT_OK;
+}
+
+// The tracking run itself: tab holds room for 3 * n_steps blends; sb holds
+// the cell-sort buffers when opts->sort_every > 0.
+static int track_grid_run(const GridDev& g, int st, const BcDev& d, int32_t integrator, float* x, int64_t n,
+                          const float* step_times, float dt, int32_t n_steps, int32_t record_every, float* traj,
+                          const jt_track_opts* opts, Blend* tab, const SortBufs& sb, hipStream_t s) {
     if (traj != nullptr && n > 0)
         JT_HIP(hipMemcpyAsync(traj, x, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
     if (n == 0 || n_steps == 0) return JT_OK;
-
     const int sort_every = opts ? opts->sort_every : 0;
-    const size_t need = jt_track_grid_workspace_size(n, n_steps, opts);
-    Arena arena(ws, ws_bytes);
-    Blend* tab = arena.take<Blend>((size_t)3 * n_steps);
-    unsigned *keys = nullptr, *skeys = nullptr;
-    int *perm = nullptr, *sperm = nullptr, *ids = nullptr, *ids2 = nullptr, *bcount = nullptr, *boffs = nullptr;
-    float *xa = nullptr, *xb = nullptr;
-    char* tmp = nullptr;
-    size_t temp = 0;
     // a counting sort by bucket below TRACK_BUCKET_SORT_MAX_N particles (2^20:
     // 12.5 against 14.0 ms for 500 steps), a radix sort above (2^24: 0.57
     // against 1.1 ms per sort, and better ordered waves)
     const bool bucketed = n < TRACK_BUCKET_SORT_MAX_N;
-    if (sort_every > 0) {
-        keys = arena.take<unsigned>(n); skeys = arena.take<unsigned>(n);
-        perm = arena.take<int>(n); sperm = arena.take<int>(n);
-        ids = arena.take<int>(n); ids2 = arena.take<int>(n);
-        xa = arena.take<float>((size_t)3 * n); xb = arena.take<float>((size_t)3 * n);
-        bcount = arena.take<int>(1 << TRACK_BUCKET_BITS_MAX); boffs = arena.take<int>(1 << TRACK_BUCKET_BITS_MAX);
-        temp = bucketed ? bucket_scan_bytes(1 << TRACK_BUCKET_BITS_MAX) : sort_temp_bytes((int)n);
-        tmp = arena.take<char>(temp);
-    }
-    if (tab == nullptr || !arena.ok() || (sort_every > 0 && tmp == nullptr))
-        return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu", need, ws_bytes);
-
     hipLaunchKernelGGL(k_blend_table, dim3((n_steps + 127) / 128), dim3(128), 0, s, g, step_times,
                        n_steps, dt, integrator, tab);
     JT_HIP(hipGetLastError());
@@ -573,16 +565,19 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
             int next_sort = (s0 / sort_every + 1) * sort_every;
             if (s1 > next_sort) s1 = next_sort;
             if (s0 % sort_every == 0) {
-                hipLaunchKernelGGL(k_cell_keys, eg, eb, 0, s, g, cur, (int)n, kb, keys, bucketed ? nullptr : perm);
+                hipLaunchKernelGGL(k_cell_keys, eg, eb, 0, s, g, cur, (int)n, kb, sb.keys,
+                                   bucketed ? nullptr : sb.perm);
                 if (bucketed)
-                    JT_HIP(bucket_sort(s, keys, (int)n, bshift, 1 << (kb.nbits - bshift), bcount, boffs, tmp, temp,
-                                       sperm));
-                else
-                    JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, perm, sperm, (int)n, 0,
-                                                              kb.nbits, s));
-                float* nx = (cur == xa) ? xb : xa;
-                int* nid = (cur_ids == ids) ? ids2 : ids;
-                ro = Reorder{cur, nx, cur_ids, nid, sperm};   // the launch below applies the permutation
+                    JT_HIP(bucket_sort(s, sb.keys, (int)n, bshift, 1 << (kb.nbits - bshift), sb.bcount, sb.boffs,
+                                       sb.tmp, sb.temp, sb.sperm));
+                else {
+                    size_t temp = sb.temp;
+                    JT_HIP(hipcub::DeviceRadixSort::SortPairs(sb.tmp, temp, sb.keys, sb.skeys, sb.perm, sb.sperm,
+                                                              (int)n, 0, kb.nbits, s));
+                }
+                float* nx = (cur == sb.xa) ? sb.xb : sb.xa;
+                int* nid = (cur_ids == sb.ids) ? sb.ids2 : sb.ids;
+                ro = Reorder{cur, nx, cur_ids, nid, sb.sperm};   // the launch below applies the permutation
                 cur = nx;
                 cur_ids = nid;
             }
@@ -609,4 +604,37 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
         JT_HIP(hipGetLastError());
     }
     return JT_OK;
+}
+
+extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, int32_t integrator,
+                             float* x, int64_t n, const float* step_times, float dt, int32_t n_steps,
+                             int32_t record_every, float* traj, const jt_track_opts* opts,
+                             void* ws, size_t ws_bytes, void* stream) {
+    GridDev g;
+    int st;
+    BcDev d;
+    int rc = track_grid_args(field, bc, integrator, x, n, step_times, n_steps, record_every, opts, g, st, d);
+    if (rc) return rc;
+    if (n == 0 || n_steps == 0) return track_grid_run(g, st, d, integrator, x, n, step_times, dt, n_steps,
+                                                      record_every, traj, opts, nullptr, SortBufs{},
+                                                      (hipStream_t)stream);
+    const int sort_every = opts ? opts->sort_every : 0;
+    const size_t need = jt_track_grid_workspace_size(n, n_steps, opts);
+    Arena arena(ws, ws_bytes);
+    Blend* tab = arena.take<Blend>((size_t)3 * n_steps);
+    SortBufs sb;
+    if (sort_every > 0) {
+        const bool bucketed = n < TRACK_BUCKET_SORT_MAX_N;
+        sb.keys = arena.take<unsigned>(n); sb.skeys = arena.take<unsigned>(n);
+        sb.perm = arena.take<int>(n); sb.sperm = arena.take<int>(n);
+        sb.ids = arena.take<int>(n); sb.ids2 = arena.take<int>(n);
+        sb.xa = arena.take<float>((size_t)3 * n); sb.xb = arena.take<float>((size_t)3 * n);
+        sb.bcount = arena.take<int>(1 << TRACK_BUCKET_BITS_MAX); sb.boffs = arena.take<int>(1 << TRACK_BUCKET_BITS_MAX);
+        sb.temp = bucketed ? bucket_scan_bytes(1 << TRACK_BUCKET_BITS_MAX) : sort_temp_bytes((int)n);
+        sb.tmp = arena.take<char>(sb.temp);
+    }
+    if (tab == nullptr || !arena.ok() || (sort_every > 0 && sb.tmp == nullptr))
+        return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu", need, ws_bytes);
+    return track_grid_run(g, st, d, integrator, x, n, step_times, dt, n_steps, record_every, traj, opts, tab, sb,
+                          (hipStream_t)stream);
 }

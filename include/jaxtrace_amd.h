@@ -178,6 +178,22 @@ int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc,
                   const jt_track_opts* opts, void* ws, size_t ws_bytes,
                   void* stream);
 
+/* jt_track_grid without a workspace.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The stage-time
+ * table lives in a static device array of the library (3 MiB per GPU, kept
+ * until the process ends), so:
+ *   - n_steps <= 65536 per call (JT_EINVAL beyond: split the run);
+ *   - no cell sorting (opts->sort_every must be 0, JT_EUNSUPPORTED otherwise);
+ *   - calls that overlap on one GPU (several streams or threads) race on that
+ *     array: serialise them.
+ * Same arguments, results and recording as jt_track_grid with sort_every = 0;
+ * the call makes no allocation (no hipMalloc/hipFree, no allocating library
+ * routine). */
+int jt_track_grid_noalloc(const jt_grid_field* field, const jt_boundary* bc, int32_t integrator, float* x,
+                          int64_t n, const float* step_times, float dt, int32_t n_steps, int32_t record_every,
+                          float* traj, const jt_track_opts* opts, void* stream);
+
 /* ---- hash-grid neighbours and SPH density ------------------------------- */
 /* cell-key packing: neighbors.py:83-91.  The reference never enables
  * jax_enable_x64, so its int64 keys are int32 and `cx << 42` is 0: the

@@ -106,6 +106,9 @@ _SIGS = {
     "jt_track_grid": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
                                 C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                 C.POINTER(TrackOpts), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "jt_track_grid_noalloc": (C.c_int, [C.POINTER(GridField), C.POINTER(Boundary), C.c_int32, C.c_void_p,
+                                        C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
+                                        C.POINTER(TrackOpts), C.c_void_p]),
     "jt_points_bounds": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_size_t, C.c_void_p]),
     "jt_hashgrid_num_cells": (C.c_int64, [C.POINTER(HashGridDesc)]),

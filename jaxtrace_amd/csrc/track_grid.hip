@@ -18,6 +18,7 @@
 #include "cell_sort.h"
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace jt {
 
@@ -637,4 +638,56 @@ extern "C" int jt_track_grid(const jt_grid_field* field, const jt_boundary* bc, 
         return fail(JT_ENOMEM, "workspace too small: need %zu bytes, got %zu", need, ws_bytes);
     return track_grid_run(g, st, d, integrator, x, n, step_times, dt, n_steps, record_every, traj, opts, tab, sb,
                           (hipStream_t)stream);
+}
+
+// ---- allocation-free tracking -------------------------------------------------
+// jt_track_grid_noalloc's only scratch, the stage-time blend table, lives in
+// this file-scope array: JT_NOALLOC_MAX_STEPS steps x 3 blends x 16 B =
+// 3 MiB, held on every GPU the library runs on until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): its device address is looked up once per
+// device with hipGetSymbolAddress and kept in noalloc_ptr[].
+#define JT_NOALLOC_MAX_STEPS (1 << 16)
+#define JT_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) Blend scratch[3 * JT_NOALLOC_MAX_STEPS];
+
+static std::mutex noalloc_mu;
+static void* noalloc_ptr[JT_NOALLOC_MAX_DEVICES];
+
+// The current device's address of `scratch` (looked up on its first use).
+static int noalloc_scratch(Blend** out) {
+    int dev = 0;
+    JT_HIP(hipGetDevice(&dev));
+    JT_REQUIRE(dev >= 0 && dev < JT_NOALLOC_MAX_DEVICES, "device %d beyond the supported %d", dev,
+               JT_NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        JT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        JT_REQUIRE(p != nullptr, "hipGetSymbolAddress returned NULL for the tracking scratch");
+        noalloc_ptr[dev] = p;
+    }
+    *out = (Blend*)noalloc_ptr[dev];
+    return JT_OK;
+}
+
+extern "C" int jt_track_grid_noalloc(const jt_grid_field* field, const jt_boundary* bc, int32_t integrator,
+                                     float* x, int64_t n, const float* step_times, float dt, int32_t n_steps,
+                                     int32_t record_every, float* traj, const jt_track_opts* opts, void* stream) {
+    GridDev g;
+    int st;
+    BcDev d;
+    int rc = track_grid_args(field, bc, integrator, x, n, step_times, n_steps, record_every, opts, g, st, d);
+    if (rc) return rc;
+    if (n_steps > JT_NOALLOC_MAX_STEPS)
+        return fail(JT_EINVAL, "jt_track_grid_noalloc takes at most %d steps per call (got %d): split the run "
+                               "into calls of at most that many steps", JT_NOALLOC_MAX_STEPS, n_steps);
+    if (opts != nullptr && opts->sort_every > 0)
+        return fail(JT_EUNSUPPORTED, "jt_track_grid_noalloc does not sort particles by cell (the sort needs "
+                                     "O(n) buffers): pass sort_every = 0, or use jt_track_grid with a workspace");
+    Blend* tab = nullptr;
+    rc = noalloc_scratch(&tab);
+    if (rc) return rc;
+    return track_grid_run(g, st, d, integrator, x, n, step_times, dt, n_steps, record_every, traj, opts, tab,
+                          SortBufs{}, (hipStream_t)stream);
 }

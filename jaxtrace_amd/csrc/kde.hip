@@ -51,21 +51,38 @@ struct KGrid {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// k_kde_tile_mfma: the same sum with 16 x 16 x 16 tiles on the matrix cores.
-// For a tile and a batch of particles p the densities are a matrix product,
+// k_kde_tile_mfma: the cut-off sum with 16 x 16 x 16 tiles on the matrix
+// cores.  For a tile and a pass of particles p the densities are a matrix
+// product,
 //   D[(x, y), z] += sum_p (ex[x, p] * ey[y, p]) * ez[p, z],
 // with the separable factors e*[., p] = exp(-0.5 d^2 / h^2) per tile
-// coordinate.  A 16^3 tile shares the 2R halo of its box among 4x the
-// points of an 8 x 8 x 16 tile (the VALU kernel this replaced).  Each of the 4 waves owns 4 of the
-// tile's 16 y rows: per 4 particles it issues one v_mfma_f32_16x16x4_f32 per
-// row (A: 16 x by 4 particles, B: 4 particles by 16 z), so a pair costs an
-// MFMA quarter-slot instead of a VALU multiply-add.  The fp32 MFMA sums run
-// over one pass of staged particles and are then added in float64.
+// coordinate.  Each of the 4 waves owns 4 of the tile's 16 y rows.
+//
+// Precision: the products run on the bfloat16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, 16 times the float32 MFMA rate) with each
+// float32 operand split into two bfloat16 halves, hi = bf16(v) and
+// lo = bf16(v - hi) (both round-to-nearest; v - hi is exact), so
+// v = hi + lo within 2^-18 |v|.  The four products hi*hi + hi*lo + lo*hi +
+// lo*lo are exact in the float32 accumulator, so each term a*b carries a
+// relative error of at most 2^-17 (7.6e-6), of either sign, and the
+// accumulation over a pass is float32, then float64 across passes.
+// A = ex * ey is formed (one float32 multiply) and split per wave; the z
+// factors are split once per particle when the factors are computed.
+//
+// Culling: a particle is staged only if its distance to the tile's box is
+// at most R (every dropped particle is farther than R from every tile
+// point), and hash-cell columns whose (x, y) distance to the box exceeds R
+// are not read at all; a column's z range is cut to the z half-width left
+// over by its (x, y) distance.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 #define KDE_T 16
 #define KDE_BATCH 256                                 // particles read per round (one per thread)
-#define KDE_SLOTS 240                                 // particles per MFMA pass (60 k-steps)
-#define KDE_STAGE (KDE_SLOTS + KDE_BATCH - 1)         // staged positions at most
+#ifndef KDE_P
+#define KDE_P 256                                     // particles per MFMA pass (a multiple of 32)
+#endif
+#define KDE_STAGE (KDE_P + KDE_BATCH - 1)             // staged positions at most
+#define KDE_LDP (KDE_P + 8)                           // padded row length of the factor tables
 #ifndef KDE_TARGET_BLOCKS
 #define KDE_TARGET_BLOCKS 32768
 #endif
@@ -77,6 +94,28 @@ __global__ void __launch_bounds__(256) k_kde_zero(float* __restrict__ out, size_
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = 0.0f;
 }
 
+// Two float32 values rounded to a packed bfloat16 pair, and back.
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(unsigned, v);
+}
+__device__ __forceinline__ float bf16_lo_f32(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi_f32(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Eight float32 values split into their bfloat16 hi and lo halves.
+__device__ __forceinline__ void split8(const float v[8], bf16x8& hi, bf16x8& lo) {
+    unsigned h[4], l[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        h[k] = pk_bf16(v[2 * k], v[2 * k + 1]);
+        l[k] = pk_bf16(v[2 * k] - bf16_lo_f32(h[k]), v[2 * k + 1] - bf16_hi_f32(h[k]));
+    }
+    typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
+    hi = __builtin_bit_cast(bf16x8, (u32x4v){h[0], h[1], h[2], h[3]});
+    lo = __builtin_bit_cast(bf16x8, (u32x4v){l[0], l[1], l[2], l[3]});
+}
+
 // nsplit blocks share a tile: block b takes tile b / nsplit and, of the
 // tile's cell columns (cx, cy), those with index = b % nsplit modulo nsplit,
 // adding its partial densities into out (zeroed first).  The particle
@@ -86,9 +125,10 @@ __global__ void __launch_bounds__(256)
 k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
                 const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
                 int nsplit, float* __restrict__ out) {
-    __shared__ float ex[KDE_SLOTS][KDE_T];
-    __shared__ float4 ey[KDE_SLOTS][KDE_T / 4];
-    __shared__ float ez[KDE_SLOTS][KDE_T];
+    __shared__ __attribute__((aligned(16))) float exs[KDE_T][KDE_LDP];        // x factors [x][p]
+    __shared__ __attribute__((aligned(16))) float eys[KDE_T][KDE_LDP];        // y factors [y][p]
+    __shared__ __attribute__((aligned(16))) __bf16 ezh[KDE_T][KDE_LDP];       // z factors [z][p], hi half
+    __shared__ __attribute__((aligned(16))) __bf16 ezl[KDE_T][KDE_LDP];       // lo half
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tiles_z = (nz + KDE_T - 1) / KDE_T, tiles_x = (nx + KDE_T - 1) / KDE_T;
     const int part = blockIdx.x % nsplit;
@@ -116,8 +156,10 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
         c0[a] = clampi((int)floorf((lo[a] - R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
         c1[a] = clampi((int)floorf((hi[a] + R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
     }
+    const float R2 = R * R;
+    const float cell = 1.0f / g.inv;
     const float nh = -0.5f * inv_h2;
-    const int col = lane & 15, kq = lane >> 4;   // MFMA operand lane: A row / B column, and k
+    const int col = lane & 15, kg = lane >> 4;   // MFMA operand lane: A row / B column, and k group
     __shared__ int wcount[2][4];
     __shared__ float sx[KDE_STAGE], sy[KDE_STAGE], sz[KDE_STAGE];
     double accd[4][4];
@@ -125,59 +167,57 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) accd[r][q] = 0.0;
-    // Sum the first n (<= KDE_SLOTS) staged particles: their factors (one
-    // particle per thread), then the MFMA k-steps over them.  Block-uniform.
+    // Sum the first n (<= KDE_P) staged particles: their factors (one
+    // particle per thread), then the MFMA passes over them.  Block-uniform.
     auto consume = [&](int n) {
-        const int n4 = (n + 3) & ~3;              // padded with zero factors to whole k-steps
-        if (tid < n4) {
+        const int n32 = (n + 31) & ~31;            // padded with zero x factors to whole 32-particle steps
+        if (tid < n32) {
             const bool real = tid < n;
             const float px = real ? sx[tid] : 0.0f, py = real ? sy[tid] : 0.0f, pz = real ? sz[tid] : 0.0f;
-            float fx[KDE_T], fy[KDE_T], fz[KDE_T];
 #pragma unroll
-            for (int i = 0; i < KDE_T; ++i) {
-                const float dx = xs[i] - px, dy = ys[i] - py, dz = zs[i] - pz;
-                fx[i] = real ? __expf(nh * (dx * dx)) : 0.0f;
-                fy[i] = __expf(nh * (dy * dy));
-                fz[i] = __expf(nh * (dz * dz));
-            }
-#pragma unroll
-            for (int i = 0; i < KDE_T; i += 4) {
-                *reinterpret_cast<float4*>(&ex[tid][i]) = make_float4(fx[i], fx[i + 1], fx[i + 2], fx[i + 3]);
-                ey[tid][i / 4] = make_float4(fy[i], fy[i + 1], fy[i + 2], fy[i + 3]);
-                *reinterpret_cast<float4*>(&ez[tid][i]) = make_float4(fz[i], fz[i + 1], fz[i + 2], fz[i + 3]);
+            for (int i = 0; i < KDE_T; i += 2) {
+                const float dx0 = xs[i] - px, dx1 = xs[i + 1] - px;
+                const float dy0 = ys[i] - py, dy1 = ys[i + 1] - py;
+                const float dz0 = zs[i] - pz, dz1 = zs[i + 1] - pz;
+                exs[i][tid] = real ? __expf(nh * (dx0 * dx0)) : 0.0f;
+                exs[i + 1][tid] = real ? __expf(nh * (dx1 * dx1)) : 0.0f;
+                eys[i][tid] = __expf(nh * (dy0 * dy0));
+                eys[i + 1][tid] = __expf(nh * (dy1 * dy1));
+                const float f0 = __expf(nh * (dz0 * dz0)), f1 = __expf(nh * (dz1 * dz1));
+                const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+                ezh[i][tid] = h0;
+                ezh[i + 1][tid] = h1;
+                ezl[i][tid] = (__bf16)(f0 - (float)h0);
+                ezl[i + 1][tid] = (__bf16)(f1 - (float)h1);
             }
         }
         __syncthreads();
         floatx4 acc[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        int p0 = 0;
-        for (; p0 + 16 <= n4; p0 += 16) {             // 4 k-steps: their LDS reads first
-            float ax[4], bz[4];
-            float4 y4[4];
+        for (int p0 = 0; p0 < n32; p0 += 32) {
+            const int pk = p0 + 8 * kg;               // this lane's 8 particles of the step
+            const float4 xa = *reinterpret_cast<const float4*>(&exs[col][pk]);
+            const float4 xb = *reinterpret_cast<const float4*>(&exs[col][pk + 4]);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&ezh[col][pk]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&ezl[col][pk]);
+            const float xv[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int p = p0 + 4 * u + kq;
-                ax[u] = ex[p][col];
-                bz[u] = ez[p][col];
-                y4[u] = ey[p][wv];                     // this wave's rows 4 wv .. 4 wv + 3
-            }
+            for (int r = 0; r < 4; ++r) {
+                const int y = 4 * wv + r;
+                const float4 ya = *reinterpret_cast<const float4*>(&eys[y][pk]);
+                const float4 yb = *reinterpret_cast<const float4*>(&eys[y][pk + 4]);
+                const float yv[8] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y, yb.z, yb.w};
+                float av[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].x, bz[u], acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].y, bz[u], acc[1], 0, 0, 0);
-                acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].z, bz[u], acc[2], 0, 0, 0);
-                acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[u] * y4[u].w, bz[u], acc[3], 0, 0, 0);
+                for (int j = 0; j < 8; ++j) av[j] = xv[j] * yv[j];
+                bf16x8 ah, al;
+                split8(av, ah, al);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[r], 0, 0, 0);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[r], 0, 0, 0);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[r], 0, 0, 0);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[r], 0, 0, 0);
             }
-        }
-        for (; p0 < n4; p0 += 4) {
-            const int p = p0 + kq;
-            const float ax = ex[p][col], bz4 = ez[p][col];
-            const float4 y4 = ey[p][wv];
-            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.x, bz4, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.y, bz4, acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.z, bz4, acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax * y4.w, bz4, acc[3], 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -189,9 +229,22 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
     const int ncy = c1[1] - c0[1] + 1, ncol = (c1[0] - c0[0] + 1) * ncy;
     for (int k = part; k < ncol; k += nsplit) {
         const int cx = c0[0] + k / ncy, cy = c0[1] + k % ncy;
-        // cells (cx, cy, c0z..c1z) are consecutive ids: their points are one run
-        const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + c0[2];
-        const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + c1[2];
+        // the column's (x, y) distance to the box, from its cells' bounds
+        // (widened by a thousandth of a cell against rounding)
+        const float slack = 1e-3f * cell;
+        const float xl = g.origin[0] + (float)cx * cell - slack, xh = g.origin[0] + (float)(cx + 1) * cell + slack;
+        const float yl = g.origin[1] + (float)cy * cell - slack, yh = g.origin[1] + (float)(cy + 1) * cell + slack;
+        const float gx_ = fmaxf(fmaxf(lo[0] - xh, xl - hi[0]), 0.0f);
+        const float gy_ = fmaxf(fmaxf(lo[1] - yh, yl - hi[1]), 0.0f);
+        const float rem = R2 - (gx_ * gx_ + gy_ * gy_);
+        if (rem < 0.0f) continue;                      // every point of the column is farther than R
+        const float rz = sqrtf(rem) + slack;
+        const int z0 = max(c0[2], clampi((int)floorf((lo[2] - rz - g.origin[2]) * g.inv), 0, g.dims[2] - 1));
+        const int z1 = min(c1[2], clampi((int)floorf((hi[2] + rz - g.origin[2]) * g.inv), 0, g.dims[2] - 1));
+        if (z0 > z1) continue;
+        // cells (cx, cy, z0..z1) are consecutive ids: their points are one run
+        const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + z0;
+        const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + z1;
         int s = 0x7fffffff, e = 0;
         for (int id = id0; id <= id1; ++id) {
             const int cs = g.cstart[id], ce = g.cend[id];
@@ -211,11 +264,12 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
                 qx = g.spos[(size_t)(j + KDE_BATCH) * 3]; qy = g.spos[(size_t)(j + KDE_BATCH) * 3 + 1];
                 qz = g.spos[(size_t)(j + KDE_BATCH) * 3 + 2];
             }
-            // keep the particles within R of the tile's box on every axis
-            // (the hash cells overshoot it); each one dropped is farther
-            // than R from every tile point
-            const bool keep = j < e && px >= lo[0] - R && px <= hi[0] + R && py >= lo[1] - R &&
-                              py <= hi[1] + R && pz >= lo[2] - R && pz <= hi[2] + R;
+            // keep the particles within R of the tile's box: each one
+            // dropped is farther than R from every tile point
+            const float dx = fmaxf(fmaxf(lo[0] - px, px - hi[0]), 0.0f);
+            const float dy = fmaxf(fmaxf(lo[1] - py, py - hi[1]), 0.0f);
+            const float dz = fmaxf(fmaxf(lo[2] - pz, pz - hi[2]), 0.0f);
+            const bool keep = j < e && (dx * dx + dy * dy) + dz * dz <= R2;
             // staging slots in thread order (deterministic): wave counts, then
             // offsets; the counts alternate between two rows, as a wave may
             // write the next round's count while another still reads these
@@ -235,13 +289,13 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
                 sx[slot] = px; sy[slot] = py; sz[slot] = pz;
             }
             staged += cnt;
-            while (staged >= KDE_SLOTS) {
+            while (staged >= KDE_P) {
                 __syncthreads();                       // the staged positions are visible
-                consume(KDE_SLOTS);
+                consume(KDE_P);
                 // move the rest to the front
-                const int rest = staged - KDE_SLOTS;
+                const int rest = staged - KDE_P;
                 float a = 0.f, b2 = 0.f, c2 = 0.f;
-                if (tid < rest) { a = sx[KDE_SLOTS + tid]; b2 = sy[KDE_SLOTS + tid]; c2 = sz[KDE_SLOTS + tid]; }
+                if (tid < rest) { a = sx[KDE_P + tid]; b2 = sy[KDE_P + tid]; c2 = sz[KDE_P + tid]; }
                 __syncthreads();
                 if (tid < rest) { sx[tid] = a; sy[tid] = b2; sz[tid] = c2; }
                 staged = rest;
@@ -259,7 +313,7 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
         const int iy = by * KDE_T + 4 * wv + r;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int ix = bx * KDE_T + 4 * kq + q;
+            const int ix = bx * KDE_T + 4 * kg + q;
             if (ix < nx && iy < ny && iz < nz && accd[r][q] != 0.0)
                 unsafeAtomicAdd(&out[((size_t)iy * nx + ix) * nz + iz], (float)(accd[r][q] * scale));
         }

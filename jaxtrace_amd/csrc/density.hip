@@ -46,28 +46,48 @@ __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
+// Per-axis min and max of the points into out[0..2] / out[3..5] (ordered
+// uint32 encodings): each thread folds a grid-stride slice, each wave folds
+// its lanes with shuffles, the block folds its waves in LDS, and one thread
+// per block adds the result atomically (the six words share one cache line,
+// so the atomics are kept to a few per block).
 __global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ P, int n, int dim, unsigned* out) {
-    __shared__ unsigned smin[3][256], smax[3][256];
+    __shared__ unsigned red[4][6];
     unsigned lmin[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, lmax[3] = {0u, 0u, 0u};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        for (int a = 0; a < dim; ++a) {
-            unsigned o = f2ord(P[(size_t)i * dim + a]);
-            lmin[a] = min(lmin[a], o);
-            lmax[a] = max(lmax[a], o);
+    const int stride = gridDim.x * blockDim.x;
+    if (dim == 3) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const float* p = P + (size_t)i * 3;
+            const unsigned a = f2ord(p[0]), b = f2ord(p[1]), c = f2ord(p[2]);
+            lmin[0] = min(lmin[0], a); lmax[0] = max(lmax[0], a);
+            lmin[1] = min(lmin[1], b); lmax[1] = max(lmax[1], b);
+            lmin[2] = min(lmin[2], c); lmax[2] = max(lmax[2], c);
+        }
+    } else {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const float* p = P + (size_t)i * 2;
+            const unsigned a = f2ord(p[0]), b = f2ord(p[1]);
+            lmin[0] = min(lmin[0], a); lmax[0] = max(lmax[0], a);
+            lmin[1] = min(lmin[1], b); lmax[1] = max(lmax[1], b);
         }
     }
-    for (int a = 0; a < 3; ++a) { smin[a][threadIdx.x] = lmin[a]; smax[a][threadIdx.x] = lmax[a]; }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        for (int o = 32; o > 0; o >>= 1) {
+            lmin[a] = min(lmin[a], (unsigned)__shfl_xor((int)lmin[a], o));
+            lmax[a] = max(lmax[a], (unsigned)__shfl_xor((int)lmax[a], o));
+        }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; ++a) { red[wv][a] = lmin[a]; red[wv][3 + a] = lmax[a]; }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int a = 0; a < 3; ++a) {
-                smin[a][threadIdx.x] = min(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
-                smax[a][threadIdx.x] = max(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
-            }
-        __syncthreads();
-    }
     if (threadIdx.x == 0)
-        for (int a = 0; a < dim; ++a) { atomicMin(&out[a], smin[a][0]); atomicMax(&out[3 + a], smax[a][0]); }
+        for (int a = 0; a < dim; ++a) {
+            unsigned mn = red[0][a], mx = red[0][3 + a];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { mn = min(mn, red[w][a]); mx = max(mx, red[w][3 + a]); }
+            atomicMin(&out[a], mn);
+            atomicMax(&out[3 + a], mx);
+        }
 }
 
 __global__ void k_decode_bounds(const unsigned* in, float* out, int dim) {
@@ -108,12 +128,17 @@ k_cell_table(const unsigned* __restrict__ skeys, int n, int* __restrict__ cstart
     if (i == n - 1 || skeys[i + 1] != k) cend[k] = i + 1;
 }
 
+// spos[i] = P[order[i]]: one 12-byte (3-D) or 8-byte (2-D) load and store
+// per point, so a random source record costs one memory request.
+struct alignas(4) Rec3 { float v[3]; };
+struct alignas(4) Rec2 { float v[2]; };
 __global__ void __launch_bounds__(256)
 k_gather_sorted(const float* __restrict__ P, const int* __restrict__ order, int n, int dim, float* __restrict__ spos) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int j = order[i];
-    for (int a = 0; a < dim; ++a) spos[(size_t)i * dim + a] = P[(size_t)j * dim + a];
+    const int j = order[i];
+    if (dim == 3) reinterpret_cast<Rec3*>(spos)[i] = reinterpret_cast<const Rec3*>(P)[j];
+    else reinterpret_cast<Rec2*>(spos)[i] = reinterpret_cast<const Rec2*>(P)[j];
 }
 
 // Decode a probed cell the way the reference's packed key would match a
@@ -413,7 +438,7 @@ extern "C" int jt_points_bounds(const float* P, int64_t n, int32_t dim, float* o
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, acc, 3, 0xffffffffu);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, acc + 3, 3, 0u);
     int blocks = (int)((n + 255) / 256);
-    blocks = blocks > 4096 ? 4096 : blocks;
+    blocks = blocks > 512 ? 512 : blocks;
     hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, s, P, (int)n, dim, acc);
     hipLaunchKernelGGL(k_decode_bounds, dim3(1), dim3(64), 0, s, acc, out_minmax, dim);
     JT_HIP(hipGetLastError());

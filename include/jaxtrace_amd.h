@@ -375,6 +375,23 @@ int jt_track_grid_adjoint(const jt_grid_field* field, const jt_boundary* bc, con
 int jt_grid_seeds(const float* xc, int32_t nx, const float* yc, int32_t ny, const float* zc, int32_t nz,
                   float* out, void* stream);
 
+/* Replaces random_seeds (jaxtrace/tracking/seeding.py:140-182): out (n, 3)
+ * float32 = lo + u * (hi - lo) with u = jax.random.uniform(PRNGKey(rng_seed),
+ * (n, 3)), the threefry2x32 bits in JAX's non-partitionable layout (the
+ * default before JAX 0.5).  lo, hi: HOST float[3] (the validated bounds);
+ * out: device.  n <= 1431655765 (3n words of one counter range). */
+int jt_random_seeds(int64_t n, uint64_t rng_seed, const float* lo, const float* hi, float* out, void* stream);
+
+/* The expansion of _generate_inlet_grid_particles
+ * (jaxtrace/tracking/boundary.py:1321-1441): out (n, 3) float32 with the
+ * flow-axis coordinate inlet_position and, on the two other axes in
+ * increasing order, the point indices[i] of np.meshgrid(c0, c1, 'ij')
+ * raveled (c0 (n0,), c1 (n1,): the np.linspace face axes).  indices (device
+ * int32) are chosen on the host as the reference chooses them; an index
+ * outside [0, n0*n1) gives NaN there. */
+int jt_inlet_grid_particles(const int32_t* indices, int64_t n, int32_t axis, float inlet_position, const float* c0,
+                            int32_t n0, const float* c1, int32_t n1, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

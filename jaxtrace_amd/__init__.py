@@ -19,10 +19,11 @@ from .boundary import (
     clamp_to_bounds,
     reflect_boundary,
     absorbing_boundary,
+    generate_inlet_grid_particles,
 )
 from .tracking import TrackerOptions, Trajectory, ParticleTracker, create_tracker
 from .adjoint import track_rk4
-from .seeding import uniform_grid_seeds
+from .seeding import random_seeds, uniform_grid_seeds
 from .density import HashGridNeighbors, SPHDensityEstimator, KDEEstimator, scott_bandwidth, silverman_bandwidth
 
 __version__ = "0.1.0"
@@ -34,5 +35,5 @@ __all__ = [
     "apply_periodic", "clamp_to_bounds", "reflect_boundary", "absorbing_boundary",
     "TrackerOptions", "Trajectory", "ParticleTracker", "create_tracker",
     "track_rk4", "HashGridNeighbors", "SPHDensityEstimator", "KDEEstimator", "scott_bandwidth", "silverman_bandwidth",
-    "uniform_grid_seeds",
+    "uniform_grid_seeds", "random_seeds", "generate_inlet_grid_particles",
 ]

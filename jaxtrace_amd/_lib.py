@@ -92,6 +92,9 @@ _lib = None
 _SIGS = {
     "jt_last_error": (C.c_char_p, []),
     "jt_version": (C.c_int32, []),
+    "jt_random_seeds": (C.c_int, [C.c_int64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "jt_inlet_grid_particles": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_float, C.c_void_p, C.c_int32,
+                                          C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "jt_grid_seeds": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                 C.c_void_p, C.c_void_p]),
     "jt_grid_sample": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_int64, C.c_float,

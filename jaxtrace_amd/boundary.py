@@ -239,3 +239,54 @@ def reflect_boundary(x, bounds):
 
 def absorbing_boundary(x, bounds, outside_value: float = np.nan):
     return absorbing_boundary_factory(bounds, outside_value)(x)
+
+
+def _inlet_grid_indices(n_particles: int, total: int) -> np.ndarray:
+    """The face-grid indices of boundary.py:1387-1424: n systematic samples
+    int(i * total / n) (gaps filled at the largest one if rounding merged two),
+    or every grid point then int(i * total / extra) for the extra ones."""
+    if n_particles <= total:
+        step = total / n_particles
+        idx = (np.arange(n_particles, dtype=np.float64) * step).astype(np.int64)
+        idx = np.unique(np.clip(idx, 0, total - 1))
+        while len(idx) < n_particles and len(idx) < total:
+            gaps = np.diff(np.concatenate([[0], idx, [total]]))
+            g = int(np.argmax(gaps[:-1]))
+            new = idx[0] // 2 if g == 0 else (idx[g - 1] + idx[g]) // 2
+            idx = np.unique(np.append(idx, new))
+        return idx[:n_particles]
+    extra = n_particles - total
+    step = total / extra
+    return np.concatenate([np.arange(total), (np.arange(extra, dtype=np.float64) * step).astype(np.int64)])
+
+
+def generate_inlet_grid_particles(n_particles: int, inlet_position: float, axis_idx: int, bounds,
+                                  concentrations: Dict[str, int], device=None, return_device: bool = False):
+    """boundary.py:1321-1441 (3-D bounds): n_particles on the inlet face at
+    inlet_position along axis_idx, on the face grid np.linspace(lo, hi, c) of
+    the two other axes (c from concentrations).  The index choice is the
+    reference's host arithmetic; the (n, 3) expansion is the
+    jt_inlet_grid_particles kernel.  NumPy, or a torch tensor with
+    return_device."""
+    b = _ensure_bounds_shape(bounds)
+    if axis_idx not in (0, 1, 2):
+        raise ValueError(f"axis_idx must be 0, 1 or 2, got {axis_idx}")
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    names = ["x", "y", "z"]
+    other = [i for i in range(3) if i != axis_idx]
+    res = [max(1, int(concentrations[names[i]])) for i in other]
+    c0 = np.linspace(b[0, other[0]], b[1, other[0]], res[0], dtype=np.float32)
+    c1 = np.linspace(b[0, other[1]], b[1, other[1]], res[1], dtype=np.float32)
+    n = int(n_particles)
+    out = torch.empty((max(n, 0), 3), dtype=torch.float32, device=dev)
+    if n > 0:
+        idx = torch.from_numpy(_inlet_grid_indices(n, res[0] * res[1]).astype(np.int32)).to(dev)
+        c0d, c1d = (torch.from_numpy(a).to(dev) for a in (c0, c1))
+        lib = L.load()
+        with torch.cuda.device(dev):
+            L.check(lib.jt_inlet_grid_particles(L.ptr(idx), n, int(axis_idx), C.c_float(np.float32(inlet_position)),
+                                                L.ptr(c0d), res[0], L.ptr(c1d), res[1], L.ptr(out),
+                                                L.stream_handle()), "jt_inlet_grid_particles")
+    if return_device:
+        return out
+    return out.cpu().numpy()

@@ -66,7 +66,112 @@ __global__ void k_grid_seed_lines(const float* __restrict__ xc, const float* __r
     }
 }
 
+// ---- random_seeds: jax.random.uniform bits, then lo + u (hi - lo) -----------
+// Threefry-2x32 with 20 rounds (Salmon et al., SC11): rotations 13 15 26 6 /
+// 17 29 16 24, key parity constant 0x1BD11BDA, a key injection after every
+// four rounds.
+__device__ __forceinline__ unsigned rotl32(unsigned x, int r) { return (x << r) | (x >> (32 - r)); }
+
+__device__ __forceinline__ void threefry2x32(unsigned k0, unsigned k1, unsigned& x0, unsigned& x1) {
+    const unsigned ks[3] = {k0, k1, 0x1BD11BDAu ^ k0 ^ k1};
+    const int rot[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+    x0 += ks[0];
+    x1 += ks[1];
+#pragma unroll
+    for (int r = 0; r < 20; ++r) {
+        x0 += x1;
+        x1 = rotl32(x1, rot[r & 7]);
+        x1 ^= x0;
+        if ((r & 3) == 3) {
+            const int s = (r + 1) >> 2;
+            x0 += ks[s % 3];
+            x1 += ks[(s + 1) % 3] + (unsigned)s;
+        }
+    }
+}
+
+// The m = 3n words of jax.random.uniform(key, (n, 3)) laid out as JAX's
+// non-partitionable random_bits: iota(m) (one 0 appended when m is odd) split
+// into halves x0 | x1 of h words; block i hashes (x0[i], x1[i]) and gives
+// word i and word h + i.  A word becomes the float in [1, 2) with its top 23
+// bits as mantissa, minus 1 (seeding.py:167-172 then scale to the bounds).
+__global__ void __launch_bounds__(256)
+k_random_seeds(unsigned k0, unsigned k1, unsigned long long m, unsigned long long h, float lo0, float lo1,
+               float lo2, float w0, float w1, float w2, float* __restrict__ out) {
+    const float lo[3] = {lo0, lo1, lo2}, w[3] = {w0, w1, w2};
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < h;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long j = h + i;
+        unsigned x0 = (unsigned)i, x1 = j < m ? (unsigned)j : 0u;
+        threefry2x32(k0, k1, x0, x1);
+        const float u0 = __uint_as_float((x0 >> 9) | 0x3f800000u) - 1.0f;
+        const int c0 = (int)(i % 3);
+        out[i] = lo[c0] + u0 * w[c0];
+        if (j < m) {
+            const float u1 = __uint_as_float((x1 >> 9) | 0x3f800000u) - 1.0f;
+            const int c1 = (int)(j % 3);
+            out[j] = lo[c1] + u1 * w[c1];
+        }
+    }
+}
+
+// ---- _generate_inlet_grid_particles: the inlet-face grid at given indices ----
+// out[i] = (inlet_pos on the flow axis; c0[idx / n1], c1[idx % n1] on the
+// two face axes in increasing axis order): np.meshgrid(c0, c1, 'ij') raveled.
+__global__ void __launch_bounds__(256)
+k_inlet_grid(const int* __restrict__ idx, int n, int axis, float inlet_pos, const float* __restrict__ c0, int n0,
+             const float* __restrict__ c1, int n1, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = idx[i];
+    const int a0 = axis == 0 ? 1 : 0, a1 = axis == 2 ? 1 : 2;
+    float p[3];
+    p[axis] = inlet_pos;
+    if (g >= 0 && g < n0 * n1) {
+        p[a0] = c0[g / n1];
+        p[a1] = c1[g % n1];
+    } else {                                            // an index off the face grid: NaN, no read
+        p[a0] = p[a1] = __int_as_float(0x7fc00000);
+    }
+    out[3 * (size_t)i] = p[0];
+    out[3 * (size_t)i + 1] = p[1];
+    out[3 * (size_t)i + 2] = p[2];
+}
+
 }  // namespace
+
+extern "C" int jt_random_seeds(int64_t n, uint64_t rng_seed, const float* lo, const float* hi, float* out,
+                               void* stream) {
+    JT_REQUIRE(n >= 0 && n < (1LL << 40), "n out of range");
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(lo && hi && out, "NULL pointer");
+    const unsigned long long m = (unsigned long long)n * 3u;
+    JT_REQUIRE(m <= 0xffffffffull, "at most 2^32 - 1 random words per call (n <= 1431655765)");
+    const unsigned long long h = (m + 1) / 2;
+    float w[3];
+    for (int a = 0; a < 3; ++a) w[a] = hi[a] - lo[a];                 // seeding.py:172, (hi - lo) first
+    const unsigned k0 = (unsigned)(rng_seed >> 32), k1 = (unsigned)(rng_seed & 0xffffffffull);   // PRNGKey
+    const unsigned long long want = (h + 255) / 256;
+    const unsigned blocks = (unsigned)(want < 65536ull ? want : 65536ull);
+    hipLaunchKernelGGL(k_random_seeds, dim3(blocks), dim3(256), 0, (hipStream_t)stream, k0, k1, m, h, lo[0], lo[1],
+                       lo[2], w[0], w[1], w[2], out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+extern "C" int jt_inlet_grid_particles(const int32_t* indices, int64_t n, int32_t axis, float inlet_position,
+                                       const float* c0, int32_t n0, const float* c1, int32_t n1, float* out,
+                                       void* stream) {
+    JT_REQUIRE(n >= 0 && n < (1LL << 31), "n out of range");
+    JT_REQUIRE(axis >= 0 && axis <= 2, "bad flow axis %d", axis);
+    JT_REQUIRE(n0 >= 1 && n1 >= 1 && (long long)n0 * n1 < (1LL << 31), "bad inlet-face grid %d x %d", n0, n1);
+    if (n == 0) return JT_OK;
+    JT_REQUIRE(indices && c0 && c1 && out, "NULL pointer");
+    hipLaunchKernelGGL(k_inlet_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, indices,
+                       (int)n, axis, inlet_position, c0, n0, c1, n1, out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
 
 extern "C" int jt_grid_seeds(const float* xc, int32_t nx, const float* yc, int32_t ny, const float* zc, int32_t nz,
                              float* out, void* stream) {

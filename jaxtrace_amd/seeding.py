@@ -1,7 +1,7 @@
-"""Seed generation on the device: ``uniform_grid_seeds``.
+"""Seed generation on the device: ``uniform_grid_seeds`` and ``random_seeds``.
 
 Mirrors jaxtrace/tracking/seeding.py (``_validate_bounds`` :88-137,
-``uniform_grid_seeds`` :185-247).  The three float32 coordinate arrays are
+``random_seeds`` :140-182, ``uniform_grid_seeds`` :185-247).  The three float32 coordinate arrays are
 built with np.linspace exactly as the reference builds them (a few hundred
 values); the (N, 3) meshgrid expansion, the part that grows with N, is the
 ``jt_grid_seeds`` HIP kernel writing straight into device memory, so a
@@ -10,6 +10,7 @@ values); the (N, 3) meshgrid expansion, the part that grows with N, is the
 """
 from __future__ import annotations
 
+import ctypes as C
 from typing import Tuple, Union
 
 import numpy as np
@@ -85,6 +86,29 @@ def uniform_grid_seeds(resolution, bounds, include_boundaries: bool = True, devi
     with torch.cuda.device(dev):
         L.check(lib.jt_grid_seeds(L.ptr(ax[0]), len(xc), L.ptr(ax[1]), len(yc), L.ptr(ax[2]), len(zc),
                                   L.ptr(out), L.stream_handle()), "jt_grid_seeds")
+    if return_device:
+        return out
+    return out.cpu().numpy()
+
+
+def random_seeds(n: int, bounds, rng_seed: int = 0, dtype=np.float32, device=None, return_device: bool = False):
+    """seeding.py:140-182 on the GPU: n positions lo + u * (hi - lo) with u =
+    jax.random.uniform(PRNGKey(rng_seed), (n, 3)) computed by the
+    jt_random_seeds kernel (threefry2x32, JAX's non-partitionable layout).
+    Returns (n, 3) float32, NumPy like the reference or a torch tensor on
+    ``device`` with ``return_device``."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if n <= 0:
+        z = torch.zeros((0, 3), dtype=torch.float32, device=dev)
+        return z if return_device else np.zeros((0, 3), dtype=F32)
+    b = _validate_bounds(bounds)
+    lib = L.load()
+    out = torch.empty((int(n), 3), dtype=torch.float32, device=dev)
+    lo = (C.c_float * 3)(*[float(v) for v in b[0]])
+    hi = (C.c_float * 3)(*[float(v) for v in b[1]])
+    with torch.cuda.device(dev):
+        L.check(lib.jt_random_seeds(int(n), int(rng_seed) & 0xFFFFFFFFFFFFFFFF, C.cast(lo, C.c_void_p),
+                                    C.cast(hi, C.c_void_p), L.ptr(out), L.stream_handle()), "jt_random_seeds")
     if return_device:
         return out
     return out.cpu().numpy()

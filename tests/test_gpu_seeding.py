@@ -35,3 +35,33 @@ def test_grid_seeds_full_size(gpu):
     d = jt.uniform_grid_seeds(256, [[0, 0, 0], [1, 1, 1]], return_device=True)
     want = O.uniform_grid_seeds(256, [[0, 0, 0], [1, 1, 1]])
     assert torch.equal(d.cpu(), torch.from_numpy(want))
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 42), (1000, 7), (333_333, 2 ** 33 + 5), (1 << 22, 3)])
+def test_random_seeds_bitwise(gpu, n, seed):
+    """random_seeds (seeding.py:140-182): the jt_random_seeds kernel's
+    threefry words, uniforms and scaled positions equal the oracle's bitwise
+    (odd and even word counts, a seed above 2^32)."""
+    b = [[-1.0, 0.0, 2.0], [1.0, 0.5, 6.0]]
+    got = jt.random_seeds(n, b, rng_seed=seed)
+    np.testing.assert_array_equal(got, O.random_seeds(n, b, rng_seed=seed))
+
+
+def test_random_seeds_edges(gpu):
+    assert jt.random_seeds(0, [[0, 0, 0], [1, 1, 1]]).shape == (0, 3)
+    t = jt.random_seeds(10, [0, 1, 0, 2, 0, 3], rng_seed=1, return_device=True)
+    assert t.is_cuda and t.shape == (10, 3)
+    with pytest.raises(ValueError):
+        jt.random_seeds(4, [[0, 0, 0], [0, 1, 1]])
+
+
+@pytest.mark.parametrize("n,axis,conc", [(7, 1, {"x": 5, "y": 3, "z": 5}), (4096, 0, {"x": 64, "y": 64, "z": 64}),
+                                         (5000, 2, {"x": 40, "y": 50, "z": 1}), (1, 0, {"x": 1, "y": 1, "z": 1}),
+                                         (100, 2, {"x": 7, "y": 9, "z": 3})])
+def test_inlet_grid_particles_bitwise(gpu, n, axis, conc):
+    """_generate_inlet_grid_particles (boundary.py:1321-1441): fewer, equal
+    and more particles than inlet-face points, each flow axis."""
+    import jaxtrace_amd as jt
+    b = [[0.0, -1.0, 0.5], [2.0, 1.0, 1.5]]
+    got = jt.generate_inlet_grid_particles(n, 0.125, axis, b, conc)
+    np.testing.assert_array_equal(got, O.inlet_grid_particles(n, 0.125, axis, b, conc))

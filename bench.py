@@ -45,6 +45,45 @@ def kernel_source_sha(files) -> str:
     return h.hexdigest()
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "kernel_counters.json")
+
+
+def counter_entry(key):
+    """rocprofv3 counter summary of one kernel from profiles/kernel_counters.json,
+    if it was taken from the current sources of that kernel (sha256 match)."""
+    try:
+        with open(COUNTERS) as fh:
+            prof = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    e = prof.get("kernels", {}).get(key)
+    if not e or e.get("source_sha256") != kernel_source_sha(e.get("source_files", [])):
+        return None
+    return e
+
+
+def hbm_roofline(key, units, ms, **extra):
+    """HBM roofline of one kernel launch from measured traffic: the kernel's
+    HBM bytes per unit (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a
+    plain copy's readings) x the units the launch processed, over the launch
+    time `ms`.  Counters are per unit, so they hold for any launch length."""
+    e = counter_entry(key)
+    out = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None}
+    if e is None:
+        out["note"] = f"no counter profile of {key} for the current kernel sources"
+    else:
+        traffic = e["hbm_bytes_per_unit"] * units
+        ach = traffic / (ms * 1e-3) / 1e9
+        out.update(achieved=ach, frac=ach / HBM_PEAK_GBS, traffic=traffic,
+                   hbm_bytes_per_unit=e["hbm_bytes_per_unit"], unit_of_work=e["unit"],
+                   l2_hit_rate=e.get("l2_hit_rate"), counters=f"profiles/kernel_counters.json#{key}")
+        for k in ("ta_busy", "td_busy", "valu_busy", "valu_insts_per_wave_unit", "binding"):
+            if k in e:
+                out[k] = e[k]
+    out.update(extra)
+    return out
+
+
 def algorithmic_bytes_per_particle_step(elem_bytes: int) -> int:
     return 24 + 4 * 2 * 8 * 3 * elem_bytes
 
@@ -63,6 +102,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
+    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "hashgrid", "nearest", "seeding"],
+                    help="run one section only and print its JSON (profiler runs)")
+    ap.add_argument("--adjoint-reps", type=int, default=3,
+                    help="adjoint repetitions per storage; the last one is reported")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known HBM traffic) for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -222,7 +265,7 @@ def trajectory_error_vs_fp64(torch, jt, data, times, grid, x0, bc, n_steps=1000,
     return out
 
 
-def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32):
+def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32, reps=3):
     """Adjoint workload: 128^3 x 8 random Fourier field (as above, t in [0,1]),
     2^20 particles (seed 7), RK4 dt = 2e-3, 500 steps, periodic box,
     L = mean |x_T - x*|^2 (x* seed 8), checkpoints every 32 steps.  Returns
@@ -236,7 +279,7 @@ def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32):
     out = {}
     for storage in ("float32", "bfloat16"):
         res = {}
-        for rep in range(3):                      # the first two repetitions warm up
+        for rep in range(reps):                   # the earlier repetitions warm up
             U = data.clone().requires_grad_(True)
             X = x0.clone().requires_grad_(True)
             torch.cuda.synchronize()
@@ -269,8 +312,11 @@ def bench_integrators(torch, L, lib, field, bc, x0, args, steps=200, warmup=25):
     """RK2 and Euler (rk2.py:117-180, euler.py:111-157) through the same fused
     kernel on the headline workload: FP32 field, 2^24 particles, mixed
     boundary, launches of `steps_per_launch` steps with the cell sort.  Timed:
-    `steps` steps after `warmup`.  Byte model: 24 B of state + per stage 2
-    snapshots x 8 corners x 12 B (RK2 408 B, Euler 216 B per particle-step)."""
+    `steps` steps after `warmup`.  Roofline: measured HBM bytes per
+    particle-step (profiles/kernel_counters.json); the gather model (24 B of
+    state + per stage 2 snapshots x 8 corners x 12 B: RK2 408 B, Euler 216 B
+    per particle-step) is reported beside it: registers, L1 and L2 serve most
+    of those gathers."""
     n = x0.shape[0]
     spl = launch_steps(args, args.sort_every)
     out = {}
@@ -280,11 +326,11 @@ def bench_integrators(torch, L, lib, field, bc, x0, args, steps=200, warmup=25):
         full = [r for r in durs if r[2] == spl] or durs
         launch_ms = float(np.mean([r[1] for r in full]))
         bpps = 24 + stages * 2 * 8 * 12
-        ach = bpps * n * full[0][2] / (launch_ms * 1e-3) / 1e9
+        gather = bpps * n * full[0][2] / (launch_ms * 1e-3) / 1e9
         out[name] = {"value": n * steps / wall, "unit": "particle-steps/s", "launch_ms": launch_ms,
                      "steps_per_launch": full[0][2],
-                     "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": ach / HBM_PEAK_GBS, "bytes_per_particle_step": bpps}}
+                     "roofline": hbm_roofline(f"k_track_{name}_f32", n * full[0][2], launch_ms,
+                                              gather_model={"bytes_per_particle_step": bpps, "GBs": gather})}
     out["workload"] = f"the headline FP32 workload, {steps} steps after {warmup}"
     return out
 
@@ -384,10 +430,12 @@ def bench_octree(torch, jt, W, L, lib, dev, steps=20):
     nodes), 8 snapshots of the Poiseuille-plus-vortex field, 2^22 uniform
     particles (seed 5), RK4 dt = 1e-3 with the continuous inlet at x = 0
     (64 x 64 grid replacement), particles Hilbert-sorted every 10 steps.
-    Timed: `steps` steps after a warm-up run.  Roofline: the point-location
-    byte model 24 + 4 [L 29 + c 28 + a 64 + 96] B per particle-step, with L,
-    c, a the mean levels descended, leaf candidates scanned and box hits
-    counted by jt_octree_locate_stats at the initial positions."""
+    Timed: `steps` steps after a warm-up run (the whole call, cell sorts
+    included).  Roofline: measured HBM bytes per particle-step
+    (profiles/kernel_counters.json); beside it the point-location byte model
+    24 + 4 [L 29 + c 28 + a 64 + 96] B per particle-step, with L, c, a the
+    mean levels descended, leaf candidates scanned and box hits counted by
+    jt_octree_locate_stats at the initial positions."""
     pts, conn = W.kuhn_mesh(128, 0.2, 4)
     times = (np.arange(8) / 14.0).astype(np.float32)
     data = W.poiseuille_vortex(pts, times)
@@ -415,16 +463,15 @@ def bench_octree(torch, jt, W, L, lib, dev, steps=20):
                                        L.stream_handle()), "jt_octree_locate_stats")
     Lm, cm, am = (float(v) for v in stats.double().mean(0).cpu())
     bpps = 24 + 4 * (Lm * 29 + cm * 28 + am * 64 + 96)
-    achieved = bpps * rate / 1e9
     return {"value": rate, "unit": "particle-steps/s", "ms": ms, "steps": steps, "particles": n,
             "octree_build_s": build_s,
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_particle_step": bpps,
-                         "levels": Lm, "candidates_scanned": cm, "box_hits": am},
+            "roofline": hbm_roofline("k_oct_track", n * steps, ms,
+                                     locate_model={"bytes_per_particle_step": bpps, "GBs": bpps * rate / 1e9,
+                                                   "levels": Lm, "candidates_scanned": cm, "box_hits": am}),
             "workload": "128^3 x 6 jittered tets, 8 snapshots, 2^22 particles, RK4 dt=1e-3, continuous inlet"}
 
 
-def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
+def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde")):
     """Density workload: 2^24 particles from the 64-component Gaussian mixture
     (seed 6).
     SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
@@ -433,10 +480,12 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     (jt_sph_eval); the hash-grid construction (HashGridNeighbors on device
     points: bounds with their read-back, keys, radix sort, cell table, sorted
     positions; byte model 36 B per point); and the estimator's whole setup.
-    Roofline: 9 x 8 + C_q/3 x 16 + 4 B per query: C_q is the reference's
-    candidate count (27 probes, from jt_hashgrid_query_stats on a sample of
-    queries), of which x32 keys make 9 distinct cells, each read once; queries
-    over 512 neighbours are redone by the top-K pass and are not counted.
+    Roofline: measured HBM bytes per query of the jt_sph_eval call (its two
+    kernels) and per point of the build (profiles/kernel_counters.json);
+    beside them the candidate model 9 x 8 + C_q/3 x 16 + 4 B per query (C_q
+    the reference's candidate count over its 27 probes, from
+    jt_hashgrid_query_stats on a sample of queries; x32 keys make 9 distinct
+    cells, each read once) and the 36 B per point build model.
     KDE: Scott bandwidth, evaluate_3d grid of kde_res^3 points, cut-off
     KDE_CUTOFF_SIGMAS h, the cut-off the sampled GPU parity test
     (tests/test_gpu_density.py::test_kde_cutoff_tiles_sampled) holds to 1e-5
@@ -444,11 +493,12 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     (2 pi)^(-3/2) exp(-c^2/2), so their mean is too, and the line reports
     that bound relative to the density floor (1e-3 of the maximum); timed: the
     jt_kde_grid_cutoff launch.  Rate: pairs within the cut-off per second
-    (counted by jt_hashgrid_query_stats on a sample).  Roofline: the FP32
-    MFMA peak against 2 flops per (grid point, particle) pair of each 16^3
-    tile and the particles within R of the tile's box (the products the
-    kernel's matrix cores evaluate, estimated from 512 sampled tiles), four
-    bfloat16 products each, against the dense BF16 MFMA peak."""
+    (counted by jt_hashgrid_query_stats on a sample).  Roofline: the dense
+    BF16 MFMA peak against 8 flops (four bfloat16 products of the split
+    float32 operands) per (grid point, particle) pair of each 16^3 tile and
+    the particles within R of the tile's box, the products the kernel's
+    matrix cores evaluate (estimated from 512 sampled tiles).
+    parts: which of 'hashgrid', 'sph', 'kde' to run."""
     F = np.float32
     P = W.gaussian_mixture(1 << 24, seed=6)
     Pd = torch.from_numpy(P).to(dev)
@@ -459,6 +509,17 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     del X, Y, Z
     M = int(Qd.shape[0])
     out = {}
+    if "hashgrid" in parts and "sph" not in parts:
+        jt.HashGridNeighbors(Pd, h, 64, "x32", dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        jt.HashGridNeighbors(Pd, h, 64, "x32", dev)
+        torch.cuda.synchronize()
+        hg_ms = (time.perf_counter() - t0) * 1e3
+        out["hashgrid_build"] = {"ms": hg_ms, "roofline": hbm_roofline("hashgrid_build", len(P), hg_ms)}
+        return out
+    if "sph" not in parts:
+        return _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
     # the hash grid alone (x32 keys, cell h): bounds, keys, radix sort, cell
     # table, sorted positions; after one warm-up build
     jt.HashGridNeighbors(Pd, h, 64, "x32", dev)
@@ -497,18 +558,27 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
     # x32 keys: the direct pass reads the 9 distinct probe cells of the
     # 27-probe stencil once (the other 18 repeat them)
     bpq = 9 * 8 + cq / 3 * 16 + 4
-    ach = bpq * M / (sph_ms * 1e-3) / 1e9
     out["sph"] = {"value": M / (sph_ms * 1e-3), "unit": "queries/s", "ms": sph_ms, "queries": M,
                   "estimator_setup_ms": build_ms,
-                  "hashgrid_build": {"ms": hg_ms, "bytes_per_point": 36,
-                                     "roofline": {"bound": "hbm", "achieved": 36.0 * len(P) / (hg_ms * 1e-3) / 1e9,
-                                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                                  "frac": 36.0 * len(P) / (hg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
-                  "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": ach / HBM_PEAK_GBS, "bytes_per_query": bpq, "mean_candidates": cq,
-                               "mean_within_radius": float(st[:, 1].double().mean())},
+                  "hashgrid_build": {"ms": hg_ms,
+                                     "roofline": hbm_roofline("hashgrid_build", len(P), hg_ms,
+                                                              model={"bytes_per_point": 36,
+                                                                     "GBs": 36.0 * len(P) / (hg_ms * 1e-3) / 1e9})},
+                  "roofline": hbm_roofline("sph_eval", M, sph_ms,
+                                           candidate_model={"bytes_per_query": bpq, "GBs": bpq * M / (sph_ms * 1e-3) / 1e9,
+                                                            "mean_candidates": cq,
+                                                            "mean_within_radius": float(st[:, 1].double().mean())}),
                   "workload": "2^24 Gaussian-mixture particles, 256^3 query grid, cubic spline h=2/256, x32 keys"}
     del est, rho
+    if "kde" not in parts:
+        return out
+    return _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
+
+
+def _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out):
+    """The KDE half of bench_density."""
+    F = np.float32
+    e0, e1 = _events(torch)
     kde = jt.KDEEstimator(Pd, bandwidth_rule="scott", normalize=True, resolution=kde_res,
                           bounds=((0, 1), (0, 1), (0, 1)), cutoff_sigmas=KDE_CUTOFF_SIGMAS)
     x, y, z = kde._axes_3d()
@@ -566,6 +636,7 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16):
                   "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
                                "frac": ach / MFMA_BF16_TFLOPS, "tile_pairs": tile_pairs,
                                "tile_pairs_sample": "512 random 16^3 tiles (seed 14), scaled to all tiles",
+                               "hbm": hbm_roofline("k_kde_tile_mfma", 1, kde_ms),
                                "flops_per_tile_pair": 8,
                                "note": "dense bfloat16 MFMA peak; each float32 operand is split into bfloat16 hi/lo "
                                        "halves and the four products are summed in float32"},
@@ -601,6 +672,16 @@ def main():
     lib = L.load()
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(0)
+    if args.only and args.only != "track":
+        fn = {"adjoint": lambda: bench_adjoint(torch, jt, W, dev, reps=args.adjoint_reps),
+              "octree": lambda: bench_octree(torch, jt, W, L, lib, dev),
+              "sph": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("sph",)),
+              "kde": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde",)),
+              "hashgrid": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("hashgrid",)),
+              "nearest": lambda: bench_nearest(torch, jt, W, L, lib, dev),
+              "seeding": lambda: bench_seeding(torch, jt, W, L, lib, dev)}[args.only]
+        print(json.dumps({"only": args.only, "result": fn()}), flush=True)
+        return
 
     t_gen = time.perf_counter()
     data, times, grid = W.random_fourier(n=GRID_N, n_snap=N_SNAP, t_end=2.0, seed=2, device=dev)
@@ -622,7 +703,7 @@ def main():
     avg_launch_ms = float(np.mean([r[1] for r in full]))
     steps_per = full[0][2]
     bpps32 = algorithmic_bytes_per_particle_step(4)
-    achieved = bpps32 * n * steps_per / (avg_launch_ms * 1e-3) / 1e9
+    gather32 = bpps32 * n * steps_per / (avg_launch_ms * 1e-3) / 1e9
 
     result = {
         "metric": "RK4 particle-steps/s at 16M particles, FP32 vs BF16 field; adjoint/fwd ratio",
@@ -642,31 +723,13 @@ def main():
                    "particles": n, "grid": [GRID_N] * 3, "snapshots": N_SNAP, "dt": DT,
                    "steps_per_launch": spl, "cell_sort_every": args.sort_every,
                    "snapshot_layout": "z-halo, 4 components per node"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_track<RK4,f32x4,z-halo,fast path>", "launch_ms": avg_launch_ms,
-                     "call_ms_with_cell_sort": avg_call_ms,
-                     "particle_steps_per_launch": n * steps_per,
-                     "bytes_per_particle_step": bpps32},
+        "roofline": hbm_roofline(
+            "k_track_rk4_f32", n * steps_per, avg_launch_ms, kernel="k_track<RK4,f32x4,z-halo,fast path>",
+            launch_ms=avg_launch_ms, call_ms_with_cell_sort=avg_call_ms, particle_steps_per_launch=n * steps_per,
+            gather_model={"bytes_per_particle_step": bpps32, "GBs": gather32,
+                          "note": "24 B of state + 4 stages x 2 snapshots x 8 corners x 12 B: the corner "
+                                  "gathers the algorithm makes, most of them served by registers, L1 and L2"}),
     }
-    prof = os.path.join(ROOT, "profiles", "track_grid_rk4_f32_traffic.json")
-    if os.path.exists(prof):
-        try:
-            tr = json.load(open(prof))
-            # counters hold for one launch shape of one kernel build only
-            if (tr.get("particles") == n and tr.get("steps_per_launch") == steps_per
-                    and tr.get("source_sha256") == kernel_source_sha(tr.get("source_files", []))):
-                f32c = tr.get("float32_25_step_launches", {})
-                result["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
-                result["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
-                result["roofline"]["measured"] = {
-                    k: f32c.get(k) for k in ("l2_hit_rate", "ta_busy_fraction", "td_busy_fraction",
-                                             "valu_insts_per_wave_step", "vmem_read_insts_per_wave_step")}
-                result["roofline"]["measured"]["note"] = (
-                    "rocprofv3 counters of this kernel build at this launch shape: the kernel is bound by its "
-                    "vector-memory address/data units (TA/TD busy), not by HBM bandwidth")
-        except Exception:
-            pass
 
     if not args.no_bf16:
         field16 = field32.with_storage("bfloat16", layout="halo")
@@ -674,12 +737,18 @@ def main():
         rate16 = n * args.steps / wall16
         full16 = [r for r in durs16 if r[2] == spl] or durs16
         avg16 = float(np.mean([r[1] for r in full16]))
-        ach16 = algorithmic_bytes_per_particle_step(2) * n * full16[0][2] / (avg16 * 1e-3) / 1e9
+        g16 = algorithmic_bytes_per_particle_step(2) * n * full16[0][2] / (avg16 * 1e-3) / 1e9
         result["bf16"] = {"value": rate16, "speedup_vs_f32": rate16 / rate32, "launch_ms": avg16,
-                          "roofline_achieved_GBs": ach16, "roofline_frac": ach16 / HBM_PEAK_GBS,
-                          "bytes_per_particle_step": algorithmic_bytes_per_particle_step(2)}
+                          "roofline": hbm_roofline("k_track_rk4_bf16", n * full16[0][2], avg16,
+                                                   gather_model={"bytes_per_particle_step":
+                                                                 algorithmic_bytes_per_particle_step(2),
+                                                                 "GBs": g16})}
         del field16
 
+    if args.only == "track":
+        result["rk2_euler"] = bench_integrators(torch, L, lib, field32, bc, x0, args, steps=50, warmup=25)
+        print(json.dumps(result), flush=True)
+        return
     if not args.no_extras:
         try:
             result["rk2_euler"] = bench_integrators(torch, L, lib, field32, bc, x0, args)

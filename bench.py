@@ -77,7 +77,7 @@ def hbm_roofline(key, units, ms, **extra):
         out.update(achieved=ach, frac=ach / HBM_PEAK_GBS, traffic=traffic,
                    hbm_bytes_per_unit=e["hbm_bytes_per_unit"], unit_of_work=e["unit"],
                    l2_hit_rate=e.get("l2_hit_rate"), counters=f"profiles/kernel_counters.json#{key}")
-        for k in ("ta_busy", "td_busy", "valu_busy", "valu_insts_per_wave_unit", "binding"):
+        for k in ("ta_busy", "td_busy", "valu_insts_per_wave", "binding"):
             if k in e:
                 out[k] = e[k]
     out.update(extra)

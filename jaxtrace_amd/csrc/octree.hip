@@ -56,6 +56,12 @@ __device__ __forceinline__ void blended_node(const OctDev& o, const Blend& bl, f
 // Candidates fetched per round of the leaf scan: 1 measured fastest on the
 // 128^3 x 6 tet workload (3.4e9 particle-steps/s; 2: 3.3e9, 3: 2.5e9,
 // 4: 2.0e9, the larger batches paying in registers and spills).
+// Candidates per load round of the leaf scan.  Batches of 2-4 measured
+// slower (they pay in registers); the one-candidate rounds keep the batched
+// form, passes of predicated loads per dependency level, because the same
+// scan written as a plain loop with early continues compiled to slower code
+// (16.5 against 12.0 ms per 10-step launch of 2^22 particles).
+constexpr int OCT_BATCH = 1;
 
 // point_in_tetrahedron_fast (:218-250): Cramer barycentrics of q in the
 // tetrahedron v[0..3]; true when all lie within the reference's tolerances.
@@ -163,31 +169,47 @@ __device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q
     const int* cand = o.nelem + (size_t)node * o.max_elems;
     const int cnt = o.ncount[node];
     if constexpr (!STATS) {
-        // :309-360, first hit wins, candidates in stored order.  Each
-        // candidate's loads are issued per dependency level (id, box bounds,
-        // connectivity, vertices); batching 2-4 candidates per round was
-        // measured slower (it pays in registers).
-        for (int k = 0; k < cnt; ++k) {
-            const int e = cand[k];
-            if (e < 0) continue;                            // :343, id >= 0
-            float eb[6];
+        // :309-360, first hit wins.  Candidates are fetched OCT_BATCH at a
+        // time: all their element ids, then all box bounds, then all
+        // connectivity, then all vertices are issued before any test (one
+        // round trip per dependency level), then tested in stored order.
+        for (int k0 = 0; k0 < cnt; k0 += OCT_BATCH) {
+            int e[OCT_BATCH];
+            float eb[OCT_BATCH][6];
+            bool inb[OCT_BATCH];
 #pragma unroll
-            for (int a = 0; a < 6; ++a) eb[a] = o.ebounds[(size_t)e * 6 + a];
-            if (!(q[0] >= eb[0] && q[1] >= eb[1] && q[2] >= eb[2] && q[0] <= eb[3] && q[1] <= eb[4] &&
-                  q[2] <= eb[5]))
-                continue;
-            int ni[4];
+            for (int j = 0; j < OCT_BATCH; ++j) e[j] = (k0 + j < cnt) ? cand[k0 + j] : -1;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) ni[c] = o.conn[(size_t)e * 4 + c];
-            float v[4][3];
+            for (int j = 0; j < OCT_BATCH; ++j)
+                if (e[j] >= 0)
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+                    for (int a = 0; a < 6; ++a) eb[j][a] = o.ebounds[(size_t)e[j] * 6 + a];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) v[c][a] = o.points[(size_t)ni[c] * 3 + a];
-            float b[4];
-            if (tet_bary(v, q, b)) {
-                tet_value(o, bl, ni, b, out);
-                return 1;
+            for (int j = 0; j < OCT_BATCH; ++j)
+                inb[j] = e[j] >= 0 && q[0] >= eb[j][0] && q[1] >= eb[j][1] && q[2] >= eb[j][2] &&
+                         q[0] <= eb[j][3] && q[1] <= eb[j][4] && q[2] <= eb[j][5];
+            int ni[OCT_BATCH][4];
+#pragma unroll
+            for (int j = 0; j < OCT_BATCH; ++j)
+                if (inb[j])
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) ni[j][c] = o.conn[(size_t)e[j] * 4 + c];
+            float v[OCT_BATCH][4][3];
+#pragma unroll
+            for (int j = 0; j < OCT_BATCH; ++j)
+                if (inb[j])
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) v[j][c][a] = o.points[(size_t)ni[j][c] * 3 + a];
+#pragma unroll
+            for (int j = 0; j < OCT_BATCH; ++j) {
+                if (!inb[j]) continue;
+                float b[4];
+                if (tet_bary(v[j], q, b)) {
+                    tet_value(o, bl, ni[j], b, out);
+                    return 1;
+                }
             }
         }
         return oct_fallback(o, bl, oma, cand, q, out);

@@ -291,6 +291,30 @@ int  jt_octree_export(const jt_octree* t, float* nodes_min, float* nodes_max, in
                       float* centroids);
 void jt_octree_free(jt_octree* t);
 
+/* build_octree_mesh_optimized (octree_fem_interpolator_optimized.py:41-215)
+ * on the GPU, arrays bit-identical to jt_octree_build's.  points (n_points, 3)
+ * f32 and conn (n_elems, 4) int32 are DEVICE arrays; element_bounds
+ * (n_elems, 2, 3) and centroids (n_elems, 3) f32 are written.  The tree (at
+ * most max_nodes nodes; JT_ENOMEM naming max_nodes beyond it) stays in the
+ * workspace (jt_octree_build_device_workspace_size(n_elems, max_nodes)
+ * bytes) for jt_octree_export_device; *n_nodes, *max_elems (the widest
+ * element row) and *tree_depth size its outputs.  The tree's shape depends
+ * on the data, so this call synchronises `stream` (it reads two counts per
+ * level). */
+size_t jt_octree_build_device_workspace_size(int64_t n_elems, int64_t max_nodes);
+int jt_octree_build_device(const float* points, int64_t n_points, const int32_t* conn, int64_t n_elems,
+                           int32_t max_per_leaf, int32_t max_depth, int64_t max_nodes, float* element_bounds,
+                           float* centroids, void* ws, size_t ws_bytes, int64_t* n_nodes, int32_t* max_elems,
+                           int32_t* tree_depth, void* stream);
+/* The flattened arrays of the tree jt_octree_build_device left in ws (same
+ * n_elems, max_nodes), in depth-first pre-order, into DEVICE buffers:
+ * nodes_min/max (n_nodes, 3) f32, nodes_elements (n_nodes, max_elems) int32
+ * (-1 pad), counts (n_nodes) int32, children (n_nodes, 8) int32, is_leaf
+ * (n_nodes) u8.  Asynchronous on stream. */
+int jt_octree_export_device(const void* ws, size_t ws_bytes, int64_t n_elems, int64_t max_nodes, int64_t n_nodes,
+                            int32_t max_elems, float* nodes_min, float* nodes_max, int32_t* nodes_elements,
+                            int32_t* counts, int32_t* children, uint8_t* is_leaf, void* stream);
+
 /* Device-resident octree field: the exported arrays plus nodal snapshots. */
 typedef struct jt_octree_field {
     const float*   points;            /* (n_points, 3)             */

@@ -230,14 +230,15 @@ class StructuredTimeSeriesField(_GridSeries):
 class OctreeFEMTimeSeriesFieldOptimized:
     """octree_fem_time_series_optimized.py:17-127: nodal snapshots on a
     tetrahedral mesh, located through the element octree of
-    build_octree_mesh_optimized (built on the host by the library, arrays
-    identical to the reference's) and interpolated with P1 shape functions."""
+    build_octree_mesh_optimized (built on the GPU by default, build_on='host'
+    for the library's C++ host build; arrays identical to the reference's
+    either way) and interpolated with P1 shape functions."""
 
     kind = "octree"
 
     def __init__(self, data, times, positions, connectivity, interpolation: str = "linear",
                  extrapolation: str = "constant", _source_info: Optional[dict] = None,
-                 max_elements_per_leaf: int = 32, max_depth: int = 12, device=None):
+                 max_elements_per_leaf: int = 32, max_depth: int = 12, device=None, build_on: str = "device"):
         lib = L.load()
         self.device = torch.device(device) if device is not None else L.device()
         data = np.ascontiguousarray(np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data, F32))
@@ -266,32 +267,20 @@ class OctreeFEMTimeSeriesFieldOptimized:
         self.t_min, self.t_max = float(times[0]), float(times[-1])
         self._source_info = _source_info or {}
         self.storage = "float32"
-        h = C.c_void_p()
-        L.check(lib.jt_octree_build(pts.ctypes.data, N, conn.ctypes.data, conn.shape[0],
-                                    int(max_elements_per_leaf), int(max_depth), C.byref(h)), "jt_octree_build")
-        try:
-            nn, me, dep = C.c_int64(), C.c_int32(), C.c_int32()
-            L.check(lib.jt_octree_info(h, C.byref(nn), C.byref(me), C.byref(dep)), "jt_octree_info")
-            n_nodes, max_elems = int(nn.value), int(me.value)
-            M = conn.shape[0]
-            tree = {
-                "nodes_min": np.empty((n_nodes, 3), F32), "nodes_max": np.empty((n_nodes, 3), F32),
-                "nodes_elements": np.empty((n_nodes, max_elems), np.int32),
-                "nodes_elem_counts": np.empty(n_nodes, np.int32),
-                "nodes_children": np.empty((n_nodes, 8), np.int32),
-                "nodes_is_leaf": np.empty(n_nodes, np.uint8),
-                "element_bounds": np.empty((M, 2, 3), F32), "element_centroids": np.empty((M, 3), F32),
-            }
-            L.check(lib.jt_octree_export(h, *[tree[k].ctypes.data for k in (
-                "nodes_min", "nodes_max", "nodes_elements", "nodes_elem_counts", "nodes_children",
-                "nodes_is_leaf", "element_bounds", "element_centroids")]), "jt_octree_export")
-        finally:
-            lib.jt_octree_free(h)
-        self.octree = tree
-        self.tree_depth = int(dep.value)
         self.max_depth = int(max_depth)
         dev = self.device
-        self._dev = {k: torch.from_numpy(v).to(dev) for k, v in tree.items() if k != "element_centroids"}
+        M = conn.shape[0]
+        if build_on == "device":
+            self._dev = self._build_on_device(pts, conn, int(max_elements_per_leaf), int(max_depth))
+            self._octree_host = None
+        elif build_on == "host":
+            tree = self._build_on_host(pts, conn, int(max_elements_per_leaf), int(max_depth))
+            self._octree_host = tree
+            self._dev = {k: torch.from_numpy(v).to(dev) for k, v in tree.items()}
+        else:
+            raise ValueError(f"build_on must be 'device' or 'host', got {build_on!r}")
+        n_nodes, max_elems = self._dev["nodes_elements"].shape
+        self._dev.pop("element_centroids", None)
         self._dev["points"] = torch.from_numpy(pts).to(dev)
         self._dev["conn"] = torch.from_numpy(conn).to(dev)
         self._dev["data"] = torch.from_numpy(data).to(dev)
@@ -306,6 +295,89 @@ class OctreeFEMTimeSeriesFieldOptimized:
         d.time_interp = L.JT_TIME.get(interp, 0)
         d.time_extrap = L.JT_EXTRAP[extra]
         self._desc = d
+
+    def _build_on_host(self, pts, conn, max_per_leaf, max_depth):
+        """jt_octree_build: the C++ host build, exported as NumPy arrays."""
+        lib = L.load()
+        h = C.c_void_p()
+        L.check(lib.jt_octree_build(pts.ctypes.data, pts.shape[0], conn.ctypes.data, conn.shape[0], max_per_leaf,
+                                    max_depth, C.byref(h)), "jt_octree_build")
+        try:
+            nn, me, dep = C.c_int64(), C.c_int32(), C.c_int32()
+            L.check(lib.jt_octree_info(h, C.byref(nn), C.byref(me), C.byref(dep)), "jt_octree_info")
+            n_nodes, max_elems, M = int(nn.value), int(me.value), conn.shape[0]
+            tree = {
+                "nodes_min": np.empty((n_nodes, 3), F32), "nodes_max": np.empty((n_nodes, 3), F32),
+                "nodes_elements": np.empty((n_nodes, max_elems), np.int32),
+                "nodes_elem_counts": np.empty(n_nodes, np.int32),
+                "nodes_children": np.empty((n_nodes, 8), np.int32),
+                "nodes_is_leaf": np.empty(n_nodes, np.uint8),
+                "element_bounds": np.empty((M, 2, 3), F32), "element_centroids": np.empty((M, 3), F32),
+            }
+            L.check(lib.jt_octree_export(h, *[tree[k].ctypes.data for k in (
+                "nodes_min", "nodes_max", "nodes_elements", "nodes_elem_counts", "nodes_children",
+                "nodes_is_leaf", "element_bounds", "element_centroids")]), "jt_octree_export")
+        finally:
+            lib.jt_octree_free(h)
+        self.tree_depth = int(dep.value)
+        return tree
+
+    def _build_on_device(self, pts, conn, max_per_leaf, max_depth):
+        """jt_octree_build_device: the level-by-level GPU build; the arrays
+        stay on the device.  The node capacity starts at 4 M / max_per_leaf
+        and grows if the tree needs more."""
+        lib = L.load()
+        dev = self.device
+        M = conn.shape[0]
+        pts_d = torch.from_numpy(pts).to(dev)
+        conn_d = torch.from_numpy(conn).to(dev)
+        eb = torch.empty((M, 2, 3), dtype=torch.float32, device=dev)
+        cent = torch.empty((M, 3), dtype=torch.float32, device=dev)
+        cap = max(1024, 4 * M // max(max_per_leaf, 1) + 64)
+        nn, me, dep = C.c_int64(), C.c_int32(), C.c_int32()
+        while True:
+            wsb = int(lib.jt_octree_build_device_workspace_size(M, cap))
+            if wsb == 0:
+                raise MemoryError(f"octree node capacity {cap} out of range")
+            ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+            rc = lib.jt_octree_build_device(L.ptr(pts_d), pts.shape[0], L.ptr(conn_d), M, max_per_leaf, max_depth,
+                                            cap, L.ptr(eb), L.ptr(cent), L.ptr(ws), wsb, C.byref(nn), C.byref(me),
+                                            C.byref(dep), L.stream_handle())
+            if rc == L.JT_ENOMEM and b"max_nodes" in lib.jt_last_error():
+                cap *= 4
+                continue
+            L.check(rc, "jt_octree_build_device")
+            break
+        n_nodes, max_elems = int(nn.value), int(me.value)
+        out = {"nodes_min": torch.empty((n_nodes, 3), dtype=torch.float32, device=dev),
+               "nodes_max": torch.empty((n_nodes, 3), dtype=torch.float32, device=dev),
+               "nodes_elements": torch.empty((n_nodes, max_elems), dtype=torch.int32, device=dev),
+               "nodes_elem_counts": torch.empty(n_nodes, dtype=torch.int32, device=dev),
+               "nodes_children": torch.empty((n_nodes, 8), dtype=torch.int32, device=dev),
+               "nodes_is_leaf": torch.empty(n_nodes, dtype=torch.uint8, device=dev)}
+        L.check(lib.jt_octree_export_device(L.ptr(ws), wsb, M, cap, n_nodes, max_elems,
+                                            *[L.ptr(out[k]) for k in ("nodes_min", "nodes_max", "nodes_elements",
+                                                                      "nodes_elem_counts", "nodes_children",
+                                                                      "nodes_is_leaf")], L.stream_handle()),
+                "jt_octree_export_device")
+        out["element_bounds"] = eb
+        out["element_centroids"] = cent
+        self._centroids_dev = cent
+        self.tree_depth = int(dep.value)
+        del ws
+        return out
+
+    @property
+    def octree(self):
+        """The flattened octree arrays (build_octree_mesh_optimized's, :178-196)
+        as NumPy arrays (copied from the device on first use)."""
+        if self._octree_host is None:
+            tree = {k: v.cpu().numpy() for k, v in self._dev.items()
+                    if k in ("nodes_min", "nodes_max", "nodes_elements", "nodes_elem_counts", "nodes_children",
+                             "nodes_is_leaf", "element_bounds")}
+            tree["element_centroids"] = self._centroids_dev.cpu().numpy()
+            self._octree_host = tree
+        return self._octree_host
 
     def descriptor(self):
         if self.interpolation == "cubic":

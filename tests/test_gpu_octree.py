@@ -142,3 +142,42 @@ def test_full_size_mesh_properties(gpu):
     for s in range(5, 10):
         x = OB.continuous_inlet(OI.rk4_step(x, tg[s], dt, fn), bounds, "x", "positive", "grid", conc)
     np.testing.assert_array_equal(out[2].cpu().numpy()[idx], x)
+
+
+@pytest.mark.parametrize("n,leaf,depth,shuffle", [(8, 8, 12, False), (5, 2, 12, True), (6, 4, 3, True),
+                                                  (3, 1000, 12, False), (4, 1, 2, True), (4, 3, 0, False)])
+def test_device_build_matches_oracle_and_host(gpu, n, leaf, depth, shuffle):
+    """The GPU build (levels of stable partitions, then depth-first
+    renumbering) gives the arrays of build_octree_mesh_optimized
+    (octree_fem_interpolator_optimized.py:41-215) bitwise: against the oracle
+    and the C++ host build, with the element order shuffled (the partitions
+    must keep it), leaves cut by max_depth (rows wider than max_per_leaf) and
+    a root that is a leaf."""
+    pts, conn = W.kuhn_mesh(n, 0.2, 7)
+    if shuffle:
+        conn = conn[np.random.default_rng(n).permutation(len(conn))]
+    times = np.array([0.0, 1.0], F32)
+    data = W.poiseuille_vortex(pts, times)
+    fd = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn, max_elements_per_leaf=leaf, max_depth=depth)
+    fh = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn, max_elements_per_leaf=leaf, max_depth=depth,
+                                              build_on="host")
+    want = OO.build(pts, conn, leaf, depth)
+    for k, v in want.items():
+        np.testing.assert_array_equal(fd.octree[k], v.astype(fd.octree[k].dtype), err_msg=k)
+    for k, v in fh.octree.items():
+        np.testing.assert_array_equal(fd.octree[k], v, err_msg=k)
+    assert fd.tree_depth == fh.tree_depth
+
+
+def test_device_build_full_size_matches_host(gpu):
+    """The unstructured workload's mesh (128^3 hexes x 6 tets, 12,582,912
+    elements, jittered): every array of the GPU build equals the host
+    build's."""
+    pts, conn = W.kuhn_mesh(128, 0.2, 4)
+    times = np.array([0.0, 1.0], F32)
+    data = np.zeros((2, len(pts), 3), F32)
+    fd = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn)
+    fh = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn, build_on="host")
+    for k, v in fh.octree.items():
+        np.testing.assert_array_equal(fd.octree[k], v, err_msg=k)
+    assert fd.tree_depth == fh.tree_depth and len(fd.octree["nodes_min"]) > 100_000

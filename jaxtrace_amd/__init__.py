@@ -24,6 +24,7 @@ from .boundary import (
 from .tracking import TrackerOptions, Trajectory, ParticleTracker, create_tracker
 from .adjoint import track_rk4
 from .seeding import random_seeds, uniform_grid_seeds
+from .vtk_io import VTKUnstructuredTimeSeriesReader, load_octree_field, read_octree_series
 from .density import HashGridNeighbors, SPHDensityEstimator, KDEEstimator, scott_bandwidth, silverman_bandwidth
 
 __version__ = "0.1.0"
@@ -36,4 +37,5 @@ __all__ = [
     "TrackerOptions", "Trajectory", "ParticleTracker", "create_tracker",
     "track_rk4", "HashGridNeighbors", "SPHDensityEstimator", "KDEEstimator", "scott_bandwidth", "silverman_bandwidth",
     "uniform_grid_seeds", "random_seeds", "generate_inlet_grid_particles",
+    "VTKUnstructuredTimeSeriesReader", "load_octree_field", "read_octree_series",
 ]

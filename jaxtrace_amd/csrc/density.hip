@@ -564,6 +564,127 @@ k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int 
     if (lane == 0) out_rho[qi] = nvalid <= K ? acc : __uint_as_float(JT_SPH_REDO);
 }
 
+// The top-K pass of jt_sph_eval for uniform masses and smoothing: one wave
+// per query that k_sph_direct marked JT_SPH_REDO (more than K valid
+// candidates), others exit at once.  The distinct probe cells' capped slots
+// (x32: 9 cells, each standing for mult = 3 stencil copies) are laid end to
+// end as in k_sph_direct and their distance bits (invalid: +inf) kept in
+// LDS.  A radix select, 8 bits a pass, finds the distinct-list key of rank
+// (K - 1) / mult: its distance Td, and b distinct candidates strictly nearer.
+// The K nearest of the reference's multiset are then the mult copies of
+// those b and K - mult b copies at Td (all within r, as more than K are), so
+// the density is mult * sum_{d < Td} W(d) + (K - mult b) W(Td).
+// LDS per wave: list_cap distance words, 256 counters and the 32-entry
+// probe tables.
+__global__ void __launch_bounds__(256)
+k_sph_select(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K, int list_cap, float hmean,
+             int kern, float* __restrict__ out_rho) {
+    extern __shared__ unsigned sel_lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int qi = blockIdx.x * 4 + wv;
+    if (qi >= m) return;                       // whole wave exits together (no block barrier below)
+    if (__float_as_uint(out_rho[qi]) != JT_SPH_REDO) return;
+    unsigned* dist = sel_lds + (size_t)wv * (list_cap + 256 + 64);
+    unsigned* hist = dist + list_cap;
+    int* toff = (int*)(hist + 256);
+    int* tst = toff + 32;
+    const int D = g.dim;
+    float q[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
+    int cq[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) cq[a] = cell_coord(q[a], g.origin[a], g.inv);
+    const bool x32 = g.key_mode == JT_KEY_X32;
+    const int mult = x32 ? 3 : 1;
+    const int probes = ((D == 3) ? 27 : 9) / mult;
+    int st = 0, cnt = 0;
+    if (lane < probes) {
+        const int p = x32 ? lane + probes : lane;
+        const int di = (D == 3) ? p / 9 - 1 : p / 3 - 1;
+        const int dj = (D == 3) ? (p / 3) % 3 - 1 : p % 3 - 1;
+        const int dk = (D == 3) ? p % 3 - 1 : 0;
+        const int cell = (D == 3) ? probe_cell(g, cq[0] + di, cq[1] + dj, cq[2] + dk)
+                                  : probe_cell(g, cq[0] + di, cq[1] + dj, 0);
+        if (cell >= 0) { st = g.cstart[cell]; cnt = min(g.cend[cell] - st, cap); }
+    }
+    int incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    if (lane < probes) { toff[lane] = incl - cnt; tst[lane] = st; }
+    if (lane == 0) toff[probes] = total;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    int p = 0;
+    for (int f = lane; f < total; f += 64) {
+        while (f >= toff[p + 1]) ++p;
+        const int slot = tst[p] + (f - toff[p]);
+        const float* pp = g.spos + (size_t)slot * D;
+        float dx = pp[0] - q[0], dy = pp[1] - q[1];
+        float d2 = dx * dx + dy * dy;
+        if (D == 3) { float dz = pp[2] - q[2]; d2 = d2 + dz * dz; }
+        const float d = sqrtf(d2);
+        dist[f] = d <= r ? __float_as_uint(d) : 0x7f800000u;
+    }
+    unsigned prefix = 0u, pmask = 0u;
+    const int kr0 = (K - 1) / mult;
+    int kr = kr0;                                   // rank sought among the keys under the prefix
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int b = lane; b < 256; b += 64) hist[b] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int c = lane; c < total; c += 64) {
+            const unsigned db = dist[c];
+            if ((db & pmask) == prefix) atomicAdd(&hist[(db >> shift) & 255u], 1u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                       h3 = hist[4 * lane + 3];
+        const unsigned tot = h0 + h1 + h2 + h3;
+        unsigned inc = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned v = __shfl_up(inc, o);
+            if (lane >= o) inc += v;
+        }
+        const unsigned excl = inc - tot;
+        const bool mine = (unsigned)kr >= excl && (unsigned)kr < inc;   // exactly one lane
+        const int src = __builtin_ctzll(__ballot(mine));
+        int bin = 4 * lane;
+        unsigned below = excl, r2 = (unsigned)kr - excl;
+        if (r2 >= h0) {
+            r2 -= h0; below += h0; ++bin;
+            if (r2 >= h1) {
+                r2 -= h1; below += h1; ++bin;
+                if (r2 >= h2) { below += h2; ++bin; }
+            }
+        }
+        bin = __shfl(bin, src);
+        below = __shfl(below, src);
+        kr -= (int)below;
+        prefix |= (unsigned)bin << shift;
+        pmask |= 255u << shift;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // the counters are read before
+        __builtin_amdgcn_wave_barrier();                           // the next pass clears them
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    float s = 0.0f;
+    for (int c = lane; c < total; c += 64) {
+        const unsigned db = dist[c];
+        if (db < prefix) s += sph_w(kern, D, __uint_as_float(db), hmean);
+    }
+    s = wave_sum(s);
+    const int nearer = kr0 - kr;                    // distinct candidates strictly nearer than Td
+    if (lane == 0) {
+        const float wt = sph_w(kern, D, __uint_as_float(prefix), hmean);
+        out_rho[qi] = (x32 ? s * 3.0f : s) + (float)(K - mult * nearer) * wt;
+    }
+}
+
 static int query_common(const jt_hashgrid* h, HGDev& g, int64_t m, int32_t cap, int32_t K, int& nc_pad, int& waves) {
     int rc = make_hg(h, g);
     if (rc) return rc;
@@ -666,6 +787,15 @@ extern "C" int jt_sph_eval(const jt_hashgrid* h, const float* Q, int64_t m, floa
     hipLaunchKernelGGL(k_sph_direct, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, (hipStream_t)stream, g, Q,
                        (int)m, radius, per_cell_cap, max_neighbors, masses, smoothing, h_mean, kernel, out_rho);
     JT_HIP(hipGetLastError());
+    const int mult = g.key_mode == JT_KEY_X32 ? 3 : 1;   // stencil copies of each distinct probe cell
+    const int list_cap = ((g.dim == 3 ? 27 : 9) / mult) * per_cell_cap;
+    const size_t sel_lds = (size_t)(list_cap + 256 + 64) * 4 * 4;
+    if (masses == nullptr && smoothing == nullptr && sel_lds <= 64 * 1024) {
+        hipLaunchKernelGGL(k_sph_select, dim3((unsigned)((m + 3) / 4)), dim3(256), sel_lds, (hipStream_t)stream, g,
+                           Q, (int)m, radius, per_cell_cap, max_neighbors, list_cap, h_mean, kernel, out_rho);
+        JT_HIP(hipGetLastError());
+        return JT_OK;
+    }
     size_t lds = ((size_t)nc_pad * 12 + 256 * 4) * waves;      // + k_query<1>'s radix-select counters
     hipLaunchKernelGGL(k_query<1>, dim3((unsigned)((m + waves - 1) / waves)), dim3(64 * waves), lds,
                        (hipStream_t)stream, g, Q, (int)m, radius, per_cell_cap, max_neighbors, nc_pad,

@@ -63,20 +63,39 @@ def test_sph_density(gpu, key_mode, kernel):
     np.testing.assert_allclose(got[~m], want[~m], rtol=1e-5, atol=1e-5 * floor)
 
 
-def test_sph_truncation_path(gpu):
+@pytest.mark.parametrize("key_mode", ["x32", "x64"])
+@pytest.mark.parametrize("K", [100, 128, 512])
+def test_sph_truncation_path(gpu, key_mode, K):
     """Dense cluster: more than max_neighbors valid candidates, so the
-    wave takes the stable first-K selection."""
+    wave takes the stable first-K selection (x32: K not a multiple of the
+    stencil's 3 copies of each cell, so the cut falls inside a copy group)."""
     rng = np.random.default_rng(6)
     P = (0.5 + 0.002 * rng.standard_normal((20_000, 3))).astype(F32)
     h = 0.004
     Q = (0.5 + 0.003 * rng.standard_normal((256, 3))).astype(F32)
-    est = jt.SPHDensityEstimator(P, dimensions="3d", smoothing_length=h, key_mode="x64", max_neighbors=100,
+    est = jt.SPHDensityEstimator(P, dimensions="3d", smoothing_length=h, key_mode=key_mode, max_neighbors=K,
                                  normalize=False)
-    want, idx, _ = OD.sph_density(P, Q, h, 64, 100, "x64", False)
+    want, idx, _ = OD.sph_density(P, Q, h, 64, K, key_mode, False)
     np.testing.assert_allclose(est.evaluate(Q), want, rtol=1e-5)
 
 
-def test_sph_truncation_ties(gpu):
+def test_sph_truncation_per_point_masses(gpu):
+    """Per-point masses keep the sorted first-K selection (which tied
+    candidates are kept matters then)."""
+    rng = np.random.default_rng(16)
+    P = (0.5 + 0.002 * rng.standard_normal((20_000, 3))).astype(F32)
+    masses = rng.uniform(0.5, 2.0, len(P)).astype(F32)
+    h = 0.004
+    Q = (0.5 + 0.003 * rng.standard_normal((128, 3))).astype(F32)
+    for key_mode in ("x32", "x64"):
+        est = jt.SPHDensityEstimator(P, dimensions="3d", smoothing_length=h, key_mode=key_mode, max_neighbors=100,
+                                     normalize=False, masses=masses)
+        want, _, _ = OD.sph_density(P, Q, h, 64, 100, key_mode, False, masses=masses)
+        np.testing.assert_allclose(est.evaluate(Q), want, rtol=1e-5)
+
+
+@pytest.mark.parametrize("key_mode", ["x32", "x64"])
+def test_sph_truncation_ties(gpu, key_mode):
     """The K-th nearest inside a group of equal distances.  Points on a
     lattice of spacing 2^-9 around lattice-point queries have bit-identical
     float32 distances per shell (|k|^2 = 9 holds 30 points, and the 100th
@@ -89,9 +108,9 @@ def test_sph_truncation_ties(gpu):
     Q = (F32(0.5) + d * np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, -1, 2], [2, 2, 2]], F32)).astype(F32)
     h = float(4 * d)
     for K in (100, 57, 123):
-        est = jt.SPHDensityEstimator(P, dimensions="3d", smoothing_length=h, key_mode="x64", max_neighbors=K,
+        est = jt.SPHDensityEstimator(P, dimensions="3d", smoothing_length=h, key_mode=key_mode, max_neighbors=K,
                                      normalize=False)
-        want, _, _ = OD.sph_density(P, Q, h, 64, K, "x64", False)
+        want, _, _ = OD.sph_density(P, Q, h, 64, K, key_mode, False)
         np.testing.assert_allclose(est.evaluate(Q), want, rtol=1e-5)
 
 

@@ -2,9 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  This package restates, op by op in NumPy float32
 (with float64 twins where noted), the algorithm the reference JAXTrace runs
-under ``jax.jit``.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
-as the timed CPU baseline; the product package ``jaxtrace_amd`` never imports
+under ``jax.jit``.  Only ``tests/``, ``__graft_entry__.smoke()`` and, in ``bench.py``, the
+``cpu_baseline`` leg and the float64 reference tracks of the trajectory-error
+check may import it, and only as the checker or as the timed CPU baseline; the product package ``jaxtrace_amd`` never imports
 it and has no CPU fallback.
 
 Every function cites the reference file:line it follows (paths relative to the

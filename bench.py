@@ -6,10 +6,15 @@ divergence-free random Fourier field (64 modes, |k| <= 8, unit RMS speed,
 seed 2) stored in float32 (headline) and bfloat16.  A "step" is one RK4 step
 of every particle; inputs are resident in HBM when timing starts.
 
-Prints ONE JSON line (rank 0).  Roofline: algorithmic bytes per launch
-(792 B/particle-step in FP32, 408 B in BF16: 24 B of state plus 4 stages x 2
-snapshots x 8 corners x 3 components) over the average launch duration timed
-with HIP events on the launch stream.
+Prints ONE JSON line (rank 0).  Roofline: the tracking kernel's HBM bytes
+per particle-step measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE, scaled by
+a plain copy's readings; profiles/kernel_counters.json) x the particle-steps of
+one launch, over the launch's average duration timed with HIP events on the
+launch stream, against 8 TB/s.  The algorithmic gather model (792 B per
+particle-step in FP32, 408 B in BF16: 24 B of state plus 4 stages x 2
+snapshots x 8 corners x 3 components) is reported beside it.  The other
+sections (BF16, RK2/Euler, octree, adjoint, density, seeds, VTK ingestion)
+report their own rates and rooflines the same way.
 """
 from __future__ import annotations
 
@@ -102,7 +107,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
-    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "hashgrid", "nearest", "seeding"],
+    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "hashgrid", "nearest", "seeding",
+                                       "vtk"],
                     help="run one section only and print its JSON (profiler runs)")
     ap.add_argument("--adjoint-reps", type=int, default=3,
                     help="adjoint repetitions per storage; the last one is reported")
@@ -389,10 +395,127 @@ def bench_seeding(torch, jt, W, L, lib, dev, res=256, reps=20):
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
     gbs = n * 12 / (ms * 1e-3) / 1e9
+    out_d = {"value": n / (ms * 1e-3), "unit": "seeds/s", "ms": ms, "seeds": n,
+             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": gbs / HBM_PEAK_GBS, "bytes_per_seed": 12},
+             "workload": f"uniform_grid_seeds {res}^3 in the unit box, mean of {reps} calls"}
+    del out
+    out_d["random_seeds"] = _bench_random_seeds(torch, L, lib, dev, n, reps)
+    out_d["inlet_grid_particles"] = _bench_inlet(torch, L, lib, dev, 1 << 22, reps)
+    return out_d
+
+
+def _timed(torch, call, reps):
+    """Mean HIP-event time of reps calls on the current stream, after one."""
+    call()
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch)
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _bench_random_seeds(torch, L, lib, dev, n, reps):
+    """random_seeds' JAX branch (seeding.py:140-174) on the GPU: n seeds of
+    jax.random.uniform(PRNGKey(42), (n, 3)) scaled to the unit box
+    (jt_random_seeds: one threefry2x32-20 block per two output words).
+    12 B written per seed; the 20 rounds are VALU work."""
+    out = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    lo = (C.c_float * 3)(0.0, 0.0, 0.0)
+    hi = (C.c_float * 3)(1.0, 1.0, 1.0)
+
+    def call():
+        L.check(lib.jt_random_seeds(n, 42, lo, hi, L.ptr(out), L.stream_handle()), "jt_random_seeds")
+
+    ms = _timed(torch, call, reps)
+    gbs = n * 12 / (ms * 1e-3) / 1e9
     return {"value": n / (ms * 1e-3), "unit": "seeds/s", "ms": ms, "seeds": n,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_seed": 12},
-            "workload": f"uniform_grid_seeds {res}^3 in the unit box, mean of {reps} calls"}
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_seed": 12,
+                         "note": "the 20 threefry rounds per two words (integer VALU work) cost more than the "
+                                 "12 B of writes per seed"},
+            "workload": f"random_seeds n = {n}, rng_seed 42, unit box, mean of {reps} calls"}
+
+
+def _bench_inlet(torch, L, lib, dev, n, reps):
+    """_generate_inlet_grid_particles' face-grid expansion
+    (boundary.py:1321-1441) for n particles on a 64 x 64 inlet face (the
+    octree workload's concentrations), indices resident: 4 B read and 12 B
+    written per particle."""
+    c = torch.linspace(0.0, 1.0, 64, dtype=torch.float32, device=dev)
+    idx = torch.randint(0, 64 * 64, (n,), generator=torch.Generator(device="cpu").manual_seed(5),
+                        dtype=torch.int32).to(dev)
+    out = torch.empty((n, 3), dtype=torch.float32, device=dev)
+
+    def call():
+        L.check(lib.jt_inlet_grid_particles(L.ptr(idx), n, 0, C.c_float(0.0), L.ptr(c), 64, L.ptr(c), 64, L.ptr(out),
+                                            L.stream_handle()), "jt_inlet_grid_particles")
+
+    ms = _timed(torch, call, reps)
+    gbs = n * 16 / (ms * 1e-3) / 1e9
+    return {"value": n / (ms * 1e-3), "unit": "particles/s", "ms": ms, "particles": n,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_particle": 16},
+            "workload": f"inlet face grid 64 x 64 on x = 0, {n} particles, mean of {reps} calls"}
+
+
+def bench_vtk(torch, jt, W, L, lib, dev, n=48, steps=4):
+    """The on-disk input path (vtk_reader.py:426-942, example_workflow.py:
+    436-660): a series of `steps` .pvtu files, each naming two .vtu pieces
+    (raw appended binary) of the n^3 x 6 jittered-tet mesh with the
+    Poiseuille-vortex velocity as 'Displacement', written to a temporary
+    directory; then load_octree_field (host parse of every file, GPU octree
+    build, uploads).  Host I/O bound: reported as file MB/s, with the parse
+    and the field construction timed apart."""
+    import shutil
+    import tempfile
+    from jaxtrace_amd import vtk_io as V
+    pts, conn = W.kuhn_mesh(n, 0.2, 4)
+    pts, conn = pts.astype(np.float32), conn.astype(np.int32)
+    half = len(conn) // 2
+    pieces = []
+    for cc in (conn[:half], conn[half:]):
+        used, inv = np.unique(cc, return_inverse=True)
+        pieces.append((pts[used], inv.reshape(cc.shape).astype(np.int32)))
+    allp = np.concatenate([pieces[0][0], pieces[1][0]])
+    times = (np.arange(steps) / 14.0).astype(np.float32)
+    vel = W.poiseuille_vortex(allp, times).astype(np.float32)
+    d = tempfile.mkdtemp(prefix="jt_vtk_")
+    try:
+        na = len(pieces[0][0])
+        for k in range(steps):
+            srcs = []
+            for j, (pp, cc) in enumerate(pieces):
+                v = vel[k, :na] if j == 0 else vel[k, na:]
+                f = f"p{j}_{k}.vtu"
+                V.write_vtu(os.path.join(d, f), pp, cc, {"Displacement": v}, encoding="appended")
+                srcs.append(f)
+            V.write_pvtu(os.path.join(d, f"flow_{k:04d}.pvtu"), srcs, ["Displacement"])
+        nbytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        pattern = os.path.join(d, "flow_*.pvtu")
+        t0 = time.perf_counter()
+        s = V.read_octree_series(pattern, max_timesteps=steps)
+        t_parse = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = jt.OctreeFEMTimeSeriesFieldOptimized(s["velocity_data"], s["times"], s["positions"], s["connectivity"])
+        torch.cuda.synchronize()
+        t_field = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        f2 = jt.load_octree_field(pattern, max_timesteps=steps)
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t0
+        assert f2.connectivity.shape == (len(conn), 4)
+        del f, f2
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"value": nbytes / t_parse / 1e6, "unit": "file MB/s (parse)", "file_bytes": nbytes,
+            "parse_s": t_parse, "field_build_s": t_field, "load_octree_field_s": t_all,
+            "workload": f"{steps} .pvtu files x 2 raw-appended .vtu pieces, {n}^3 x 6 tets ({len(conn)}), "
+                        f"{len(allp)} points, 'Displacement' per file; host cores parse, the GPU builds the octree"}
 
 
 def bench_nearest(torch, jt, W, L, lib, dev, m=1 << 20):
@@ -679,7 +802,8 @@ def main():
               "kde": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde",)),
               "hashgrid": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("hashgrid",)),
               "nearest": lambda: bench_nearest(torch, jt, W, L, lib, dev),
-              "seeding": lambda: bench_seeding(torch, jt, W, L, lib, dev)}[args.only]
+              "seeding": lambda: bench_seeding(torch, jt, W, L, lib, dev),
+              "vtk": lambda: bench_vtk(torch, jt, W, L, lib, dev)}[args.only]
         print(json.dumps({"only": args.only, "result": fn()}), flush=True)
         return
 
@@ -788,7 +912,7 @@ def main():
         result["adjoint_fwd_ratio"] = None
     if not args.no_extras:
         for key, fn in (("octree", bench_octree), ("density", bench_density), ("nearest_node", bench_nearest),
-                        ("seeding", bench_seeding)):
+                        ("seeding", bench_seeding), ("vtk_ingestion", bench_vtk)):
             try:
                 result[key] = fn(torch, jt, W, L, lib, dev)
             except Exception as e:  # reported, never fatal

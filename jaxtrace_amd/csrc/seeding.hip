@@ -95,21 +95,23 @@ __device__ __forceinline__ void threefry2x32(unsigned k0, unsigned k1, unsigned&
 // into halves x0 | x1 of h words; block i hashes (x0[i], x1[i]) and gives
 // word i and word h + i.  A word becomes the float in [1, 2) with its top 23
 // bits as mantissa, minus 1 (seeding.py:167-172 then scale to the bounds).
+// m < 2^32 (checked on the host), so word indices are 32-bit: the component
+// i % 3 is a multiply-high, not a 64-bit division.
 __global__ void __launch_bounds__(256)
-k_random_seeds(unsigned k0, unsigned k1, unsigned long long m, unsigned long long h, float lo0, float lo1,
-               float lo2, float w0, float w1, float w2, float* __restrict__ out) {
+k_random_seeds(unsigned k0, unsigned k1, unsigned m, unsigned h, float lo0, float lo1, float lo2, float w0,
+               float w1, float w2, float* __restrict__ out) {
     const float lo[3] = {lo0, lo1, lo2}, w[3] = {w0, w1, w2};
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < h;
-         i += (unsigned long long)gridDim.x * blockDim.x) {
-        const unsigned long long j = h + i;
-        unsigned x0 = (unsigned)i, x1 = j < m ? (unsigned)j : 0u;
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < h; i += stride) {
+        const unsigned j = h + i;                       // < m + 1 <= 2^32 - 1 + 1: j < m or j == m (odd m)
+        unsigned x0 = i, x1 = j < m ? j : 0u;
         threefry2x32(k0, k1, x0, x1);
         const float u0 = __uint_as_float((x0 >> 9) | 0x3f800000u) - 1.0f;
-        const int c0 = (int)(i % 3);
+        const unsigned c0 = i % 3u;
         out[i] = lo[c0] + u0 * w[c0];
         if (j < m) {
             const float u1 = __uint_as_float((x1 >> 9) | 0x3f800000u) - 1.0f;
-            const int c1 = (int)(j % 3);
+            const unsigned c1 = j % 3u;
             out[j] = lo[c1] + u1 * w[c1];
         }
     }
@@ -153,8 +155,8 @@ extern "C" int jt_random_seeds(int64_t n, uint64_t rng_seed, const float* lo, co
     const unsigned k0 = (unsigned)(rng_seed >> 32), k1 = (unsigned)(rng_seed & 0xffffffffull);   // PRNGKey
     const unsigned long long want = (h + 255) / 256;
     const unsigned blocks = (unsigned)(want < 65536ull ? want : 65536ull);
-    hipLaunchKernelGGL(k_random_seeds, dim3(blocks), dim3(256), 0, (hipStream_t)stream, k0, k1, m, h, lo[0], lo[1],
-                       lo[2], w[0], w[1], w[2], out);
+    hipLaunchKernelGGL(k_random_seeds, dim3(blocks), dim3(256), 0, (hipStream_t)stream, k0, k1, (unsigned)m,
+                       (unsigned)h, lo[0], lo[1], lo[2], w[0], w[1], w[2], out);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

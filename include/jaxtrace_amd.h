@@ -100,6 +100,16 @@ typedef struct jt_grid_field {
     int32_t bounds_mode;      /* JT_BOUNDS_*                                   */
     int32_t time_interp;      /* JT_TIME_*                                     */
     int32_t time_extrap;      /* JT_EXTRAP_*                                   */
+    const float* pair_data;   /* optional, float32 fields with ncomp 4 and a z
+                                 halo: consecutive-pair records, C order
+                                 (n_snap-1, nx, ny, nz, 12): for pair (s, s+1)
+                                 and node (i, j, k) the xyz of V_s(k),
+                                 V_s(k+1), V_s+1(k), V_s+1(k+1) (k+1 the halo
+                                 plane's node for k = nz-1), as
+                                 jt_grid_pair_records writes them.  Tracking
+                                 steps that blend (s, s+1) gather a cell row
+                                 of both snapshots in one 48-byte record.
+                                 NULL: not used.                               */
 } jt_grid_field;
 
 /* Boundary applied after every full step (tracker.py:238-239). */
@@ -137,6 +147,11 @@ int32_t     jt_version(void);
 /* Replaces one field_fn call: TimeSeriesField._sample_all_nodes_at_time_jax
  * (time_series.py:155-245) composed with StructuredGridSampler._trilinear_jax
  * (structured.py:167-250).  out[i] = u(x[i], t). */
+/* The consecutive-pair records of field (float32, ncomp 4, z halo) into
+ * out, a device array of (n_snap-1) * nx * ny * nz * 12 floats (the
+ * pair_data layout above; field->pair_data is ignored).  n_snap >= 2. */
+int jt_grid_pair_records(const jt_grid_field* field, float* out, void* stream);
+
 int jt_grid_sample(const jt_grid_field* field, const float* x, int64_t n,
                    float t, float* out, void* stream);
 

@@ -37,6 +37,7 @@ class GridField(C.Structure):
         ("dtype", C.c_int32), ("ncomp", C.c_int32), ("z_halo", C.c_int32),
         ("origin", c_float3), ("spacing", c_float3),
         ("bounds_mode", C.c_int32), ("time_interp", C.c_int32), ("time_extrap", C.c_int32),
+        ("pair_data", C.c_void_p),
     ]
 
 
@@ -107,6 +108,7 @@ _SIGS = {
                                 C.c_void_p, C.c_void_p]),
     "jt_grid_sample": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_int64, C.c_float,
                                  C.c_void_p, C.c_void_p]),
+    "jt_grid_pair_records": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_void_p]),
     "jt_apply_boundary": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p]),
     "jt_inlet_exits_workspace_size": (C.c_size_t, [C.c_int64]),
     "jt_inlet_exits": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,

@@ -34,6 +34,8 @@ struct GridDev {
     float cmax[3];           // f32(max(n-1, 0)): clip bound of the grid coordinate
     int bmode;               // JT_BOUNDS_*
     int interp, extrap;      // JT_TIME_*, JT_EXTRAP_*
+    const float* pair;       // consecutive-pair records (jt_grid_field::pair_data) or nullptr
+    long long pair_stride;   // floats per pair (nx * ny * nz * 12)
 };
 
 // searchsorted(times, t, side='right') for non-decreasing times.
@@ -556,6 +558,53 @@ __device__ __forceinline__ void gather_cached(const GridDev& g, const Blend& bl,
     }
 }
 
+// gather_cached<ST_F32x4, true> from the pair records of snapshots
+// (i0, i0 + 1): one row of a cell is 12 consecutive floats (48 B, three
+// 16-byte loads) holding both snapshots' z-pairs, against four 16-byte loads
+// of the padded per-snapshot layout.  Same values, same arithmetic.
+__device__ __forceinline__ void gather_cached_pair(const GridDev& g, const Blend& bl, float px, float py, float pz,
+                                                   Corners& C, CornerCache<ST_F32x4>& cc) {
+    C.valid = true;
+    int i0, i1, j0, j1, k0, k1;
+    axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
+    axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
+    axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
+    const char* pp = (const char*)(g.pair + (size_t)bl.i0 * (size_t)g.pair_stride);
+    const bool same = pp == cc.pa && pp == cc.pb && i0 == cc.i0 && i1 == cc.i1 && j0 == cc.j0 && j1 == cc.j1 &&
+                      k0 == cc.k0;
+    if (!same) {
+        const unsigned rs = (unsigned)g.nz * 48u;
+        const unsigned r0 = (unsigned)__umul24((unsigned)i0, (unsigned)g.ny);
+        const unsigned r1 = (unsigned)__umul24((unsigned)i1, (unsigned)g.ny);
+        const unsigned kb = (unsigned)k0 * 48u;
+        const unsigned row[4] = {(unsigned)__umul24(r0 + (unsigned)j0, rs) + kb,
+                                 (unsigned)__umul24(r0 + (unsigned)j1, rs) + kb,
+                                 (unsigned)__umul24(r1 + (unsigned)j0, rs) + kb,
+                                 (unsigned)__umul24(r1 + (unsigned)j1, rs) + kb};
+        float4 f[4][3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float4* q = (const float4*)(pp + row[r]);
+            f[r][0] = q[0]; f[r][1] = q[1]; f[r][2] = q[2];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cc.A[2 * r][0] = f[r][0].x; cc.A[2 * r][1] = f[r][0].y; cc.A[2 * r][2] = f[r][0].z;
+            cc.A[2 * r + 1][0] = f[r][0].w; cc.A[2 * r + 1][1] = f[r][1].x; cc.A[2 * r + 1][2] = f[r][1].y;
+            cc.B[2 * r][0] = f[r][1].z; cc.B[2 * r][1] = f[r][1].w; cc.B[2 * r][2] = f[r][2].x;
+            cc.B[2 * r + 1][0] = f[r][2].y; cc.B[2 * r + 1][1] = f[r][2].z; cc.B[2 * r + 1][2] = f[r][2].w;
+        }
+        cc.i0 = i0; cc.i1 = i1; cc.j0 = j0; cc.j1 = j1; cc.k0 = k0;
+    }
+    cc.pa = pp;
+    cc.pb = pp;
+    const float al = bl.a, oma = 1.0f - bl.a;      // blend_n's 'interp' arithmetic
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) C.v[q][c] = oma * cc.A[q][c] + al * cc.B[q][c];
+}
+
 // sample<ST, true, true> through a CornerCache.
 template <int ST, bool IO = false>
 __device__ __forceinline__ void sample_cached(const GridDev& g, const Blend& bl, float px, float py, float pz,
@@ -790,6 +839,33 @@ __device__ __forceinline__ void fast_step(const GridDev& g, const BcDev& bc, con
     } else {
         auto f = [&](const Blend& b, const float q[3], float out[3]) {
             sample<ST, true, true>(g, b, q[0], q[1], q[2], out);
+        };
+        rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);
+        cache_reset(cc);
+    }
+    apply_bc_axes(bc, p);                               // tracker.py:238-239
+}
+
+// fast_step over the consecutive-pair records (float32 fields that carry
+// them): a step whose three stage blends are 'interp' between snapshots
+// (i0, i0 + 1) gathers through gather_cached_pair; any other step samples the
+// per-snapshot layout without the cache and empties it.
+template <int INTEG>
+__device__ __forceinline__ void fast_step_pair(const GridDev& g, const BcDev& bc, const Blend* __restrict__ tab, int s,
+                                               float p[3], float dt, float dth, float dt6,
+                                               CornerCache<ST_F32x4>& cc) {
+    const Blend* t = tab + 3 * s;
+    if (t[0].mode == BL_INTERP && t[1].mode == BL_INTERP && t[2].mode == BL_INTERP && t[0].i1 == t[0].i0 + 1 &&
+        t[1].i1 == t[1].i0 + 1 && t[2].i1 == t[2].i0 + 1) {
+        auto fp = [&](const Blend& b, const float q[3], float out[3]) {
+            Corners C;
+            gather_cached_pair(g, b, q[0], q[1], q[2], C, cc);
+            lerp_corners(g, C, out);
+        };
+        rk_step<INTEG>(fp, tab, s, p, dt, dth, dt6);
+    } else {
+        auto f = [&](const Blend& b, const float q[3], float out[3]) {
+            sample<ST_F32x4, true, true>(g, b, q[0], q[1], q[2], out);
         };
         rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);
         cache_reset(cc);

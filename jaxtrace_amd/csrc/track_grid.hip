@@ -51,6 +51,16 @@ int make_grid_dev(const jt_grid_field* f, GridDev& g, int& st) {
         return fail(JT_EUNSUPPORTED, "16-bit snapshots must be stored with ncomp=4");
     g.data = f->data;
     g.snap_stride = nodes * f->ncomp;
+    g.pair = nullptr;
+    g.pair_stride = 0;
+    if (f->pair_data != nullptr) {
+        JT_REQUIRE(f->dtype == JT_DTYPE_F32 && f->ncomp == 4 && f->z_halo == 1,
+                   "pair_data needs a float32 field with 4 components and a z halo");
+        const long long ps = (long long)f->nx * f->ny * f->nz * 12;
+        JT_REQUIRE(ps * 4 < (1LL << 32), "pair records of one snapshot pair exceed 4 GiB");
+        g.pair = f->pair_data;
+        g.pair_stride = ps;
+    }
     g.times = f->times;
     g.T = f->n_snap;
     g.nx = f->nx; g.ny = f->ny; g.nz = f->nz;
@@ -151,7 +161,9 @@ template <int ST> struct TrackBlock {
 
 // PF: the fast path (fast_step: periodic sampler index math, per-lane corner
 // reuse, per-axis boundary epilogue), chosen on the host where it applies.
-template <int INTEG, int ST, bool ZH, bool PF>
+// PR: the fast path over the field's consecutive-pair records
+// (fast_step_pair), chosen on the host when a float32 field carries them.
+template <int INTEG, int ST, bool ZH, bool PF, bool PR = false>
 __global__ void __launch_bounds__(TrackBlock<ST>::v, TrackWaves<ST>::v)
 k_track(GridDev g, BcDev bc, Reorder ro, int n,
         const Blend* __restrict__ tab, int s0, int s1, float dt, float dth, float dt6,
@@ -169,7 +181,9 @@ k_track(GridDev g, BcDev bc, Reorder ro, int n,
     CornerCache<ST> cc;
     cache_reset(cc);
     for (int s = s0; s < s1; ++s) {
-        if constexpr (PF) {
+        if constexpr (PR) {
+            fast_step_pair<INTEG>(g, bc, tab, s, p, dt, dth, dt6, cc);
+        } else if constexpr (PF) {
             fast_step<INTEG, ST>(g, bc, tab, s, p, dt, dth, dt6, cc);
         } else {
             rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);     // rk4.py:84-106 / rk2.py / euler.py
@@ -184,6 +198,27 @@ k_track(GridDev g, BcDev bc, Reorder ro, int n,
     ro.xout[3 * (size_t)i] = p[0];
     ro.xout[3 * (size_t)i + 1] = p[1];
     ro.xout[3 * (size_t)i + 2] = p[2];
+}
+
+// Consecutive-pair records from a float32 z-halo field with 4 components:
+// thread e takes record (p, i, j, k) and writes its 12 floats with three
+// 16-byte stores: the xyz of V_p(k), V_p(k+1), V_p+1(k), V_p+1(k+1), where
+// k+1 = nz is the halo plane.
+__global__ void __launch_bounds__(256)
+k_pair_records(const float4* __restrict__ snaps, unsigned n_rec, unsigned nz, unsigned nxy, unsigned nzs,
+               float4* __restrict__ out) {
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_rec) return;
+    const unsigned k = e % nz;
+    const unsigned r = e / nz;                          // p * nxy + ij
+    const unsigned p = r / nxy, ij = r - p * nxy;
+    const size_t snap = (size_t)nxy * nzs;
+    const size_t a0 = (size_t)p * snap + (size_t)ij * nzs + k;
+    const float4 a = snaps[a0], b = snaps[a0 + 1], c = snaps[a0 + snap], d = snaps[a0 + snap + 1];
+    float4* o = out + 3 * (size_t)e;
+    o[0] = make_float4(a.x, a.y, a.z, b.x);
+    o[1] = make_float4(b.y, b.z, c.x, c.y);
+    o[2] = make_float4(c.z, d.x, d.y, d.z);
 }
 
 template <int ST, bool ZH>
@@ -338,6 +373,13 @@ static void launch_track(hipStream_t s, const GridDev& g, const BcDev& bc, const
                          const Blend* tab, int s0, int s1, float dt, float dth, float dt6, int rec, float* traj) {
     const unsigned B = TrackBlock<ST>::v;
     const unsigned nb = ((unsigned)n + B - 1) / B;
+    if constexpr (PF && ST == ST_F32x4) {
+        if (g.pair != nullptr) {                        // the fast path over consecutive-pair records
+            hipLaunchKernelGGL((k_track<INTEG, ST, ZH, PF, true>), dim3((nb + 7) / 8 * 8), dim3(B), 0, s, g, bc, ro, n,
+                               tab, s0, s1, dt, dth, dt6, rec, traj);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_track<INTEG, ST, ZH, PF>), dim3((nb + 7) / 8 * 8), dim3(B), 0, s, g, bc, ro, n, tab, s0, s1,
                        dt, dth, dt6, rec, traj);
 }
@@ -396,6 +438,25 @@ extern "C" int jt_grid_sample(const jt_grid_field* field, const float* x, int64_
     dim3 block(256), grid((unsigned)((n + 255) / 256));
     if (g.nzs != g.nz) launch_sample<true>(st, grid, block, s, g, x, (int)n, t, out);
     else launch_sample<false>(st, grid, block, s, g, x, (int)n, t, out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+extern "C" int jt_grid_pair_records(const jt_grid_field* field, float* out, void* stream) {
+    JT_REQUIRE(field != nullptr && out != nullptr, "NULL pointer");
+    jt_grid_field f = *field;
+    f.pair_data = nullptr;
+    GridDev g;
+    int st;
+    int rc = make_grid_dev(&f, g, st);
+    if (rc) return rc;
+    JT_REQUIRE(st == ST_F32x4 && g.nzs == g.nz + 1, "pair records need a float32 field with 4 components and a z halo");
+    JT_REQUIRE(g.T >= 2, "pair records need at least two snapshots");
+    const long long n_rec = (long long)(g.T - 1) * g.nx * g.ny * g.nz;
+    JT_REQUIRE(n_rec < (1LL << 31), "too many pair records (%lld)", n_rec);
+    hipLaunchKernelGGL(k_pair_records, dim3((unsigned)((n_rec + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)g.data, (unsigned)n_rec, (unsigned)g.nz, (unsigned)(g.nx * g.ny),
+                       (unsigned)g.nzs, (float4*)out);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

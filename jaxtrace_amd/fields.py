@@ -59,7 +59,7 @@ class _GridSeries:
     kind = "grid"
 
     def _setup(self, data, times, grid_x, grid_y, grid_z, bounds_handling, interpolation,
-               extrapolation, storage, layout, device):
+               extrapolation, storage, layout, device, pair_records=False):
         self.device = torch.device(device) if device is not None else L.device()
         mode = str(bounds_handling).lower()
         if mode not in L.JT_BOUNDS:
@@ -133,7 +133,36 @@ class _GridSeries:
         d.bounds_mode = L.JT_BOUNDS[mode]
         d.time_interp = L.JT_TIME.get(interp, 0)
         d.time_extrap = L.JT_EXTRAP[extra]
+        self._pairs = None
         self._desc = d
+        self._build_pair_records(pair_records)
+
+    def _build_pair_records(self, want):
+        """Consecutive-pair records (jt_grid_pair_records): for each snapshot
+        pair (s, s+1) and node, both snapshots' z-pairs in one 48-byte record,
+        so the tracking fast path gathers a cell row with three 16-byte loads
+        instead of four.  They cost (T-1) * Nx * Ny * Nz * 48 bytes of device
+        memory.  want None: built for float32 'halo' fields with periodic
+        bounds (the fast path's fields) when they take at most a quarter of
+        the free device memory; True: built for any float32 'halo' field;
+        False: not built."""
+        applicable = (self.storage == "float32" and self._desc.ncomp == 4 and self.z_halo == 1 and self.T >= 2)
+        if want is None:
+            nbytes = (self.T - 1) * self.Nx * self.Ny * self.Nz * 48
+            want = applicable and self.bounds_handling == "periodic" and \
+                nbytes <= torch.cuda.mem_get_info(self.device)[0] // 4
+        elif want and not applicable:
+            raise ValueError("pair_records needs float32 storage, layout 'halo' or 'halo4' and at least two snapshots")
+        if not want:
+            self._desc.pair_data = None
+            return
+        lib = L.load()
+        pairs = torch.empty((self.T - 1, self.Nx, self.Ny, self.Nz, 12), dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            L.check(lib.jt_grid_pair_records(C.byref(self._desc), L.ptr(pairs), L.stream_handle()),
+                    "jt_grid_pair_records")
+        self._pairs = pairs
+        self._desc.pair_data = pairs.data_ptr()
 
     # -- kernel descriptor ---------------------------------------------------
     def descriptor(self) -> L.GridField:
@@ -193,7 +222,7 @@ class StructuredTimeSeriesField(_GridSeries):
     def __init__(self, data, times, grid_x, grid_y, grid_z, bounds_handling: str = "clamp",
                  interpolation: str = "linear", extrapolation: str = "constant",
                  storage: str = "float32", layout: str = "halo", device=None,
-                 _source_info: Optional[dict] = None):
+                 _source_info: Optional[dict] = None, pair_records: Optional[bool] = None):
         v = data if isinstance(data, torch.Tensor) else torch.from_numpy(
             np.ascontiguousarray(np.asarray(data, dtype=F32)))
         if v.ndim == 3:
@@ -205,7 +234,7 @@ class StructuredTimeSeriesField(_GridSeries):
         if v.ndim != 5:
             raise ValueError(f"data must be (T,Nx,Ny,Nz,3) or (T,N,3), got {tuple(v.shape)}")
         self._setup(v, times, grid_x, grid_y, grid_z, bounds_handling, interpolation,
-                    extrapolation, storage, layout, device)
+                    extrapolation, storage, layout, device, pair_records)
         self._source_info = _source_info or {}
 
     def sample_at_positions(self, positions, t: float):
@@ -219,12 +248,14 @@ class StructuredTimeSeriesField(_GridSeries):
     def get_time_bounds(self):
         return (self.t_min, self.t_max)
 
-    def with_storage(self, storage: str, layout: str = "halo") -> "StructuredTimeSeriesField":
+    def with_storage(self, storage: str, layout: str = "halo",
+                     pair_records: Optional[bool] = None) -> "StructuredTimeSeriesField":
         """Same field, snapshots re-stored as ``storage`` (RNE from float32)."""
         src = self._snaps[:, :, :, :self.Nz, self._slots].to(torch.float32)
         return StructuredTimeSeriesField(src, self.times, self.grid_x, self.grid_y, self.grid_z,
                                          self.bounds_handling, self.interpolation,
-                                         self.extrapolation, storage, layout, self.device)
+                                         self.extrapolation, storage, layout, self.device,
+                                         pair_records=pair_records)
 
 
 class OctreeFEMTimeSeriesFieldOptimized:

@@ -336,3 +336,45 @@ def test_continuous_inlet_random_bitwise(gpu, dist, direction):
                        lambda p: OB.continuous_inlet(p, b, "x", direction, dist), "rk4")
     np.testing.assert_array_equal(traj.positions, pos)
     assert (pos[1:, :, 0] == (0.0 if direction == "positive" else 1.0)).any()   # replacements happened
+
+
+@pytest.mark.parametrize("mode", ["periodic", "clamp"])
+def test_pair_records_layout(gpu, mode):
+    """jt_grid_pair_records: for each snapshot pair (s, s+1) and node (i, j, k)
+    the xyz of V_s(k), V_s(k+1), V_s+1(k), V_s+1(k+1), with k+1 = Nz the halo
+    plane (plane 0 for periodic bounds, Nz-1 otherwise); an odd grid."""
+    rng = np.random.default_rng(3)
+    data = rng.standard_normal((3, 5, 6, 7, 3)).astype(F32)
+    gx, gy, gz = (np.linspace(0, 1, n, dtype=F32) for n in (5, 6, 7))
+    f = jt.StructuredTimeSeriesField(data, np.array([0, 0.5, 1], F32), gx, gy, gz, bounds_handling=mode,
+                                     pair_records=True)
+    rec = f._pairs.cpu().numpy()
+    nxt = np.concatenate([data[:, :, :, 1:], data[:, :, :, (0 if mode == "periodic" else -1):][:, :, :, :1]], 3)
+    want = np.concatenate([data[:-1], nxt[:-1], data[1:], nxt[1:]], axis=-1)
+    np.testing.assert_array_equal(rec, want)
+    with pytest.raises(ValueError):
+        jt.StructuredTimeSeriesField(data, np.array([0, 0.5, 1], F32), gx, gy, gz, storage="bfloat16",
+                                     pair_records=True)
+    assert jt.StructuredTimeSeriesField(data, np.array([0, 0.5, 1], F32), gx, gy, gz, bounds_handling=mode,
+                                        pair_records=False)._desc.pair_data is None
+
+
+@pytest.mark.parametrize("pairs", [True, False])
+@pytest.mark.parametrize("span,n", [((0.0, 0.5), 51), ((0.9, 1.15), 26)])
+def test_pair_records_tracking_bitwise(gpu, pairs, span, n):
+    """RK4 on the periodic fast path with and without the pair records, over
+    snapshot changes (0 .. 0.5) and out past the last snapshot time (0.9 ..
+    1.15: steps that blend (s, s) or extrapolate take the per-snapshot
+    layout, the others the records): both bitwise equal to the oracle at
+    every step, with the cell sort on."""
+    data, times, g, x0 = _cfg1(3000)
+    f = jt.StructuredTimeSeriesField(data, times, g, g, g, bounds_handling="periodic", pair_records=pairs)
+    assert (f._desc.pair_data is not None) == pairs
+    bc = jt.periodic_boundary([[0, 0, 0], [1, 1, 1]])
+    tr = jt.create_tracker("rk4", f, bc, sort_every=5, steps_per_launch=5)
+    traj = tr.track_particles(x0, span, n)
+    meta = OF.GridMeta(g, g, g)
+    field_fn = lambda p, t: OF.structured_series_sample_corners(data, times, meta, p, t, "periodic")
+    tt, pos = OT.track(x0, span, n, field_fn, lambda p: OB.per_axis(p, [[0, 0, 0], [1, 1, 1]], ["periodic"] * 3),
+                       "rk4")
+    np.testing.assert_array_equal(traj.positions, pos)

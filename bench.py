@@ -100,7 +100,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--particles", type=int, default=N_PARTICLES)
     ap.add_argument("--steps-per-launch", type=int, default=50)
-    ap.add_argument("--sort-every", type=int, default=25)
+    ap.add_argument("--sort-every", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
@@ -846,9 +846,13 @@ def main():
                                "x periodic / y,z reflective, FP32 field",
                    "particles": n, "grid": [GRID_N] * 3, "snapshots": N_SNAP, "dt": DT,
                    "steps_per_launch": spl, "cell_sort_every": args.sort_every,
-                   "snapshot_layout": "z-halo, 4 components per node"},
+                   "snapshot_layout": "z-halo, 4 components per node" + (
+                       ", and consecutive-pair records (both snapshots' z-pairs of a node in 48 B)"
+                       if field32._desc.pair_data else "")},
         "roofline": hbm_roofline(
-            "k_track_rk4_f32", n * steps_per, avg_launch_ms, kernel="k_track<RK4,f32x4,z-halo,fast path>",
+            "k_track_rk4_f32", n * steps_per, avg_launch_ms,
+            kernel="k_track<RK4,f32,pair records,fast path>" if field32._desc.pair_data else
+            "k_track<RK4,f32x4,z-halo,fast path>",
             launch_ms=avg_launch_ms, call_ms_with_cell_sort=avg_call_ms, particle_steps_per_launch=n * steps_per,
             gather_model={"bytes_per_particle_step": bpps32, "GBs": gather32,
                           "note": "24 B of state + 4 stages x 2 snapshots x 8 corners x 12 B: the corner "

@@ -40,7 +40,7 @@ class TrackerOptions:
       return_device      Trajectory arrays stay torch tensors on the GPU
       steps_per_launch   steps fused into one kernel launch (0 = library default)
       sort_every         re-sort particles by grid cell every k steps (0 = off;
-                         None = from 65536 particles, every 25 steps on
+                         None = from 65536 particles, every 50 steps on
                          structured fields and every 10 on octree fields)
       output_path        stream the recorded frames into this .npy file,
                          (frames, N, 3) float32, while tracking runs; the
@@ -143,7 +143,7 @@ class ParticleTracker:
         se = self.options.sort_every
         if se is None:
             kind = getattr(self.field, "kind", "grid")
-            se = 0 if n < 65536 else {"grid": 25, "octree": 10}.get(kind, 0)
+            se = 0 if n < 65536 else {"grid": 50, "octree": 10}.get(kind, 0)
         o.sort_every = int(se)
         o.block = 0
         return o

@@ -346,7 +346,21 @@ typedef struct jt_octree_field {
     const void*  data;                /* (n_snap, n_points, 3) f32 */
     const float* times;               /* (n_snap,) f32             */
     int32_t n_snap, time_interp, time_extrap;
+    const float* element_records;     /* optional (n_elems, 24) f32, 16-byte
+                                         aligned, as jt_octree_element_records
+                                         writes them: per element its box (min
+                                         xyz, max xyz), the xyz of its 4 vertices,
+                                         its 4 point ids (int32 bits), 2 zeros.
+                                         The leaf scan then reads one contiguous
+                                         96-byte record per candidate instead of
+                                         box, connectivity and 4 vertex rows.
+                                         NULL: not used.                        */
 } jt_octree_field;
+
+/* The per-element records of field (element_records layout above; the
+ * field's element_records is ignored) into out, a device array of
+ * n_elems * 24 floats, 16-byte aligned. */
+int jt_octree_element_records(const jt_octree_field* field, float* out, void* stream);
 
 /* Replaces OctreeFEMTimeSeriesFieldOptimized.sample_at_positions
  * (octree_fem_time_series_optimized.py:87-113).  found (nullable) receives 1

@@ -72,6 +72,7 @@ class OctreeField(C.Structure):
         ("max_elems", C.c_int32), ("max_depth", C.c_int32),
         ("data", C.c_void_p), ("times", C.c_void_p),
         ("n_snap", C.c_int32), ("time_interp", C.c_int32), ("time_extrap", C.c_int32),
+        ("element_records", C.c_void_p),
     ]
 
 
@@ -109,6 +110,7 @@ _SIGS = {
     "jt_grid_sample": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_int64, C.c_float,
                                  C.c_void_p, C.c_void_p]),
     "jt_grid_pair_records": (C.c_int, [C.POINTER(GridField), C.c_void_p, C.c_void_p]),
+    "jt_octree_element_records": (C.c_int, [C.POINTER(OctreeField), C.c_void_p, C.c_void_p]),
     "jt_apply_boundary": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p]),
     "jt_inlet_exits_workspace_size": (C.c_size_t, [C.c_int64]),
     "jt_inlet_exits": (C.c_int, [C.POINTER(Boundary), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,

@@ -31,6 +31,7 @@ struct OctDev {
     long long snap_stride;    // Np * 3
     const float* times;
     int T, interp, extrap;
+    const float4* erec;       // (M, 6) float4 element records, or nullptr
 };
 
 __device__ __forceinline__ float det3(const float c0[3], const float c1[3], const float c2[3]) {
@@ -53,14 +54,13 @@ __device__ __forceinline__ void blended_node(const OctDev& o, const Blend& bl, f
     corner<ST_F32x3>(b0, b1, bl, oma, node, v);
 }
 
-// Candidates fetched per round of the leaf scan: 1 measured fastest on the
-// 128^3 x 6 tet workload (3.4e9 particle-steps/s; 2: 3.3e9, 3: 2.5e9,
-// 4: 2.0e9, the larger batches paying in registers and spills).
-// Candidates per load round of the leaf scan.  Batches of 2-4 measured
-// slower (they pay in registers); the one-candidate rounds keep the batched
-// form, passes of predicated loads per dependency level, because the same
-// scan written as a plain loop with early continues compiled to slower code
-// (16.5 against 12.0 ms per 10-step launch of 2^22 particles).
+// Candidates fetched per round of the leaf scan (without element records):
+// 1 measured fastest on the 128^3 x 6 tet workload (3.4e9 particle-steps/s;
+// 2: 3.3e9, 3: 2.5e9, 4: 2.0e9, the larger batches paying in registers and
+// spills).  The one-candidate rounds keep the batched form, passes of
+// predicated loads per dependency level, because the same scan written as a
+// plain loop with early continues compiled to slower code (16.5 against
+// 12.0 ms per 10-step launch of 2^22 particles).
 constexpr int OCT_BATCH = 1;
 
 // point_in_tetrahedron_fast (:218-250): Cramer barycentrics of q in the
@@ -127,7 +127,7 @@ __device__ __forceinline__ void leaf_cache_reset(LeafCache& lc) { lc.node = -1; 
 // Returns 1 if an element was found (value in out), else 0 (fallback value in out).
 // STATS: st[0..2] += levels descended, candidates scanned, candidates whose
 // box contains q (the inputs of the point-location byte model).
-template <bool STATS = false, bool CACHE = false>
+template <bool STATS = false, bool CACHE = false, bool REC = false>
 __device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q[3], float out[3],
                                  int* st = nullptr, LeafCache* lc = nullptr) {
     const float oma = 1.0f - bl.a;
@@ -168,6 +168,41 @@ __device__ inline int oct_sample(const OctDev& o, const Blend& bl, const float q
     }
     const int* cand = o.nelem + (size_t)node * o.max_elems;
     const int cnt = o.ncount[node];
+    if constexpr (REC && !STATS) {
+        // the same first-hit scan over the element records: one contiguous
+        // 96-byte record per candidate (box, vertices, point ids), so a
+        // candidate costs one dependent round trip after its id (24.6 ->
+        // 23.0 ms per 20 steps of 2^22 particles on the 128^3 x 6 tet mesh;
+        // as a plain loop with early continues it took 34 ms, with two or
+        // three candidates per round 84 and 91 ms)
+        for (int k0 = 0; k0 < cnt; k0 += OCT_BATCH) {
+            int e[OCT_BATCH];
+            float4 r[OCT_BATCH][6];
+#pragma unroll
+            for (int j = 0; j < OCT_BATCH; ++j) e[j] = (k0 + j < cnt) ? cand[k0 + j] : -1;
+#pragma unroll
+            for (int j = 0; j < OCT_BATCH; ++j)
+                if (e[j] >= 0)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) r[j][c] = o.erec[(size_t)e[j] * 6 + c];
+#pragma unroll
+            for (int j = 0; j < OCT_BATCH; ++j) {
+                if (!(e[j] >= 0 && q[0] >= r[j][0].x && q[1] >= r[j][0].y && q[2] >= r[j][0].z &&
+                      q[0] <= r[j][0].w && q[1] <= r[j][1].x && q[2] <= r[j][1].y))
+                    continue;
+                const float v[4][3] = {{r[j][1].z, r[j][1].w, r[j][2].x}, {r[j][2].y, r[j][2].z, r[j][2].w},
+                                       {r[j][3].x, r[j][3].y, r[j][3].z}, {r[j][3].w, r[j][4].x, r[j][4].y}};
+                float b[4];
+                if (tet_bary(v, q, b)) {
+                    const int ni[4] = {__float_as_int(r[j][4].z), __float_as_int(r[j][4].w),
+                                       __float_as_int(r[j][5].x), __float_as_int(r[j][5].y)};
+                    tet_value(o, bl, ni, b, out);
+                    return 1;
+                }
+            }
+        }
+        return oct_fallback(o, bl, oma, cand, q, out);
+    }
     if constexpr (!STATS) {
         // :309-360, first hit wins.  Candidates are fetched OCT_BATCH at a
         // time: all their element ids, then all box bounds, then all
@@ -267,7 +302,7 @@ k_oct_sample(OctDev o, const float* __restrict__ x, int n, float t, float* __res
 // Fused multi-step tracking through the octree field; particles may be
 // cell-sorted (ids maps a slot to the caller's particle for trajectory rows).
 // Blocks map XCD-contiguously: block b takes chunk (b % 8) * (grid / 8) + b / 8.
-template <int INTEG>
+template <int INTEG, bool REC>
 __global__ void __launch_bounds__(256)
 k_oct_track(OctDev o, BcDev bc, float* __restrict__ xs, int n, const Blend* __restrict__ tab, int s0, int s1,
             float dt, float dth, float dt6, int rec_every, float* __restrict__ traj, const int* __restrict__ ids) {
@@ -279,7 +314,9 @@ k_oct_track(OctDev o, BcDev bc, float* __restrict__ xs, int n, const Blend* __re
     float p[3] = {xs[3 * (size_t)i], xs[3 * (size_t)i + 1], xs[3 * (size_t)i + 2]};
     LeafCache lc;
     leaf_cache_reset(lc);
-    auto f = [&](const Blend& b, const float q[3], float out[3]) { oct_sample<false, true>(o, b, q, out, nullptr, &lc); };
+    auto f = [&](const Blend& b, const float q[3], float out[3]) {
+        oct_sample<false, true, REC>(o, b, q, out, nullptr, &lc);
+    };
     for (int s = s0; s < s1; ++s) {
         rk_step<INTEG>(f, tab, s, p, dt, dth, dt6);
         apply_bc(bc, p);
@@ -291,6 +328,53 @@ k_oct_track(OctDev o, BcDev bc, float* __restrict__ xs, int n, const Blend* __re
     xs[3 * (size_t)i] = p[0];
     xs[3 * (size_t)i + 1] = p[1];
     xs[3 * (size_t)i + 2] = p[2];
+}
+
+template <bool REC>
+static void launch_oct_track_rec(int integrator, dim3 grid, dim3 block, hipStream_t s, const OctDev& o,
+                                 const BcDev& d, float* x, int n, const Blend* tab, int s0, int s1, float dt, float dth,
+                                 float dt6, int rec, float* traj, const int* ids) {
+    if (integrator == JT_RK4)
+        hipLaunchKernelGGL((k_oct_track<JT_RK4, REC>), grid, block, 0, s, o, d, x, n, tab, s0, s1, dt, dth, dt6, rec,
+                           traj, ids);
+    else if (integrator == JT_RK2)
+        hipLaunchKernelGGL((k_oct_track<JT_RK2, REC>), grid, block, 0, s, o, d, x, n, tab, s0, s1, dt, dth, dt6, rec,
+                           traj, ids);
+    else
+        hipLaunchKernelGGL((k_oct_track<JT_EULER, REC>), grid, block, 0, s, o, d, x, n, tab, s0, s1, dt, dth, dt6, rec,
+                           traj, ids);
+}
+
+// The tracking kernel over the element records when the field carries them.
+static void launch_oct_track(int integrator, bool rec_layout, dim3 grid, dim3 block, hipStream_t s, const OctDev& o,
+                             const BcDev& d, float* x, int n, const Blend* tab, int s0, int s1, float dt, float dth,
+                             float dt6, int rec, float* traj, const int* ids) {
+    if (rec_layout) launch_oct_track_rec<true>(integrator, grid, block, s, o, d, x, n, tab, s0, s1, dt, dth, dt6, rec,
+                                               traj, ids);
+    else launch_oct_track_rec<false>(integrator, grid, block, s, o, d, x, n, tab, s0, s1, dt, dth, dt6, rec, traj,
+                                     ids);
+}
+
+// Per-element records: box, vertices, point ids (the element_records layout).
+__global__ void __launch_bounds__(256)
+k_elem_records(const float* __restrict__ ebounds, const int* __restrict__ conn, const float* __restrict__ points,
+               int m, float4* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const float* b = ebounds + (size_t)e * 6;
+    int ni[4];
+    float v[4][3];
+    for (int c = 0; c < 4; ++c) {
+        ni[c] = conn[(size_t)e * 4 + c];
+        for (int a = 0; a < 3; ++a) v[c][a] = points[(size_t)ni[c] * 3 + a];
+    }
+    float4* r = out + (size_t)e * 6;
+    r[0] = make_float4(b[0], b[1], b[2], b[3]);
+    r[1] = make_float4(b[4], b[5], v[0][0], v[0][1]);
+    r[2] = make_float4(v[0][2], v[1][0], v[1][1], v[1][2]);
+    r[3] = make_float4(v[2][0], v[2][1], v[2][2], v[3][0]);
+    r[4] = make_float4(v[3][1], v[3][2], __int_as_float(ni[0]), __int_as_float(ni[1]));
+    r[5] = make_float4(__int_as_float(ni[2]), __int_as_float(ni[3]), 0.0f, 0.0f);
 }
 
 // Hilbert key of each particle's cell on a 1024^3 lattice over the root box.
@@ -324,12 +408,29 @@ static int make_oct(const jt_octree_field* f, OctDev& o) {
     o.max_elems = f->max_elems; o.max_depth = f->max_depth; o.n_elems = (int)f->n_elems;
     o.data = (const float*)f->data; o.snap_stride = f->n_points * 3;
     o.times = f->times; o.T = f->n_snap; o.interp = f->time_interp; o.extrap = f->time_extrap;
+    o.erec = (const float4*)f->element_records;
+    JT_REQUIRE(((uintptr_t)f->element_records & 15u) == 0, "element_records must be 16-byte aligned");
     return JT_OK;
 }
 
 }  // namespace jt
 
 using namespace jt;
+
+extern "C" int jt_octree_element_records(const jt_octree_field* field, float* out, void* stream) {
+    JT_REQUIRE(field != nullptr && out != nullptr, "NULL pointer");
+    JT_REQUIRE(((uintptr_t)out & 15u) == 0, "out must be 16-byte aligned");
+    jt_octree_field f = *field;
+    f.element_records = nullptr;
+    OctDev o;
+    int rc = make_oct(&f, o);
+    if (rc) return rc;
+    JT_REQUIRE(o.n_elems < (1 << 30), "too many elements");
+    hipLaunchKernelGGL(k_elem_records, dim3((unsigned)((o.n_elems + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       o.ebounds, o.conn, o.points, o.n_elems, (float4*)out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
 
 extern "C" int jt_octree_sample(const jt_octree_field* field, const float* x, int64_t n, float t, float* out,
                                 int32_t* found, void* stream) {
@@ -439,12 +540,8 @@ extern "C" int jt_track_octree(const jt_octree_field* field, const jt_boundary* 
                 cur_ids = nid;
             }
         }
-        if (integrator == JT_RK4)
-            hipLaunchKernelGGL(k_oct_track<JT_RK4>, grid, block, 0, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
-        else if (integrator == JT_RK2)
-            hipLaunchKernelGGL(k_oct_track<JT_RK2>, grid, block, 0, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
-        else
-            hipLaunchKernelGGL(k_oct_track<JT_EULER>, grid, block, 0, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6, record_every, traj, cur_ids);
+        launch_oct_track(integrator, o.erec != nullptr, grid, block, s, o, d, cur, (int)n, tab, s0, s1, dt, dth, dt6,
+                         record_every, traj, cur_ids);
         JT_HIP(hipGetLastError());
         s0 = s1;
     }

@@ -269,7 +269,8 @@ class OctreeFEMTimeSeriesFieldOptimized:
 
     def __init__(self, data, times, positions, connectivity, interpolation: str = "linear",
                  extrapolation: str = "constant", _source_info: Optional[dict] = None,
-                 max_elements_per_leaf: int = 32, max_depth: int = 12, device=None, build_on: str = "device"):
+                 max_elements_per_leaf: int = 32, max_depth: int = 12, device=None, build_on: str = "device",
+                 element_records: bool = True):
         lib = L.load()
         self.device = torch.device(device) if device is not None else L.device()
         data = np.ascontiguousarray(np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data, F32))
@@ -326,6 +327,16 @@ class OctreeFEMTimeSeriesFieldOptimized:
         d.time_interp = L.JT_TIME.get(interp, 0)
         d.time_extrap = L.JT_EXTRAP[extra]
         self._desc = d
+        # per-element records (box, vertices, point ids; 96 B per element)
+        # for the tracking kernels' leaf scan
+        self._erec = None
+        if element_records:
+            lib = L.load()
+            self._erec = torch.empty((M, 24), dtype=torch.float32, device=dev)
+            with torch.cuda.device(dev):
+                L.check(lib.jt_octree_element_records(C.byref(d), L.ptr(self._erec), L.stream_handle()),
+                        "jt_octree_element_records")
+            d.element_records = self._erec.data_ptr()
 
     def _build_on_host(self, pts, conn, max_per_leaf, max_depth):
         """jt_octree_build: the C++ host build, exported as NumPy arrays."""

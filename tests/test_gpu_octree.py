@@ -18,11 +18,12 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 
 
-def _field(n=8, T=4, leaf=8, depth=12, seed=0):
+def _field(n=8, T=4, leaf=8, depth=12, seed=0, element_records=True):
     pts, conn = W.kuhn_mesh(n, 0.2, seed)
     times = (np.arange(T) / 14.0).astype(F32)
     data = W.poiseuille_vortex(pts, times)
-    f = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn, max_elements_per_leaf=leaf, max_depth=depth)
+    f = jt.OctreeFEMTimeSeriesFieldOptimized(data, times, pts, conn, max_elements_per_leaf=leaf, max_depth=depth,
+                                             element_records=element_records)
     return f, pts, conn, data, times
 
 
@@ -74,8 +75,13 @@ def test_internal_node_fallback_uses_last_element(gpu):
     assert not found.any()
 
 
-def test_track_inlet_per_step_bitwise(gpu):
-    f, pts, conn, data, times = _field(n=10, T=8)
+@pytest.mark.parametrize("element_records", [True, False])
+def test_track_inlet_per_step_bitwise(gpu, element_records):
+    """Tracking through the octree field with the continuous grid inlet, per
+    step bitwise against the oracle, with the leaf scan over the per-element
+    records and over the separate box, connectivity and point arrays."""
+    f, pts, conn, data, times = _field(n=10, T=8, element_records=element_records)
+    assert (f._desc.element_records is not None) == element_records
     bounds = [[0, 0, 0], [1, 1, 1]]
     conc = {"x": 64, "y": 64, "z": 64}
     bc = jt.continuous_inlet_boundary_factory(bounds, "x", "positive", "grid", conc)
@@ -181,3 +187,15 @@ def test_device_build_full_size_matches_host(gpu):
     for k, v in fh.octree.items():
         np.testing.assert_array_equal(fd.octree[k], v, err_msg=k)
     assert fd.tree_depth == fh.tree_depth and len(fd.octree["nodes_min"]) > 100_000
+
+
+def test_element_records_layout(gpu):
+    """jt_octree_element_records: per element its box (min xyz, max xyz), the
+    xyz of its 4 vertices, its 4 point ids as int32 bits and 2 zeros."""
+    f, pts, conn, data, times = _field(n=5)
+    rec = f._erec.cpu().numpy()
+    eb = f.octree["element_bounds"].reshape(-1, 6)
+    np.testing.assert_array_equal(rec[:, :6], eb)
+    np.testing.assert_array_equal(rec[:, 6:18], pts[conn].reshape(-1, 12))
+    np.testing.assert_array_equal(rec[:, 18:22].view(np.int32), conn)
+    np.testing.assert_array_equal(rec[:, 22:], 0)

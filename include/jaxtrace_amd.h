@@ -241,7 +241,10 @@ typedef struct jt_hashgrid {
 } jt_hashgrid;
 
 /* Per-axis min then max of points (n, dim) into out_minmax (2*dim floats,
- * device).  ws: >= 24 bytes of device scratch. */
+ * device).  ws: device scratch of at least 24 bytes; each 24 bytes hold one
+ * block's partial result, and up to JT_BOUNDS_WS_BYTES are used (a smaller
+ * ws runs fewer blocks). */
+#define JT_BOUNDS_WS_BYTES 24576
 int jt_points_bounds(const float* P, int64_t n, int32_t dim, float* out_minmax,
                      void* ws, size_t ws_bytes, void* stream);
 int64_t jt_hashgrid_num_cells(const jt_hashgrid* g);

@@ -60,6 +60,7 @@ class HashGridDesc(C.Structure):
 
 
 JT_KEY_X32, JT_KEY_X64 = 0, 1
+JT_BOUNDS_WS_BYTES = 24576          # include/jaxtrace_amd.h JT_BOUNDS_WS_BYTES
 JT_SPH = {"cubic_spline": 0, "wendland": 1, "gaussian": 2}
 
 

@@ -1,10 +1,11 @@
 // Hash-grid neighbour search and SPH density (jaxtrace/density/neighbors.py,
 // jaxtrace/density/sph.py).
 //
-// Build: dense cell key per point -> stable LSD radix sort of (key, index)
-// (equal keys keep increasing point index, as the reference's stable argsort)
-// -> per-cell [start, end) table -> positions gathered into cell order, so the
-// candidates of one cell are contiguous in memory.
+// Build: a stable counting sort of the points by dense cell key, by passes of
+// at most 8 key bits (equal keys keep increasing point index, as the
+// reference's stable argsort); the last pass writes the order and the
+// positions in cell order, so the candidates of one cell are contiguous in
+// memory; then the per-cell [start, end) table.
 //
 // Query: one 64-lane wavefront per query point.  The 27 (2-D: 9) stencil
 // probes are visited in the reference's lexicographic order; lane s handles
@@ -13,7 +14,6 @@
 // Distances, validity (slot < count and d <= r) and the stable "first K by
 // distance" selection follow neighbors.py:153-165 bit for bit.
 #include "jt_common.h"
-#include <hipcub/hipcub.hpp>
 
 namespace jt {
 
@@ -47,30 +47,46 @@ __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
 }
 
 // Per-axis min and max of the points into out[0..2] / out[3..5] (ordered
-// uint32 encodings): each thread folds a grid-stride slice, each wave folds
-// its lanes with shuffles, the block folds its waves in LDS, and one thread
-// per block adds the result atomically (the six words share one cache line,
-// so the atomics are kept to a few per block).
-__global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ P, int n, int dim, unsigned* out) {
+// uint32 encodings): each thread folds a grid-stride slice (16-byte loads of
+// 4 points (3-D) or 2 points (2-D) when the array is 16-byte aligned), each
+// wave folds its lanes with shuffles, the block folds its waves in LDS and
+// writes its six words to out[6 * block ...]; k_decode_bounds folds the
+// blocks' words.  (Global atomics on the one shared cache line cost about
+// 20 ns each, serialised: with 1,024 blocks they took longer than the read.)
+__global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ P, int n, int dim, int vec, unsigned* out) {
     __shared__ unsigned red[4][6];
     unsigned lmin[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, lmax[3] = {0u, 0u, 0u};
+    auto fold = [&](int a, float v) {
+        const unsigned u = f2ord(v);
+        lmin[a] = min(lmin[a], u);
+        lmax[a] = max(lmax[a], u);
+    };
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
-    if (dim == 3) {
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const float* p = P + (size_t)i * 3;
-            const unsigned a = f2ord(p[0]), b = f2ord(p[1]), c = f2ord(p[2]);
-            lmin[0] = min(lmin[0], a); lmax[0] = max(lmax[0], a);
-            lmin[1] = min(lmin[1], b); lmax[1] = max(lmax[1], b);
-            lmin[2] = min(lmin[2], c); lmax[2] = max(lmax[2], c);
+    int done = 0;                                       // points covered by the vector loop
+    if (vec && dim == 3) {
+        const float4* P4 = reinterpret_cast<const float4*>(P);
+        const int ng = n >> 2;
+#pragma unroll 4
+        for (int gi = tid; gi < ng; gi += stride) {
+            const float4 u = P4[3 * gi], v = P4[3 * gi + 1], w = P4[3 * gi + 2];
+            fold(0, u.x); fold(1, u.y); fold(2, u.z); fold(0, u.w);
+            fold(1, v.x); fold(2, v.y); fold(0, v.z); fold(1, v.w);
+            fold(2, w.x); fold(0, w.y); fold(1, w.z); fold(2, w.w);
         }
-    } else {
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const float* p = P + (size_t)i * 2;
-            const unsigned a = f2ord(p[0]), b = f2ord(p[1]);
-            lmin[0] = min(lmin[0], a); lmax[0] = max(lmax[0], a);
-            lmin[1] = min(lmin[1], b); lmax[1] = max(lmax[1], b);
+        done = ng << 2;
+    } else if (vec && dim == 2) {
+        const float4* P4 = reinterpret_cast<const float4*>(P);
+        const int ng = n >> 1;
+#pragma unroll 4
+        for (int gi = tid; gi < ng; gi += stride) {
+            const float4 u = P4[gi];
+            fold(0, u.x); fold(1, u.y); fold(0, u.z); fold(1, u.w);
         }
+        done = ng << 1;
     }
+    for (int i = done + tid; i < n; i += stride)
+        for (int a = 0; a < dim; ++a) fold(a, P[(size_t)i * dim + a]);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         for (int o = 32; o > 0; o >>= 1) {
@@ -81,18 +97,40 @@ __global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ P, int
     if ((threadIdx.x & 63) == 0)
         for (int a = 0; a < 3; ++a) { red[wv][a] = lmin[a]; red[wv][3 + a] = lmax[a]; }
     __syncthreads();
-    if (threadIdx.x == 0)
-        for (int a = 0; a < dim; ++a) {
-            unsigned mn = red[0][a], mx = red[0][3 + a];
-            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { mn = min(mn, red[w][a]); mx = max(mx, red[w][3 + a]); }
-            atomicMin(&out[a], mn);
-            atomicMax(&out[3 + a], mx);
-        }
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        unsigned mn = red[0][a], mx = red[0][3 + a];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { mn = min(mn, red[w][a]); mx = max(mx, red[w][3 + a]); }
+        out[blockIdx.x * 6 + a] = mn;
+        out[blockIdx.x * 6 + 3 + a] = mx;
+    }
 }
 
-__global__ void k_decode_bounds(const unsigned* in, float* out, int dim) {
-    int a = threadIdx.x;
-    if (a < dim) { out[a] = ord2f(in[a]); out[dim + a] = ord2f(in[3 + a]); }
+// Fold the blocks' partial bounds (6 words each) and decode them to floats.
+__global__ void __launch_bounds__(256) k_decode_bounds(const unsigned* in, int nblocks, float* out, int dim) {
+    __shared__ unsigned red[4][6];
+    unsigned v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], in[b * 6 + a]);
+            v[3 + a] = max(v[3 + a], in[b * 6 + 3 + a]);
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], (unsigned)__shfl_xor((int)v[a], o));
+            v[3 + a] = max(v[3 + a], (unsigned)__shfl_xor((int)v[3 + a], o));
+        }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; ++k) red[wv][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < dim) {
+        const int a = threadIdx.x;
+        unsigned mn = red[0][a], mx = red[0][3 + a];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { mn = min(mn, red[w][a]); mx = max(mx, red[w][3 + a]); }
+        out[a] = ord2f(mn);
+        out[dim + a] = ord2f(mx);
+    }
 }
 
 // neighbors.py:70: floor((P - origin) * inv) as int32.
@@ -103,20 +141,21 @@ __device__ __forceinline__ int cell_coord(float p, float o, float inv) {
     return (int)f;
 }
 
-__global__ void __launch_bounds__(256)
-k_point_keys(HGDev g, const float* __restrict__ P, unsigned* __restrict__ keys, int* __restrict__ idx) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.n) return;
-    int c[3] = {0, 0, 0};
-    for (int a = 0; a < g.dim; ++a) c[a] = cell_coord(P[(size_t)i * g.dim + a], g.origin[a], g.inv);
-    unsigned k;
-    if (g.dim == 3)
-        k = (g.key_mode == JT_KEY_X64) ? (unsigned)(((long long)c[0] * g.dims[1] + c[1]) * g.dims[2] + c[2])
-                                       : (unsigned)((long long)c[1] * g.dims[2] + c[2]);
-    else
-        k = (g.key_mode == JT_KEY_X64) ? (unsigned)((long long)c[0] * g.dims[1] + c[1]) : (unsigned)c[1];
-    keys[i] = k;
-    idx[i] = i;
+// Dense cell id of a point (neighbors.py:70-91 with the reference's packing
+// replaced by a dense index over the grid's dims), clamped to the table for
+// inputs outside the bounds the grid was sized from.
+__device__ __forceinline__ unsigned hg_key(const HGDev& g, float x, float y, float z, unsigned nc) {
+    const int c0 = cell_coord(x, g.origin[0], g.inv), c1 = cell_coord(y, g.origin[1], g.inv);
+    long long k;
+    if (g.dim == 3) {
+        const int c2 = cell_coord(z, g.origin[2], g.inv);
+        k = (g.key_mode == JT_KEY_X64) ? ((long long)c0 * g.dims[1] + c1) * g.dims[2] + c2
+                                       : (long long)c1 * g.dims[2] + c2;
+    } else {
+        k = (g.key_mode == JT_KEY_X64) ? (long long)c0 * g.dims[1] + c1 : (long long)c1;
+    }
+    k = k < 0 ? 0 : k;
+    return (unsigned)(k < (long long)nc ? k : (long long)nc - 1);
 }
 
 __global__ void __launch_bounds__(256)
@@ -128,17 +167,220 @@ k_cell_table(const unsigned* __restrict__ skeys, int n, int* __restrict__ cstart
     if (i == n - 1 || skeys[i + 1] != k) cend[k] = i + 1;
 }
 
-// spos[i] = P[order[i]]: one 12-byte (3-D) or 8-byte (2-D) load and store
-// per point, so a random source record costs one memory request.
 struct alignas(4) Rec3 { float v[3]; };
 struct alignas(4) Rec2 { float v[2]; };
-__global__ void __launch_bounds__(256)
-k_gather_sorted(const float* __restrict__ P, const int* __restrict__ order, int n, int dim, float* __restrict__ spos) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int j = order[i];
-    if (dim == 3) reinterpret_cast<Rec3*>(spos)[i] = reinterpret_cast<const Rec3*>(P)[j];
-    else reinterpret_cast<Rec2*>(spos)[i] = reinterpret_cast<const Rec2*>(P)[j];
+
+// ---- build: stable LSD counting sort of the points by cell ------------------
+// The reference's stable argsort of the cell keys (neighbors.py:93-95), by
+// passes of at most 8 key bits (key bits = ceil(log2 n_cells)).  A pass:
+//   k_hg_count    per tile of HG_TILE points, the histogram of the digit;
+//   k_hg_scan_*   exclusive scan of the counts (digit-major, tile-minor), so
+//                 entry (d, t) is where tile t's points of digit d start;
+//   k_hg_scatter  per tile: each wave takes 512 consecutive points in
+//                 chunks of 64; ballots on the digit bits give each lane the
+//                 lanes with its digit, and a per-wave running count per digit
+//                 in LDS carries the rank from chunk to chunk; the waves'
+//                 counts are scanned per digit, the tile's records are
+//                 reordered by digit in LDS, and each digit's run is written
+//                 at its global offset with consecutive lanes on consecutive
+//                 records.
+// Points travel as (x, y, z, index) records and the key is recomputed from
+// the position in every pass (the same float32 operations), so no key array
+// is read.  The last pass writes order, the sorted positions and the sorted
+// keys (for the cell table) directly, so there is no separate gather.
+constexpr int HG_BLOCK = 512;
+constexpr int HG_ITEMS = 8;                           // points per thread
+constexpr int HG_TILE = HG_BLOCK * HG_ITEMS;          // 4,096 points a tile
+constexpr int HG_WAVES = HG_BLOCK / 64;
+constexpr int HG_WAVE_SPAN = HG_TILE / HG_WAVES;      // consecutive points a wave ranks
+constexpr int HG_SCAN_ITEMS = 8;
+constexpr int HG_SCAN_SPAN = HG_BLOCK * HG_SCAN_ITEMS;
+
+template <int DIM, bool FIRST>
+__device__ __forceinline__ float4 hg_load(const float* __restrict__ P, const float4* __restrict__ rin, int i) {
+    if constexpr (!FIRST) {
+        return rin[i];
+    } else if constexpr (DIM == 3) {
+        const Rec3 v = reinterpret_cast<const Rec3*>(P)[i];
+        return make_float4(v.v[0], v.v[1], v.v[2], __int_as_float(i));
+    } else {
+        const Rec2 v = reinterpret_cast<const Rec2*>(P)[i];
+        return make_float4(v.v[0], v.v[1], 0.0f, __int_as_float(i));
+    }
+}
+
+// Lanes whose digit equals this lane's (among the active lanes), from one
+// ballot per digit bit.
+__device__ __forceinline__ unsigned long long hg_match(unsigned d, int nbits, bool act) {
+    unsigned long long m = __ballot(act);
+    for (int b = 0; b < nbits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long s = __ballot(act && bit);
+        m &= bit ? s : ~s;
+    }
+    return m;
+}
+
+// Exclusive scan of one value per thread over the block; total = the sum.
+__device__ __forceinline__ unsigned hg_block_excl(unsigned v, unsigned* red, unsigned& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) red[wv] = x;
+    __syncthreads();
+    unsigned pre = 0, tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        const unsigned r = red[w];
+        pre += w < wv ? r : 0u;
+        tot += r;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + x - v;
+}
+
+template <int DIM, bool FIRST>
+__global__ void __launch_bounds__(HG_BLOCK)
+k_hg_count(HGDev g, const float* __restrict__ P, const float4* __restrict__ rin, int shift, int dbits, int ntiles,
+           unsigned nc, unsigned* __restrict__ hist) {
+    __shared__ unsigned h[256];
+    const int tile = blockIdx.x;
+    const unsigned dmask = (1u << dbits) - 1u;
+    for (int b = threadIdx.x; b < 256; b += HG_BLOCK) h[b] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int base = tile * HG_TILE + wv * HG_WAVE_SPAN;
+    float4 r[HG_ITEMS];
+#pragma unroll
+    for (int k = 0; k < HG_ITEMS; ++k) {
+        const int i = base + k * 64 + lane;
+        if (i < g.n) r[k] = hg_load<DIM, FIRST>(P, rin, i);
+    }
+#pragma unroll
+    for (int k = 0; k < HG_ITEMS; ++k) {
+        const bool act = base + k * 64 + lane < g.n;
+        const unsigned d = act ? (hg_key(g, r[k].x, r[k].y, r[k].z, nc) >> shift) & dmask : 0u;
+        const unsigned long long peers = hg_match(d, dbits, act);
+        if (act && (peers & ((1ull << lane) - 1ull)) == 0ull) atomicAdd(&h[d], (unsigned)__popcll(peers));
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < (1 << dbits); b += HG_BLOCK) hist[(size_t)b * ntiles + tile] = h[b];
+}
+
+__global__ void __launch_bounds__(HG_BLOCK) k_hg_scan_sums(const unsigned* __restrict__ in, int E,
+                                                            unsigned* __restrict__ bsum) {
+    __shared__ unsigned red[HG_BLOCK / 64];
+    const int lo = blockIdx.x * HG_SCAN_SPAN;
+    unsigned s = 0;
+    for (int k = 0; k < HG_SCAN_ITEMS; ++k) {
+        const int e = lo + k * HG_BLOCK + threadIdx.x;
+        if (e < E) s += in[e];
+    }
+    unsigned total;
+    hg_block_excl(s, red, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(HG_BLOCK) k_hg_scan_apply(unsigned* __restrict__ io, int E,
+                                                             const unsigned* __restrict__ bsum) {
+    __shared__ unsigned red[HG_BLOCK / 64];
+    unsigned pre = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += HG_BLOCK) pre += bsum[b];
+    unsigned before;
+    hg_block_excl(pre, red, before);                   // before = the earlier blocks' total
+    const int lo = blockIdx.x * HG_SCAN_SPAN + threadIdx.x * HG_SCAN_ITEMS;
+    unsigned v[HG_SCAN_ITEMS], t = 0;
+#pragma unroll
+    for (int k = 0; k < HG_SCAN_ITEMS; ++k) {
+        v[k] = lo + k < E ? io[lo + k] : 0u;
+        t += v[k];
+    }
+    unsigned total;
+    unsigned run = before + hg_block_excl(t, red, total);
+#pragma unroll
+    for (int k = 0; k < HG_SCAN_ITEMS; ++k) {
+        if (lo + k < E) io[lo + k] = run;
+        run += v[k];
+    }
+}
+
+template <int DIM, bool FIRST, bool LAST>
+__global__ void __launch_bounds__(HG_BLOCK)
+k_hg_scatter(HGDev g, const float* __restrict__ P, const float4* __restrict__ rin, int shift, int dbits, int ntiles,
+             unsigned nc, const unsigned* __restrict__ offs, float4* __restrict__ rout, int* __restrict__ order,
+             float* __restrict__ spos, unsigned* __restrict__ skeys) {
+    __shared__ float4 srec[HG_TILE];
+    __shared__ unsigned char sdig[HG_TILE];
+    __shared__ unsigned wcnt[HG_WAVES][256];
+    __shared__ unsigned bst[256], goff[256], red[HG_WAVES];
+    const int tile = blockIdx.x;
+    const int nb = 1 << dbits;
+    const unsigned dmask = (unsigned)nb - 1u;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int b = threadIdx.x; b < HG_WAVES * 256; b += HG_BLOCK) (&wcnt[0][0])[b] = 0u;
+    __syncthreads();
+    const int base = tile * HG_TILE + wv * HG_WAVE_SPAN;
+    float4 r[HG_ITEMS];
+    unsigned loc[HG_ITEMS];                            // digit << 16 | rank among the wave's points of the digit
+#pragma unroll
+    for (int k = 0; k < HG_ITEMS; ++k) {
+        const int i = base + k * 64 + lane;
+        if (i < g.n) r[k] = hg_load<DIM, FIRST>(P, rin, i);
+    }
+#pragma unroll
+    for (int k = 0; k < HG_ITEMS; ++k) {
+        const bool act = base + k * 64 + lane < g.n;
+        const unsigned d = act ? (hg_key(g, r[k].x, r[k].y, r[k].z, nc) >> shift) & dmask : 0u;
+        const unsigned long long peers = hg_match(d, dbits, act);
+        // every lane of the group reads the running count before its lowest
+        // lane advances it (one wave: LDS operations stay in program order)
+        const unsigned before = wcnt[wv][d];
+        loc[k] = (d << 16) | (before + (unsigned)__popcll(peers & lt));
+        if (act && (peers & lt) == 0ull) wcnt[wv][d] = before + (unsigned)__popcll(peers);
+    }
+    __syncthreads();
+    unsigned tot = 0;
+    if ((int)threadIdx.x < nb)
+        for (int w = 0; w < HG_WAVES; ++w) {
+            const unsigned c = wcnt[w][threadIdx.x];
+            wcnt[w][threadIdx.x] = tot;
+            tot += c;
+        }
+    unsigned all;
+    const unsigned st = hg_block_excl((int)threadIdx.x < nb ? tot : 0u, red, all);
+    if ((int)threadIdx.x < nb) {
+        bst[threadIdx.x] = st;
+        goff[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + tile];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < HG_ITEMS; ++k) {
+        if (base + k * 64 + lane < g.n) {
+            const unsigned d = loc[k] >> 16;
+            const unsigned p = bst[d] + wcnt[wv][d] + (loc[k] & 0xffffu);
+            srec[p] = r[k];
+            sdig[p] = (unsigned char)d;
+        }
+    }
+    __syncthreads();
+    const int cnt = min(HG_TILE, g.n - tile * HG_TILE);
+    for (int j = threadIdx.x; j < cnt; j += HG_BLOCK) {
+        const unsigned d = sdig[j];
+        const unsigned dst = goff[d] + (unsigned)j - bst[d];
+        const float4 v = srec[j];
+        if constexpr (LAST) {
+            order[dst] = __float_as_int(v.w);
+            if constexpr (DIM == 3) reinterpret_cast<Rec3*>(spos)[dst] = Rec3{{v.x, v.y, v.z}};
+            else reinterpret_cast<Rec2*>(spos)[dst] = Rec2{{v.x, v.y}};
+            skeys[dst] = hg_key(g, v.x, v.y, v.z, nc);
+        } else {
+            rout[dst] = v;
+        }
+    }
 }
 
 // Decode a probed cell the way the reference's packed key would match a
@@ -434,24 +676,50 @@ extern "C" int jt_points_bounds(const float* P, int64_t n, int32_t dim, float* o
     JT_REQUIRE(P && out_minmax, "NULL pointer");
     JT_REQUIRE(ws && ws_bytes >= 6 * sizeof(unsigned), "workspace too small");
     hipStream_t s = (hipStream_t)stream;
-    unsigned* acc = (unsigned*)ws;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, acc, 3, 0xffffffffu);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, acc + 3, 3, 0u);
-    int blocks = (int)((n + 255) / 256);
-    blocks = blocks > 512 ? 512 : blocks;
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, s, P, (int)n, dim, acc);
-    hipLaunchKernelGGL(k_decode_bounds, dim3(1), dim3(64), 0, s, acc, out_minmax, dim);
+    unsigned* part = (unsigned*)ws;                     // 6 words per block
+    long long blocks = (n + 1023) / 1024;
+    blocks = blocks < 1024 ? blocks : 1024;
+    const long long fit = (long long)(ws_bytes / (6 * sizeof(unsigned)));
+    blocks = blocks < fit ? blocks : fit;
+    const int vec = ((uintptr_t)P & 15) == 0;
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)blocks), dim3(256), 0, s, P, (int)n, dim, vec, part);
+    hipLaunchKernelGGL(k_decode_bounds, dim3(1), dim3(256), 0, s, (const unsigned*)part, (int)blocks, out_minmax, dim);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }
 
+static int hg_scan_blocks(long long E) { return (int)((E + HG_SCAN_SPAN - 1) / HG_SCAN_SPAN); }
+
 extern "C" size_t jt_hashgrid_workspace_size(int64_t n) {
-    size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (const int*)nullptr, (int*)nullptr, (int)n, 0, 32);
+    const size_t nn = n > 0 ? (size_t)n : 1;
+    const size_t ntiles = (nn + HG_TILE - 1) / HG_TILE;
     Arena a(nullptr, 0);
-    a.take<unsigned>(n); a.take<unsigned>(n); a.take<int>(n); a.take<char>(temp);
+    a.take<float4>(nn); a.take<float4>(nn);             // records between passes (two for 3-4 passes)
+    a.take<unsigned>(nn);                               // sorted keys
+    a.take<unsigned>(256 * ntiles);                     // digit counts, then offsets
+    a.take<unsigned>(hg_scan_blocks(256LL * ntiles));   // scan block sums
     return a.used + 256;
+}
+
+template <int DIM>
+static void hg_pass(hipStream_t s, const HGDev& g, const float* P, const float4* rin, float4* rout, bool first,
+                    bool last, int shift, int db, int ntiles, unsigned nc, unsigned* hist, unsigned* bsum,
+                    int* order, float* spos, unsigned* skeys) {
+    const dim3 b(HG_BLOCK), gt((unsigned)ntiles);
+    if (first) hipLaunchKernelGGL((k_hg_count<DIM, true>), gt, b, 0, s, g, P, rin, shift, db, ntiles, nc, hist);
+    else hipLaunchKernelGGL((k_hg_count<DIM, false>), gt, b, 0, s, g, P, rin, shift, db, ntiles, nc, hist);
+    const int E = (1 << db) * ntiles;
+    const dim3 gs((unsigned)hg_scan_blocks(E));
+    hipLaunchKernelGGL(k_hg_scan_sums, gs, b, 0, s, (const unsigned*)hist, E, bsum);
+    hipLaunchKernelGGL(k_hg_scan_apply, gs, b, 0, s, hist, E, (const unsigned*)bsum);
+#define JT_HG_SCATTER(F, L) \
+    hipLaunchKernelGGL((k_hg_scatter<DIM, F, L>), gt, b, 0, s, g, P, rin, shift, db, ntiles, nc, \
+                       (const unsigned*)hist, rout, order, spos, skeys)
+    if (first && last) JT_HG_SCATTER(true, true);
+    else if (first) JT_HG_SCATTER(true, false);
+    else if (last) JT_HG_SCATTER(false, true);
+    else JT_HG_SCATTER(false, false);
+#undef JT_HG_SCATTER
 }
 
 extern "C" int jt_hashgrid_build(const jt_hashgrid* h, const float* P, void* ws, size_t ws_bytes, void* stream) {
@@ -459,29 +727,40 @@ extern "C" int jt_hashgrid_build(const jt_hashgrid* h, const float* P, void* ws,
     int rc = make_hg(h, g);
     if (rc) return rc;
     JT_REQUIRE(P && h->order && h->sorted_pos && h->cell_start && h->cell_end, "NULL pointer");
+    JT_REQUIRE(h->n_points < (1LL << 31) - HG_TILE, "n_points out of range");
     hipStream_t s = (hipStream_t)stream;
     const int n = g.n;
     const long long nc = jt_hashgrid_num_cells(h);
-    size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (const int*)nullptr, (int*)nullptr, n, 0, 32);
+    const int ntiles = (n + HG_TILE - 1) / HG_TILE;
     Arena a(ws, ws_bytes);
-    unsigned* keys = a.take<unsigned>(n);
+    float4* ra = a.take<float4>(n);
+    float4* rb = a.take<float4>(n);
     unsigned* skeys = a.take<unsigned>(n);
-    int* idx = a.take<int>(n);
-    char* tmp = a.take<char>(temp);
-    if (!keys || !skeys || !idx || !tmp || !a.ok())
+    unsigned* hist = a.take<unsigned>((size_t)256 * ntiles);
+    unsigned* bsum = a.take<unsigned>(hg_scan_blocks(256LL * ntiles));
+    if (!ra || !rb || !skeys || !hist || !bsum || !a.ok())
         return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", jt_hashgrid_workspace_size(n), ws_bytes);
-    int end_bit = 1;
-    while (end_bit < 32 && (1LL << end_bit) < nc) ++end_bit;
-    dim3 b(256), gr((n + 255) / 256);
-    hipLaunchKernelGGL(k_point_keys, gr, b, 0, s, g, P, keys, idx);
-    JT_HIP(hipGetLastError());
-    JT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, skeys, idx, h->order, n, 0, end_bit, s));
+    int bits = 1;                                      // key bits: the cell ids are < nc <= 2^bits
+    while (bits < 31 && (1LL << bits) < nc) ++bits;
+    const int npass = (bits + 7) / 8;
+    const int dbits = (bits + npass - 1) / npass;
+    const float4* rin = nullptr;
+    for (int p = 0; p < npass; ++p) {
+        const int shift = p * dbits;
+        const int db = dbits < bits - shift ? dbits : bits - shift;
+        float4* rout = (p & 1) ? rb : ra;
+        if (g.dim == 3)
+            hg_pass<3>(s, g, P, rin, rout, p == 0, p == npass - 1, shift, db, ntiles, (unsigned)nc, hist, bsum,
+                       h->order, h->sorted_pos, skeys);
+        else
+            hg_pass<2>(s, g, P, rin, rout, p == 0, p == npass - 1, shift, db, ntiles, (unsigned)nc, hist, bsum,
+                       h->order, h->sorted_pos, skeys);
+        rin = rout;
+    }
+    const dim3 b(256), gr((n + 255) / 256);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nc + 255) / 256)), b, 0, s, (unsigned*)h->cell_start, (int)nc, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nc + 255) / 256)), b, 0, s, (unsigned*)h->cell_end, (int)nc, 0u);
-    hipLaunchKernelGGL(k_cell_table, gr, b, 0, s, skeys, n, h->cell_start, h->cell_end);
-    hipLaunchKernelGGL(k_gather_sorted, gr, b, 0, s, P, (const int*)h->order, n, g.dim, h->sorted_pos);
+    hipLaunchKernelGGL(k_cell_table, gr, b, 0, s, (const unsigned*)skeys, n, h->cell_start, h->cell_end);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

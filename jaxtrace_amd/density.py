@@ -54,8 +54,9 @@ class HashGridNeighbors:
         if self.N < 1:
             raise ValueError("HashGridNeighbors needs at least one point")
         mm = torch.empty(2 * self.D, dtype=torch.float32, device=self.device)
-        ws = torch.empty(64, dtype=torch.uint8, device=self.device)
-        L.check(lib.jt_points_bounds(L.ptr(P), self.N, self.D, L.ptr(mm), L.ptr(ws), 64, L.stream_handle()),
+        ws = torch.empty(L.JT_BOUNDS_WS_BYTES, dtype=torch.uint8, device=self.device)
+        L.check(lib.jt_points_bounds(L.ptr(P), self.N, self.D, L.ptr(mm), L.ptr(ws), L.JT_BOUNDS_WS_BYTES,
+                                     L.stream_handle()),
                 "jt_points_bounds")
         mm = mm.cpu().numpy()
         self.origin = mm[:self.D].astype(F32)                              # :68

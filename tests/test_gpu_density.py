@@ -9,6 +9,7 @@ import pytest
 import torch
 
 import jaxtrace_amd as jt
+from jaxtrace_amd import _lib as L
 from jaxtrace_amd import workloads as W
 from oracle import density as OD
 
@@ -29,6 +30,59 @@ def test_build_order_and_cells(gpu, key_mode):
     assert int(cnt.sum()) == len(P) and int((cnt > 0).sum()) == o.unique_ids.size
     np.testing.assert_array_equal(np.sort(cnt[cnt > 0]), np.sort(o.counts))
     np.testing.assert_array_equal(np.sort(cs[cnt > 0]), o.starts)
+
+
+# (points, dim, cell size, key mode): the counting sort's 1, 2, 3 and 4
+# passes of <= 8 key bits, one point, ragged last tiles of 4,096 points.
+BUILD_CASES = [
+    (1, 3, 2.0 / 128, "x32"),
+    (4095, 2, 1.0 / 64, "x32"),          # 6 key bits: one pass
+    (4097, 3, 2.0 / 128, "x32"),         # 12 bits: two passes
+    (3 * 4096 + 17, 3, 1.0 / 64, "x64"), # 18 bits: three passes
+    (300_001, 3, 1.0 / 128, "x64"),      # 21 bits: three passes
+    (200_003, 2, 1.0 / 20000, "x64"),    # 29 bits: four passes
+    (1 << 20, 3, 2.0 / 256, "x32"),      # the density workload's cell size
+]
+
+
+@pytest.mark.parametrize("n,dim,h,key_mode", BUILD_CASES)
+def test_build_counting_sort(gpu, n, dim, h, key_mode):
+    P = W.gaussian_mixture(n, seed=11)[:, :dim].copy()
+    g = jt.HashGridNeighbors(P, h, 64, key_mode)
+    o = OD.HashGrid(P, h, 64, key_mode)
+    order = g.pt_indices_sorted.cpu().numpy()
+    np.testing.assert_array_equal(order, o.order)
+    np.testing.assert_array_equal(g.sorted_positions.cpu().numpy(), P[o.order])
+    # dense cell ids of the oracle's present cells: their [start, end) rows
+    C = np.floor((P - o.origin) * o.inv).astype(np.int64) - o.Cmin
+    dims = [int(v) for v in g.dims]
+    if dim == 3:
+        dense = (C[:, 0] * dims[1] + C[:, 1]) * dims[2] + C[:, 2] if key_mode == "x64" else C[:, 1] * dims[2] + C[:, 2]
+    else:
+        dense = C[:, 0] * dims[1] + C[:, 1] if key_mode == "x64" else C[:, 1]
+    ids = torch.from_numpy(dense[o.order][o.starts]).to(g.cell_start.device)
+    np.testing.assert_array_equal(g.cell_start[ids].cpu().numpy(), o.starts)
+    np.testing.assert_array_equal(g.cell_end[ids].cpu().numpy(), o.starts + o.counts)
+    cnt = g.cell_end.long() - g.cell_start.long()
+    assert int(cnt.sum()) == n and int((cnt != 0).sum()) == o.unique_ids.size
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("n,offset", [(1, 0), (5, 1), (4095, 0), (70_001, 1), (1 << 20, 0), (1 << 20, 1)])
+def test_points_bounds(gpu, dim, n, offset):
+    """jt_points_bounds on 16-byte aligned arrays (vector loads and a tail)
+    and on a view one point into its buffer (the scalar loads)."""
+    P = W.gaussian_mixture(n + offset, seed=13)[:, :dim].copy()
+    Pd = torch.from_numpy(P).to(gpu)[offset:]
+    assert Pd.is_contiguous()
+    lib = L.load()
+    mm = torch.empty(2 * dim, dtype=torch.float32, device=gpu)
+    ws = torch.empty(L.JT_BOUNDS_WS_BYTES, dtype=torch.uint8, device=gpu)
+    L.check(lib.jt_points_bounds(L.ptr(Pd), n, dim, L.ptr(mm), L.ptr(ws), L.JT_BOUNDS_WS_BYTES,
+                                 L.stream_handle()), "jt_points_bounds")
+    got = mm.cpu().numpy()
+    np.testing.assert_array_equal(got[:dim], P[offset:].min(0))
+    np.testing.assert_array_equal(got[dim:], P[offset:].max(0))
 
 
 @pytest.mark.parametrize("key_mode", ["x32", "x64"])

@@ -11,9 +11,9 @@
 // corner values) and adding  coef_snap * W_corner * abar  into dL/dU.
 //
 // The dL/dU scatter is reduced in three levels, because atomics are the cost
-// that dominates (measured on the whole chip: about 4 T lane-ops/s for LDS
-// ds_add_f64 but only 0.2 T/s for ds_add_f32, and about 21 G/s for scattered
-// global_atomic_add_f32):
+// that dominates (measured on the whole chip with random addresses: about
+// 1.8 T lane-ops/s for LDS ds_add_f64 but only 0.2 T/s for ds_add_f32 and
+// 5.7 T/s for ds_add_u32, and about 21 G/s for scattered global_atomic_add_f32):
 //   1. registers: a thread sums its contributions for one cell (the 8 corner
 //      nodes) and one snapshot pair while its stage points stay in that cell,
 //      and flushes the 48 sums only when the cell changes;
@@ -246,8 +246,9 @@ __device__ __forceinline__ void run_flush(const GridDev& g, const RunAcc& r, con
 // seg[(s - sa) * n + t] (thread order: coalesced stores).
 // PF: the tracking kernel's fast path (periodic sampler with exact reciprocal
 // spacing, z-halo layout, per-axis boundary kinds) with its per-lane corner
-// reuse: the same values as the forward run.
-template <int ST, bool ZH, bool PF = false>
+// reuse (PR: over the consecutive-pair records): the same values as the
+// forward run.
+template <int ST, bool ZH, bool PF = false, bool PR = false>
 __global__ void __launch_bounds__(256, ADJ_REC_WAVES)
 k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const int* __restrict__ perm,
                 const Blend* __restrict__ tab, int sa, int sb, float dt, float dth, float dt6,
@@ -266,7 +267,8 @@ k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const
         float* o = seg + ((size_t)(s - sa) * n + t) * 3;
         o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
         if constexpr (PF) {
-            fast_step<JT_RK4, ST>(g, bc, tab, s, x, dt, dth, dt6, cc);
+            if constexpr (PR) fast_step_pair<JT_RK4>(g, bc, tab, s, x, dt, dth, dt6, cc);
+            else fast_step<JT_RK4, ST>(g, bc, tab, s, x, dt, dth, dt6, cc);
         } else {
             rk_step<JT_RK4>(f, tab, s, x, dt, dth, dt6);
             apply_bc(bc, x);
@@ -275,9 +277,10 @@ k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const
 }
 
 // Reverse steps s_hi-1 .. s_lo of the segment starting at sa.  Thread t takes
-// segment slot slot = order[t] (slots sorted by their cell at step s_lo, so a
-// block's particles start in a small region), particle pid = seg_pid[slot];
-// its adjoint lives in lam_io[pid] between launches.
+// segment slot slot = order[t] (slots sorted by their cell at the middle step
+// s_m = (s_lo + s_hi) / 2, so a block's particles stay in a small region
+// around their middle-step cells while the kernel runs), particle
+// pid = seg_pid[slot]; its adjoint lives in lam_io[pid] between launches.
 template <int ST, bool ZH, bool PF = false>
 __global__ void __launch_bounds__(ADJ_BLOCK, ADJ_MINW)
 k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const int* __restrict__ seg_pid,
@@ -291,11 +294,12 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     if (lb * ADJ_BLOCK >= n) return;                   // whole block past the end (block-uniform)
     const int t = lb * ADJ_BLOCK + threadIdx.x;
     const bool live = t < n;                           // every other thread stays: the block synchronises
+    const int s_m = (s_lo + s_hi) >> 1;               // the box's step (a measured 5 % faster than s_lo)
     const int slot = live ? order[t] : order[0];
     const int nn[3] = {g.nx, g.ny, g.nz};
     // box: offsets (wrapped to half a grid each way, so a block that meets a
-    // periodic face stays compact) from the start cell m of the block's
-    // middle particle, spanning the block's start cells, widened by the
+    // periodic face stays compact) from the step-s_m cell m of the block's
+    // middle particle, spanning the block's step-s_m cells, widened by the
     // stencil (one node above) and ADJ_MARGIN cells each way.  (Spanning every
     // step's cells of the sub-segment instead measured slower: the larger
     // boxes outgrow the LDS and are clamped.)
@@ -310,7 +314,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
     const int mid = (min(n - 1, lb * ADJ_BLOCK + ADJ_BLOCK / 2) & (ADJ_BLOCK - 1));
     if ((int)threadIdx.x == mid) {
         int c[3];
-        cell_of(seg + ((size_t)(s_lo - sa) * n + slot) * 3, c);
+        cell_of(seg + ((size_t)(s_m - sa) * n + slot) * 3, c);
         for (int d = 0; d < 3; ++d) sred[NW * 6 + d] = c[d];
     }
     for (int e = threadIdx.x; e < 2 * TILE_NODES * 3 + 3; e += blockDim.x) tile[e] = 0.0;
@@ -321,7 +325,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
         int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
         if (live) {
             int c[3];
-            cell_of(seg + ((size_t)(s_lo - sa) * n + slot) * 3, c);
+            cell_of(seg + ((size_t)(s_m - sa) * n + slot) * 3, c);
             for (int d = 0; d < 3; ++d) lo[d] = hi[d] = wrap_off(c[d] - tl.m[d], nn[d]);
         }
         for (int d = 0; d < 3; ++d) {
@@ -658,6 +662,7 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
     dim3 badj(ADJ_BLOCK), gadj((unsigned)((((n + ADJ_BLOCK - 1) / ADJ_BLOCK) + 7) / 8 * 8));
     const bool zh = g.nzs != g.nz;
     const bool pf = zh && periodic_fast(g, st == ST_F32x3 ? 12 : (st == ST_F32x4 ? 16 : 8)) && bc_axes_only(d);
+    const bool pr = pf && st == ST_F32x4 && g.pair != nullptr;
     for (int c = n_seg - 1; c >= 0; --c) {
         const int sa = c * ckpt_every;
         const int sb = sa + ckpt_every < n_steps ? sa + ckpt_every : n_steps;
@@ -668,8 +673,12 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
             switch (st) {
             case ST_F32x3: hipLaunchKernelGGL((k_adj_recompute<ST_F32x3, true, true>), g256x, b256, 0, s, g, d, xc,
                                               (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
-            case ST_F32x4: hipLaunchKernelGGL((k_adj_recompute<ST_F32x4, true, true>), g256x, b256, 0, s, g, d, xc,
-                                              (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
+            case ST_F32x4:
+                if (pr) hipLaunchKernelGGL((k_adj_recompute<ST_F32x4, true, true, true>), g256x, b256, 0, s, g, d, xc,
+                                           (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg);
+                else hipLaunchKernelGGL((k_adj_recompute<ST_F32x4, true, true>), g256x, b256, 0, s, g, d, xc,
+                                        (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg);
+                break;
             case ST_BF16x4: hipLaunchKernelGGL((k_adj_recompute<ST_BF16x4, true, true>), g256x, b256, 0, s, g, d, xc,
                                                (int)n, (const int*)perm, tab, sa, sb, dt, dth, dt6, seg); break;
             default: hipLaunchKernelGGL((k_adj_recompute<ST_F16x4, true, true>), g256x, b256, 0, s, g, d, xc,
@@ -698,8 +707,8 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
             };
             fits(s_lo);
             while (s_lo > sa && s_hi - s_lo < R && fits(s_lo - 1)) --s_lo;
-            // order the segment slots by their cell at the start of this sub-segment
-            rc = bucket_order(s, g, seg + (size_t)(s_lo - sa) * n * 3, (int)n, shift, bits, cb, keys, counts, offs,
+            // order the segment slots by their cell at the sub-segment's middle step
+            rc = bucket_order(s, g, seg + (size_t)(((s_lo + s_hi) >> 1) - sa) * n * 3, (int)n, shift, bits, cb, keys, counts, offs,
                               tmp, temp, order);
             if (rc) return rc;
             if (pf) {

@@ -59,7 +59,7 @@ __device__ __forceinline__ float bc_deriv(int kind, float x, float lo, float hi)
 #define ADJ_BLOCK 512              // 8 waves: two per SIMD, as the registers allow
 #endif
 #ifndef ADJ_RESORT
-#define ADJ_RESORT 8
+#define ADJ_RESORT 16              // longest sub-segment: 16 measured fastest with the middle-step box (8: 5 % slower, 24: 3 %)
 #endif
 #ifndef ADJ_MARGIN
 #define ADJ_MARGIN 1               // the widest box margin tried, in cells each way

@@ -101,17 +101,60 @@ struct Stencil {
     float wx[2], wy[2], wz[2];   // corner weight of (ci, cj, ck) = wx[ci] * wy[cj] * wz[ck]
 };
 
+// gather_corners<ST_F32x4, true, true> of an 'interp' blend between snapshots
+// (i0, i0 + 1) from the consecutive-pair records: a cell row of both
+// snapshots is 12 consecutive floats (three 16-byte loads), the blend is
+// blend_n's 'interp' arithmetic.
+__device__ __forceinline__ void gather_pair(const GridDev& g, const Blend& bl, float px, float py, float pz,
+                                            Corners& C) {
+    C.valid = true;
+    int i0, i1, j0, j1, k0, k1;
+    axis_coord_pf(px, g.o[0], g.inv[0], g.nx, (float)g.nx, i0, i1, C.wx);
+    axis_coord_pf(py, g.o[1], g.inv[1], g.ny, (float)g.ny, j0, j1, C.wy);
+    axis_coord_pf(pz, g.o[2], g.inv[2], g.nz, (float)g.nz, k0, k1, C.wz);
+    const char* pp = (const char*)(g.pair + (size_t)bl.i0 * (size_t)g.pair_stride);
+    const unsigned rs = (unsigned)g.nz * 48u;
+    const unsigned r0 = (unsigned)__umul24((unsigned)i0, (unsigned)g.ny);
+    const unsigned r1 = (unsigned)__umul24((unsigned)i1, (unsigned)g.ny);
+    const unsigned kb = (unsigned)k0 * 48u;
+    const unsigned row[4] = {(unsigned)__umul24(r0 + (unsigned)j0, rs) + kb, (unsigned)__umul24(r0 + (unsigned)j1, rs) + kb,
+                             (unsigned)__umul24(r1 + (unsigned)j0, rs) + kb, (unsigned)__umul24(r1 + (unsigned)j1, rs) + kb};
+    float4 f[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float4* qq = (const float4*)(pp + row[r]);
+        f[r][0] = qq[0]; f[r][1] = qq[1]; f[r][2] = qq[2];
+    }
+    float A[8][3], B[8][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        A[2 * r][0] = f[r][0].x; A[2 * r][1] = f[r][0].y; A[2 * r][2] = f[r][0].z;
+        A[2 * r + 1][0] = f[r][0].w; A[2 * r + 1][1] = f[r][1].x; A[2 * r + 1][2] = f[r][1].y;
+        B[2 * r][0] = f[r][1].z; B[2 * r][1] = f[r][1].w; B[2 * r][2] = f[r][2].x;
+        B[2 * r + 1][0] = f[r][2].y; B[2 * r + 1][1] = f[r][2].z; B[2 * r + 1][2] = f[r][2].w;
+    }
+    const float al = bl.a, oma = 1.0f - bl.a;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) C.v[q][c] = oma * A[q][c] + al * B[q][c];
+}
+
 // u(q) with the forward sampler's exact arithmetic, plus its Jacobian
 // J[c*3 + ax] = du_c/dx_ax from the same eight blended corner values: the
 // trilinear weights' derivatives are +-1/d per axis (0 where a non-periodic
 // axis is clipped).  depends = false where u is the constant 0 of the 'zero'
 // bounds mode.
-// PF: the periodic fast path's index math.
-template <int ST, bool ZH, bool PF = false>
+// PF: the periodic fast path's index math.  PR: a float32 field with
+// consecutive-pair records: an 'interp' blend between snapshots (s, s + 1)
+// gathers each cell row of both snapshots as three 16-byte loads from the
+// records (gather_pair) instead of four; the values are the same.
+template <int ST, bool ZH, bool PF = false, bool PR = false>
 __device__ __forceinline__ void sample_jac(const GridDev& g, const Blend& bl, const float q[3], float u[3],
                                            float J[9], bool& depends) {
     Corners C;
-    gather_corners<ST, ZH, PF>(g, bl, q[0], q[1], q[2], C);
+    if (PR && bl.mode == BL_INTERP && bl.i1 == bl.i0 + 1) gather_pair(g, bl, q[0], q[1], q[2], C);
+    else gather_corners<ST, ZH, PF>(g, bl, q[0], q[1], q[2], C);
     lerp_corners(g, C, u);
     depends = !(!C.valid && g.bmode == JT_BOUNDS_ZERO);
     float gd[3];
@@ -281,7 +324,7 @@ k_adj_recompute(GridDev g, BcDev bc, const float* __restrict__ x0s, int n, const
 // s_m = (s_lo + s_hi) / 2, so a block's particles stay in a small region
 // around their middle-step cells while the kernel runs), particle
 // pid = seg_pid[slot]; its adjoint lives in lam_io[pid] between launches.
-template <int ST, bool ZH, bool PF = false>
+template <int ST, bool ZH, bool PF = false, bool PR = false>
 __global__ void __launch_bounds__(ADJ_BLOCK, ADJ_MINW)
 k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const int* __restrict__ seg_pid,
               const Blend* __restrict__ tab, int sa, int s_lo, int s_hi, float dt, float dth, float dt6,
@@ -399,7 +442,7 @@ k_adj_reverse(GridDev g, BcDev bc, int n, const int* __restrict__ order, const i
                 const Blend b = r == 0 ? bl0 : (r == 3 ? bl2 : bl1);
                 float k[3], Jn[9];
                 bool dn;
-                sample_jac<ST, ZH, PF>(g, b, q, k, Jn, dn);
+                sample_jac<ST, ZH, PF, PR>(g, b, q, k, Jn, dn);
                 if (r == 3) {
                     float xp[3];
 #pragma unroll
@@ -716,9 +759,14 @@ extern "C" int jt_track_grid_adjoint(const jt_grid_field* field, const jt_bounda
                 case ST_F32x3: hipLaunchKernelGGL((k_adj_reverse<ST_F32x3, true, true>), gadj, badj, 0, s, g, d, (int)n,
                                                   (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth, dt6,
                                                   (const float*)seg, lam, grad_U, grad_x0); break;
-                case ST_F32x4: hipLaunchKernelGGL((k_adj_reverse<ST_F32x4, true, true>), gadj, badj, 0, s, g, d, (int)n,
-                                                  (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth, dt6,
-                                                  (const float*)seg, lam, grad_U, grad_x0); break;
+                case ST_F32x4:
+                    if (pr) hipLaunchKernelGGL((k_adj_reverse<ST_F32x4, true, true, true>), gadj, badj, 0, s, g, d,
+                                               (int)n, (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt,
+                                               dth, dt6, (const float*)seg, lam, grad_U, grad_x0);
+                    else hipLaunchKernelGGL((k_adj_reverse<ST_F32x4, true, true>), gadj, badj, 0, s, g, d, (int)n,
+                                            (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt, dth, dt6,
+                                            (const float*)seg, lam, grad_U, grad_x0);
+                    break;
                 case ST_BF16x4: hipLaunchKernelGGL((k_adj_reverse<ST_BF16x4, true, true>), gadj, badj, 0, s, g, d,
                                                    (int)n, (const int*)order, (const int*)perm, tab, sa, s_lo, s_hi, dt,
                                                    dth, dt6, (const float*)seg, lam, grad_U, grad_x0); break;

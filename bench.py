@@ -104,7 +104,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
     ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "hashgrid", "nearest", "seeding",
@@ -216,6 +216,116 @@ def cpu_baseline(host_data, times, grid, x0_host, n_part, n_steps):
     return n_part * n_steps / el, el
 
 
+# CPU baselines of the other rows are timed beside their GPU lines unless
+# --no-cpu-baseline is given (main sets this).
+CPU_BASELINES = {"on": True}
+
+
+def cpu_host_info():
+    """Host facts that go with every CPU baseline: the cores this process may
+    use, the machine's count, the CPU model and the NumPy thread pools."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    pools = None
+    try:
+        import threadpoolctl
+        pools = [{"api": d.get("internal_api"), "threads": d.get("num_threads")}
+                 for d in threadpoolctl.threadpool_info()]
+    except Exception:
+        pass
+    env = {k: os.environ[k] for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+           if k in os.environ}
+    return {"host_cpus": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "cpu_model": model,
+            "numpy_thread_pools": pools, "thread_env": env}
+
+
+def cpu_sph_baseline(P, Q, h):
+    """The oracle's hash grid (the reference's HashGridNeighbors build: keys,
+    stable argsort, CSR) on all particles, then its query_many and the cubic
+    spline sum for a sample of the query grid, single process."""
+    from oracle import density as OD
+    t0 = time.perf_counter()
+    grid = OD.HashGrid(P, h, 64, "x32")
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    idx, dist = grid.query_many(Q, 2.0 * h, 512)
+    w = OD.cubic_spline(dist, np.full(dist.shape, np.float32(h), np.float32), 3)
+    (np.where(idx >= 0, w, np.float32(0))).sum(axis=1, dtype=np.float32)
+    q_s = time.perf_counter() - t0
+    info = cpu_host_info()
+    return ({"value": len(P) / build_s, "unit": "points/s", "cores": 1, "kind": "port",
+             "sample": f"HashGrid build of all {len(P)} points, x32 keys, cell h ({build_s:.1f} s)", **info},
+            {"value": len(Q) / q_s, "unit": "queries/s", "cores": 1, "kind": "port",
+             "sample": f"{len(Q)} grid queries over the {len(P)}-particle hash grid: 27-probe candidate lists, "
+                       f"512-neighbour selection, cubic-spline sum ({q_s:.1f} s)", **info})
+
+
+def cpu_kde_baseline(P, Q, h):
+    """The reference's exhaustive KDE sum (the oracle's kde_eval) for a
+    particle sample and a grid-point sample, single process."""
+    from oracle import density as OD
+    t0 = time.perf_counter()
+    OD.kde_eval(Q, P, h)
+    el = time.perf_counter() - t0
+    return {"value": len(P) * len(Q) / el, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"exhaustive Gaussian sum, {len(P)} particles x {len(Q)} grid points ({el:.1f} s); "
+                      f"the reference evaluates all 2^24 x 256^3 pairs", **cpu_host_info()}
+
+
+def cpu_octree_baseline(field, pts, conn, data, times, x0, n_steps=2):
+    """The reference's octree RK4 path restated by the oracle (every node
+    blended per stage, fixed-depth descent, leaf scan with barycentric tests,
+    continuous grid inlet), on the octree arrays the device built (equal to
+    the oracle's build, tested), for a particle sample, single process."""
+    from oracle import boundary as OB
+    from oracle import integrators as OI
+    from oracle import octree as OO
+    tree = {k: field._dev[k].cpu().numpy() for k in ("nodes_min", "nodes_max", "nodes_elements", "nodes_children",
+                                                     "element_bounds")}
+    tree["nodes_is_leaf"] = field._dev["nodes_is_leaf"].cpu().numpy().astype(bool)
+    fn = lambda p, t: OO.sample(tree, pts, conn, data, times, p, t, field.max_depth)
+    x = np.array(x0, np.float32)
+    t0 = time.perf_counter()
+    for s in range(n_steps):
+        x = OI.rk4_step(x, np.float32(s * 1e-3), 1e-3, fn)
+        x = OB.continuous_inlet(x, [[0, 0, 0], [1, 1, 1]], "x", "positive", "grid", {"x": 64, "y": 64, "z": 64})
+    el = time.perf_counter() - t0
+    return {"value": len(x0) * n_steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{len(x0)} particles x {n_steps} RK4 steps of the same workload, octree arrays from the "
+                      f"device build ({el:.1f} s)", **cpu_host_info()}
+
+
+def cpu_adjoint_baseline(data, times, g, x0, tgt, n_steps=10):
+    """Forward plus backward on the host: the oracle's float64 restatement of
+    the tracker (torch float64 autograd, every node blended per stage) for a
+    particle sample, with the same loss; the reference has no derivative of
+    its own."""
+    import torch
+    from oracle import adjoint_ref as A
+    from oracle import fields as OF
+    U = torch.from_numpy(np.asarray(data, np.float64)).requires_grad_(True)
+    X = torch.from_numpy(np.asarray(x0, np.float64)).requires_grad_(True)
+    meta = OF.GridMeta(g, g, g)
+    st = np.arange(n_steps, dtype=np.float32) * np.float32(DT)
+    t0 = time.perf_counter()
+    xT = A.track(U, times, meta, X, st, DT, ["periodic"] * 3)
+    loss = ((xT - torch.from_numpy(np.asarray(tgt, np.float64))) ** 2).sum(1).mean()
+    t1 = time.perf_counter()
+    loss.backward()
+    t2 = time.perf_counter()
+    return {"value": len(x0) * n_steps / (t2 - t0), "unit": "particle-steps/s (forward + backward)", "cores": 1,
+            "kind": "port", "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "sample": f"{len(x0)} particles x {n_steps} RK4 steps, float64 torch autograd on the host "
+                      f"(every node blended per stage), loss mean |x_T - x*|^2", **cpu_host_info()}
+
+
 def trajectory_error_vs_fp64(torch, jt, data, times, grid, x0, bc, n_steps=1000, every=10, sample=1024,
                              report=(10, 20, 50, 100, 200, 500, 1000)):
     """Trajectory error of the GPU's own output on the headline workload over
@@ -300,6 +410,9 @@ def bench_adjoint(torch, jt, W, dev, n_part=1 << 20, steps=500, K=32, reps=3):
             t2 = time.perf_counter()
             res = {"fwd_s": t1 - t0, "bwd_s": t2 - t1, "ratio_bwd_over_fwd": (t2 - t1) / (t1 - t0)}
         out[storage] = res
+    if CPU_BASELINES["on"]:
+        out["cpu_baseline"] = cpu_adjoint_baseline(data.cpu().numpy(), times, g, x0[:1 << 14].cpu().numpy(),
+                                                   tgt[:1 << 14].cpu().numpy())
     out["workload"] = (f"128^3 x 8 random Fourier, {n_part} particles, RK4 dt=2e-3, {steps} steps, "
                        f"checkpoints every {K}, loss mean |x_T - x*|^2")
     return out
@@ -586,7 +699,10 @@ def bench_octree(torch, jt, W, L, lib, dev, steps=20):
                                        L.stream_handle()), "jt_octree_locate_stats")
     Lm, cm, am = (float(v) for v in stats.double().mean(0).cpu())
     bpps = 24 + 4 * (Lm * 29 + cm * 28 + am * 64 + 96)
-    return {"value": rate, "unit": "particle-steps/s", "ms": ms, "steps": steps, "particles": n,
+    extra = {}
+    if CPU_BASELINES["on"]:
+        extra["cpu_baseline"] = cpu_octree_baseline(f, pts, conn, data, times, x0[:4096].cpu().numpy())
+    return {**extra, "value": rate, "unit": "particle-steps/s", "ms": ms, "steps": steps, "particles": n,
             "octree_build_s": build_s,
             "roofline": hbm_roofline("k_oct_track", n * steps, ms,
                                      locate_model={"bytes_per_particle_step": bpps, "GBs": bpps * rate / 1e9,
@@ -693,6 +809,11 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
                                                             "mean_within_radius": float(st[:, 1].double().mean())}),
                   "workload": "2^24 Gaussian-mixture particles, 256^3 query grid, cubic spline h=2/256, x32 keys"}
     del est, rho
+    if CPU_BASELINES["on"]:
+        qs = np.random.default_rng(15).choice(M, 32768, replace=False)
+        hb, sb = cpu_sph_baseline(P, Qd[torch.from_numpy(qs).to(dev)].cpu().numpy(), h)
+        out["sph"]["cpu_baseline"] = sb
+        out["sph"]["hashgrid_build"]["cpu_baseline"] = hb
     if "kde" not in parts:
         return out
     return _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
@@ -765,6 +886,10 @@ def _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out):
                                        "halves and the four products are summed in float32"},
                   "workload": f"2^24 Gaussian-mixture particles, {kde_res}^3 grid, Scott bandwidth, "
                               f"cut-off {KDE_CUTOFF_SIGMAS} h"}
+    if CPU_BASELINES["on"]:
+        out["kde"]["cpu_baseline"] = cpu_kde_baseline(P[:1 << 16], Qk[np.random.default_rng(16).choice(mk, 4096,
+                                                                                                      replace=False)],
+                                                      float(kde.h))
     return out
 
 
@@ -785,6 +910,7 @@ def main():
                                    "(tracking shards particles with no exchange, so the "
                                    "one-GPU number is one shard at its real shape)"}))
         sys.exit(2)
+    CPU_BASELINES["on"] = not args.no_cpu_baseline
     import torch
     if args.copy_calibration:
         copy_calibration(torch)
@@ -904,7 +1030,7 @@ def main():
             "sample": f"{npart} particles x {args.cpu_steps} RK4 steps of the same workload "
                       f"(NumPy restatement of the reference algorithm: every node blended per "
                       f"stage, then trilinear), {el:.1f} s",
-            "host_cpus": len(os.sched_getaffinity(0))}
+            **cpu_host_info()}
     try:
         if args.no_adjoint:
             raise RuntimeError("skipped (--no-adjoint)")

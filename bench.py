@@ -715,11 +715,14 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
     (seed 6).
     SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
     hash grid of cell h with 64 slots per cell, 512 neighbours, x32 keys (the
-    reference as shipped); timed: the fused query + kernel-sum launch
-    (jt_sph_eval); the hash-grid construction (HashGridNeighbors on device
-    points: bounds with their read-back, keys, radix sort, cell table, sorted
-    positions; byte model 36 B per point); and the estimator's whole setup.
-    Roofline: measured HBM bytes per query of the jt_sph_eval call (its two
+    reference as shipped); timed: the estimator's evaluation call for x32
+    keys and uniform masses (jt_sph_eval_tiled: queries ordered by their
+    cell's (y, z) column, one column's candidates staged in LDS per wave of
+    64 queries, the top-K pass for queries with more than 512 valid
+    candidates); the hash-grid construction (HashGridNeighbors on device
+    points: bounds, keys, counting-sort passes, cell table, sorted positions;
+    byte model 36 B per point); and the estimator's whole setup.
+    Roofline: measured HBM bytes per query of the jt_sph_eval_tiled call (its
     kernels) and per point of the build (profiles/kernel_counters.json);
     beside them the candidate model 9 x 8 + C_q/3 x 16 + 4 B per query (C_q
     the reference's candidate count over its 27 probes, from
@@ -776,11 +779,15 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
     build_ms = (time.perf_counter() - t0) * 1e3
     rho = torch.empty(M, dtype=torch.float32, device=dev)
     radius = F(2.0 * est._mean_h)
+    ws_bytes = int(lib.jt_sph_tiled_workspace_size(C.byref(est._hash._desc), M))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
 
     def sph():
-        L.check(lib.jt_sph_eval(C.byref(est._hash._desc), L.ptr(Qd), M, C.c_float(radius), 64, 512,
-                                L.ptr(est._m_dev), L.ptr(est._h_dev), C.c_float(F(est._mean_h)),
-                                L.JT_SPH["cubic_spline"], L.ptr(rho), L.stream_handle()), "jt_sph_eval")
+        # the estimator's path for x32 keys and uniform masses: queries ordered
+        # by cell column, one column's candidates staged in LDS per wave
+        L.check(lib.jt_sph_eval_tiled(C.byref(est._hash._desc), L.ptr(Qd), M, C.c_float(radius), 64, 512,
+                                      C.c_float(F(est._mean_h)), L.JT_SPH["cubic_spline"], L.ptr(rho), L.ptr(ws),
+                                      ws_bytes, L.stream_handle()), "jt_sph_eval_tiled")
     sph()
     torch.cuda.synchronize()
     e0.record()
@@ -808,7 +815,7 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
                                                             "mean_candidates": cq,
                                                             "mean_within_radius": float(st[:, 1].double().mean())}),
                   "workload": "2^24 Gaussian-mixture particles, 256^3 query grid, cubic spline h=2/256, x32 keys"}
-    del est, rho
+    del est, rho, ws
     if CPU_BASELINES["on"]:
         qs = np.random.default_rng(15).choice(M, 32768, replace=False)
         hb, sb = cpu_sph_baseline(P, Qd[torch.from_numpy(qs).to(dev)].cpu().numpy(), h)

@@ -273,6 +273,18 @@ int jt_sph_eval(const jt_hashgrid* g, const float* Q, int64_t m, float radius,
                 const float* smoothing, float h_mean, int32_t kernel, float* out_rho,
                 void* stream);
 
+/* The same densities as jt_sph_eval for x32 keys with uniform masses and
+ * smoothing (sph.py:269-279 with masses 1 and h_mean), the reference's
+ * default for 3-D evaluation: the queries are ordered by their cell's (y, z)
+ * column (2-D: y), whose candidates are the same for every query of it, and
+ * a wave stages one column's candidates in LDS for 64 queries.  The terms
+ * summed are jt_sph_eval's, in another order.  per_cell_cap <= 64.  ws:
+ * device scratch of jt_sph_tiled_workspace_size(g, m) bytes. */
+size_t jt_sph_tiled_workspace_size(const jt_hashgrid* g, int64_t m);
+int jt_sph_eval_tiled(const jt_hashgrid* g, const float* Q, int64_t m, float radius,
+                      int32_t per_cell_cap, int32_t max_neighbors, float h_mean, int32_t kernel,
+                      float* out_rho, void* ws, size_t ws_bytes, void* stream);
+
 /* ---- Gaussian KDE --------------------------------------------------------- */
 /* Replaces KDEEstimator._evaluate_grid's all-pairs eval_chunk
  * (kde.py:244-270): out[i] = mean_n (2pi)^(-dim/2) exp(-0.5|Q_i-P_n|^2 f32(1/h^2)),

@@ -136,6 +136,10 @@ _SIGS = {
     "jt_sph_eval": (C.c_int, [C.POINTER(HashGridDesc), C.c_void_p, C.c_int64, C.c_float, C.c_int32,
                               C.c_int32, C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_void_p,
                               C.c_void_p]),
+    "jt_sph_tiled_workspace_size": (C.c_size_t, [C.POINTER(HashGridDesc), C.c_int64]),
+    "jt_sph_eval_tiled": (C.c_int, [C.POINTER(HashGridDesc), C.c_void_p, C.c_int64, C.c_float, C.c_int32,
+                                    C.c_int32, C.c_float, C.c_int32, C.c_void_p, C.c_void_p, C.c_size_t,
+                                    C.c_void_p]),
     "jt_kde_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64, C.c_float,
                               C.c_void_p, C.c_void_p]),
     "jt_kde_grid_cutoff": (C.c_int, [C.POINTER(HashGridDesc), C.c_int64, C.c_void_p, C.c_int32,

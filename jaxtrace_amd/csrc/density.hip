@@ -14,6 +14,9 @@
 // Distances, validity (slot < count and d <= r) and the stable "first K by
 // distance" selection follow neighbors.py:153-165 bit for bit.
 #include "jt_common.h"
+#include "cell_sort.h"
+#include <algorithm>
+#include <cmath>
 
 namespace jt {
 
@@ -855,18 +858,10 @@ k_sph_direct(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int 
 // the density is mult * sum_{d < Td} W(d) + (K - mult b) W(Td).
 // LDS per wave: list_cap distance words, 256 counters and the 32-entry
 // probe tables.
-__global__ void __launch_bounds__(256)
-k_sph_select(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K, int list_cap, float hmean,
-             int kern, float* __restrict__ out_rho) {
-    extern __shared__ unsigned sel_lds[];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int qi = blockIdx.x * 4 + wv;
-    if (qi >= m) return;                       // whole wave exits together (no block barrier below)
-    if (__float_as_uint(out_rho[qi]) != JT_SPH_REDO) return;
-    unsigned* dist = sel_lds + (size_t)wv * (list_cap + 256 + 64);
-    unsigned* hist = dist + list_cap;
-    int* toff = (int*)(hist + 256);
-    int* tst = toff + 32;
+__device__ __forceinline__ void sph_select_query(const HGDev& g, const float* __restrict__ Q, int qi, float r,
+                                                 int cap, int K, float hmean, int kern, unsigned* dist,
+                                                 unsigned* hist, int* toff, int* tst, float* __restrict__ out_rho) {
+    const int lane = threadIdx.x & 63;
     const int D = g.dim;
     float q[3] = {0.f, 0.f, 0.f};
     for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
@@ -961,6 +956,191 @@ k_sph_select(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int 
     if (lane == 0) {
         const float wt = sph_w(kern, D, __uint_as_float(prefix), hmean);
         out_rho[qi] = (x32 ? s * 3.0f : s) + (float)(K - mult * nearer) * wt;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_sph_select(HGDev g, const float* __restrict__ Q, int m, float r, int cap, int K, int list_cap, float hmean,
+             int kern, float* __restrict__ out_rho) {
+    extern __shared__ unsigned sel_lds[];
+    const int wv = threadIdx.x >> 6;
+    const int qi = blockIdx.x * 4 + wv;
+    if (qi >= m) return;                       // whole wave exits together (no block barrier below)
+    if (__float_as_uint(out_rho[qi]) != JT_SPH_REDO) return;
+    unsigned* dist = sel_lds + (size_t)wv * (list_cap + 256 + 64);
+    unsigned* hist = dist + list_cap;
+    int* toff = (int*)(hist + 256);
+    sph_select_query(g, Q, qi, r, cap, K, hmean, kern, dist, hist, toff, toff + 32, out_rho);
+}
+
+// The same top-K pass over a list of the queries to redo (list[0 .. *count)),
+// with a fixed grid: each wave takes list entries a grid-wave stride apart,
+// so the launch costs nothing where no query needs it.
+__global__ void __launch_bounds__(256)
+k_sph_select_list(HGDev g, const float* __restrict__ Q, const int* __restrict__ list,
+                  const unsigned* __restrict__ count, float r, int cap, int K, int list_cap, float hmean, int kern,
+                  float* __restrict__ out_rho) {
+    extern __shared__ unsigned sel_lds[];
+    const int wv = threadIdx.x >> 6;
+    unsigned* dist = sel_lds + (size_t)wv * (list_cap + 256 + 64);
+    unsigned* hist = dist + list_cap;
+    int* toff = (int*)(hist + 256);
+    const int n = (int)*count;
+    for (int e = blockIdx.x * 4 + wv; e < n; e += gridDim.x * 4)
+        sph_select_query(g, Q, list[e], r, cap, K, hmean, kern, dist, hist, toff, toff + 32, out_rho);
+}
+
+// ---- SPH over query tiles (x32 keys, uniform masses and smoothing) ----------
+// x32 keys drop the cell's x index, so a query's candidates depend only on
+// its cell's (y, z) (2-D: y): every query of one such column key has the
+// same 9 (2-D: 3) probe cells and the same capped slot lists.  The queries
+// are ordered by that key (k_sph_tile_keys, then a counting sort), and a
+// wave takes 64 consecutive ones: it stages the candidates of one key in
+// LDS (up to 9 x per_cell_cap positions) and each lane sums its own query
+// over them, reading the same LDS word as every other lane (a broadcast).
+// A wave whose queries carry several keys repeats that for each key.  The
+// candidate multiset, the distance arithmetic and the validity test are
+// k_sph_direct's, so the terms summed are the same; only their order
+// differs.
+
+// The ordering key of a query: its column's (y, z) cell (2-D: y), then
+// (with_x) its x cell, each clamped to two cells beyond the grid and packed
+// densely.  Ordering by x within a column keeps a wave's 64 queries close in
+// x, which lets k_sph_tiled stage only the candidates near them in x.
+__global__ void __launch_bounds__(256)
+k_sph_tile_keys(HGDev g, const float* __restrict__ Q, int m, int with_x, unsigned* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int D = g.dim;
+    int cy = cell_coord(Q[(size_t)i * D + 1], g.origin[1], g.inv);
+    cy = min(max(cy, -2), g.dims[1] + 1) + 2;
+    unsigned k = (unsigned)cy;
+    if (D == 3) {
+        int cz = cell_coord(Q[(size_t)i * D + 2], g.origin[2], g.inv);
+        cz = min(max(cz, -2), g.dims[2] + 1) + 2;
+        k = (unsigned)cy * (unsigned)(g.dims[2] + 4) + (unsigned)cz;
+    }
+    if (with_x) {
+        int cx = cell_coord(Q[(size_t)i * D], g.origin[0], g.inv);
+        cx = min(max(cx, -2), g.dims[0] + 1) + 2;
+        k = k * (unsigned)(g.dims[0] + 4) + (unsigned)cx;
+    }
+    keys[i] = k;
+}
+
+#define SPH_TILE_CAP 576                        // 9 probe cells x per_cell_cap 64
+
+// D, KERN: the dimension and the SPH kernel as constants.  d2max: the
+// largest float whose correctly rounded square root is <= r, so d2 <= d2max
+// is exactly k_sph_direct's test sqrtf(d2) <= r (the square root is monotone)
+// and the root is taken only for candidates within r.
+template <int D, int KERN>
+__global__ void __launch_bounds__(256)
+k_sph_tiled(HGDev g, const float* __restrict__ Q, const int* __restrict__ order, int m, float d2max, float xr,
+            int cap, int K, float hmean, float* __restrict__ out_rho, int* __restrict__ redo,
+            unsigned* __restrict__ redo_count) {
+    __shared__ float cand[4][SPH_TILE_CAP * 3];
+    __shared__ int toff[4][16], tst[4][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int base = (blockIdx.x * 4 + wv) * 64;
+    if (base >= m) return;                     // whole wave exits together (no block barrier below)
+    const int i = base + lane;
+    const bool live = i < m;
+    const int qi = live ? order[i] : 0;
+    float q[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < D; ++a) q[a] = Q[(size_t)qi * D + a];
+    const int cy = cell_coord(q[1], g.origin[1], g.inv);
+    const int cz = D == 3 ? cell_coord(q[2], g.origin[2], g.inv) : 0;
+    const int probes = D == 3 ? 9 : 3;
+    unsigned long long pending = __ballot(live);
+    int nvalid = 0;
+    float acc = 0.0f;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int ky = __shfl(cy, leader), kz = __shfl(cz, leader);
+        const bool mine = ((pending >> lane) & 1ull) && cy == ky && cz == kz;
+        pending &= ~__ballot(mine);
+        // the key's probe cells in the reference's order (the di = 0 third
+        // of the stencil; x is dropped by the key)
+        int st = 0, cnt = 0;
+        if (lane < probes) {
+            const int dj = D == 3 ? lane / 3 - 1 : lane - 1;
+            const int dk = D == 3 ? lane % 3 - 1 : 0;
+            const int cell = D == 3 ? probe_cell(g, 0, ky + dj, kz + dk) : probe_cell(g, 0, ky + dj, 0);
+            if (cell >= 0) { st = g.cstart[cell]; cnt = min(g.cend[cell] - st, cap); }
+        }
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const int total = __shfl(incl, 63);
+        if (lane < probes) { toff[wv][lane] = incl - cnt; tst[wv][lane] = st; }
+        if (lane == 0) toff[wv][probes] = total;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // stage only the candidates within xr in x of this key's queries'
+        // x range: for any other one every query's float dx (the kernel's
+        // own subtraction, monotone in q) exceeds xr > r, so its d2 exceeds
+        // d2max and dropping it changes no term
+        float qlo = mine ? q[0] : INFINITY, qhi = mine ? q[0] : -INFINITY;
+        for (int o = 32; o > 0; o >>= 1) {
+            qlo = fminf(qlo, __shfl_xor(qlo, o));
+            qhi = fmaxf(qhi, __shfl_xor(qhi, o));
+        }
+        int pr = 0, kept = 0;
+        for (int f0 = 0; f0 < total; f0 += 64) {
+            const int f = f0 + lane;
+            float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+            bool keep = false;
+            if (f < total) {
+                while (f >= toff[wv][pr + 1]) ++pr;
+                const float* pp = g.spos + (size_t)(tst[wv][pr] + (f - toff[wv][pr])) * D;
+                c0 = pp[0]; c1 = pp[1];
+                if constexpr (D == 3) c2 = pp[2];
+                keep = c0 - qhi <= xr && qlo - c0 <= xr;
+            }
+            const unsigned long long km = __ballot(keep);
+            if (keep) {
+                const int at = kept + __popcll(km & ((1ull << lane) - 1ull));
+                cand[wv][at * 3] = c0;
+                cand[wv][at * 3 + 1] = c1;
+                cand[wv][at * 3 + 2] = c2;
+            }
+            kept += __popcll(km);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (mine) {
+            for (int f = 0; f < kept; ++f) {
+                const float dx = cand[wv][f * 3] - q[0], dy = cand[wv][f * 3 + 1] - q[1];
+                float d2 = dx * dx + dy * dy;
+                if constexpr (D == 3) { const float dz = cand[wv][f * 3 + 2] - q[2]; d2 = d2 + dz * dz; }
+                if (d2 <= d2max) {
+                    ++nvalid;
+                    acc += sph_w(KERN, D, sqrtf(d2), hmean);
+                }
+            }
+        }
+        // every lane is done with this key's candidates before the next key's
+        // overwrite them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    // more than K valid candidates: the top-K pass takes the query (listed,
+    // one atomic per wave)
+    const bool again = live && nvalid * 3 > K;
+    if (live) out_rho[qi] = again ? __uint_as_float(JT_SPH_REDO) : acc * 3.0f;
+    const unsigned long long am = __ballot(again);
+    if (am) {
+        const int first = __ffsll((long long)am) - 1;
+        unsigned at = 0;
+        if (lane == first) at = atomicAdd(redo_count, (unsigned)__popcll(am));
+        at = __shfl(at, first);
+        if (again) redo[at + __popcll(am & ((1ull << lane) - 1ull))] = qi;
     }
 }
 
@@ -1079,6 +1259,111 @@ extern "C" int jt_sph_eval(const jt_hashgrid* h, const float* Q, int64_t m, floa
     hipLaunchKernelGGL(k_query<1>, dim3((unsigned)((m + waves - 1) / waves)), dim3(64 * waves), lds,
                        (hipStream_t)stream, g, Q, (int)m, radius, per_cell_cap, max_neighbors, nc_pad,
                        nullptr, nullptr, masses, smoothing, h_mean, kernel, out_rho, 1);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+// The largest float x with sqrtf(x) <= r (correctly rounded square roots
+// are monotone, so for every float d2, sqrtf(d2) <= r exactly when
+// d2 <= that bound); -inf when no x qualifies (r negative or NaN).
+static float sqrt_le_bound(float r) {
+    if (!(r >= 0.0f)) return -INFINITY;
+    if (std::isinf(r)) return INFINITY;
+    float x = r * r;
+    while (std::isfinite(x) && std::sqrt(std::nextafter(x, INFINITY)) <= r) x = std::nextafter(x, INFINITY);
+    while (x > 0.0f && std::sqrt(x) > r) x = std::nextafter(x, -INFINITY);
+    return x;
+}
+
+// Buckets of the tile ordering key of a grid, with the x cell in the key
+// when that keeps the bucket count at most 2^24 (with_x), else without.
+static int64_t sph_tile_buckets(const HGDev& g, bool& with_x) {
+    const int64_t col = g.dim == 3 ? (int64_t)(g.dims[1] + 4) * (g.dims[2] + 4) : (int64_t)(g.dims[1] + 4);
+    const int64_t full = col * (g.dims[0] + 4);
+    with_x = full <= (1LL << 24);
+    return with_x ? full : col;
+}
+
+extern "C" size_t jt_sph_tiled_workspace_size(const jt_hashgrid* h, int64_t m) {
+    HGDev g;
+    if (make_hg(h, g)) return 0;
+    const size_t mm = (size_t)(m > 0 ? m : 1);
+    bool with_x;
+    const int64_t nb64 = sph_tile_buckets(g, with_x);
+    if (nb64 >= (1LL << 31)) return 0;
+    const int nb = (int)nb64;
+    Arena a(nullptr, 0);
+    a.take<unsigned>(mm);
+    a.take<int>(mm);
+    a.take<int>((size_t)nb);
+    a.take<int>((size_t)nb);
+    a.take<char>(bucket_scan_bytes(nb));
+    a.take<int>(mm);                                // queries for the top-K pass
+    a.take<unsigned>(1);                            // their count
+    return a.used + 256;
+}
+
+extern "C" int jt_sph_eval_tiled(const jt_hashgrid* h, const float* Q, int64_t m, float radius, int32_t per_cell_cap,
+                                 int32_t max_neighbors, float h_mean, int32_t kernel, float* out_rho, void* ws,
+                                 size_t ws_bytes, void* stream) {
+    HGDev g;
+    int nc_pad, waves;
+    int rc = query_common(h, g, m, per_cell_cap, max_neighbors, nc_pad, waves);
+    if (rc) return rc;
+    JT_REQUIRE(kernel >= JT_SPH_CUBIC && kernel <= JT_SPH_GAUSSIAN, "bad kernel %d", kernel);
+    if (g.key_mode != JT_KEY_X32)
+        return fail(JT_EUNSUPPORTED, "query tiles need x32 keys (the candidates of x64 keys depend on the x cell)");
+    JT_REQUIRE(per_cell_cap <= SPH_TILE_CAP / 9, "per_cell_cap above %d for query tiles", SPH_TILE_CAP / 9);
+    bool with_x;
+    const int64_t nb64 = sph_tile_buckets(g, with_x);
+    JT_REQUIRE(nb64 < (1LL << 31), "grid too large for query tiles");
+    if (m == 0) return JT_OK;
+    JT_REQUIRE(Q && out_rho, "NULL pointer");
+    const int nb = (int)nb64;
+    Arena arena(ws, ws_bytes);
+    unsigned* keys = arena.take<unsigned>((size_t)m);
+    int* order = arena.take<int>((size_t)m);
+    int* counts = arena.take<int>((size_t)nb);
+    int* offs = arena.take<int>((size_t)nb);
+    const size_t temp = bucket_scan_bytes(nb);
+    char* tmp = arena.take<char>(temp);
+    int* redo = arena.take<int>((size_t)m);
+    unsigned* redo_count = arena.take<unsigned>(1);
+    if (!keys || !order || !counts || !offs || !tmp || !redo || !redo_count || !arena.ok())
+        return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", jt_sph_tiled_workspace_size(h, m), ws_bytes);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_sph_tile_keys, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, g, Q, (int)m,
+                       (int)with_x, keys);
+    JT_HIP(hipGetLastError());
+    JT_HIP(bucket_sort(s, keys, (int)m, 0, nb, counts, offs, tmp, temp, order));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, redo_count, 1, 0u);
+    const float d2max = sqrt_le_bound(radius);
+    // x half-width of the staging window: |dx| > xr means dx * dx > r * r by
+    // a margin no rounding of the sum of squares can undo
+    const float xr = std::isfinite(radius) && radius >= 0.0f ? radius * 1.001f : INFINITY;
+    const dim3 gt((unsigned)((m + 255) / 256)), bt(256);
+#define JT_SPH_TILED(DD, KK)                                                                               \
+    hipLaunchKernelGGL((k_sph_tiled<DD, KK>), gt, bt, 0, s, g, Q, (const int*)order, (int)m, d2max, xr,   \
+                       per_cell_cap, max_neighbors, h_mean, out_rho, redo, redo_count)
+    if (g.dim == 3) {
+        if (kernel == JT_SPH_CUBIC) JT_SPH_TILED(3, JT_SPH_CUBIC);
+        else if (kernel == JT_SPH_WENDLAND) JT_SPH_TILED(3, JT_SPH_WENDLAND);
+        else JT_SPH_TILED(3, JT_SPH_GAUSSIAN);
+    } else {
+        if (kernel == JT_SPH_CUBIC) JT_SPH_TILED(2, JT_SPH_CUBIC);
+        else if (kernel == JT_SPH_WENDLAND) JT_SPH_TILED(2, JT_SPH_WENDLAND);
+        else JT_SPH_TILED(2, JT_SPH_GAUSSIAN);
+    }
+#undef JT_SPH_TILED
+    JT_HIP(hipGetLastError());
+    // queries with more than max_neighbors valid candidates: the top-K pass
+    // over the listed ones
+    const int list_cap = (g.dim == 3 ? 9 : 3) * per_cell_cap;
+    const size_t sel_lds = (size_t)(list_cap + 256 + 64) * 4 * 4;
+    const unsigned sel_blocks = (unsigned)std::min<int64_t>((m + 3) / 4, 2048);
+    hipLaunchKernelGGL(k_sph_select_list, dim3(sel_blocks), dim3(256), sel_lds, s, g, Q, (const int*)redo,
+                       (const unsigned*)redo_count, radius, per_cell_cap, max_neighbors, list_cap, h_mean, kernel,
+                       out_rho);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

@@ -115,7 +115,7 @@ class SPHDensityEstimator:
                  masses=None, max_particles_for_adaptive: int = 5000, neighbor_search: str = "hashgrid",
                  per_cell_cap: int = 64, max_neighbors: int = 512, normalize: bool = True,
                  resolution: Union[int, Tuple[int, int, int]] = 100, bounds=None,
-                 key_mode: str = "x32", device=None):
+                 key_mode: str = "x32", device=None, tiled_queries: bool = True):
         self.device = torch.device(device) if device is not None else L.device()
         self.dimensions, self.plane, self.position = dimensions, plane, position
         self.slab_thickness, self.kernel_type = slab_thickness, kernel_type
@@ -124,6 +124,10 @@ class SPHDensityEstimator:
         self.neighbor_search, self.per_cell_cap = neighbor_search, int(per_cell_cap)
         self.max_neighbors, self.normalize = int(max_neighbors), normalize
         self.resolution, self.bounds, self.key_mode = resolution, bounds, key_mode
+        # x32 keys with uniform masses and smoothing: evaluate queries in tiles
+        # that share a (y, z) cell column (jt_sph_eval_tiled); False keeps one
+        # wave per query (jt_sph_eval)
+        self.tiled_queries = bool(tiled_queries)
         if kernel_type not in L.JT_SPH:
             raise ValueError(f"Unknown kernel type: {kernel_type}")
         if neighbor_search != "hashgrid":
@@ -231,11 +235,21 @@ class SPHDensityEstimator:
         M = int(Q.shape[0])
         rho = torch.empty(M, dtype=torch.float32, device=self.device)
         radius = F32(2.0 * self._mean_h)                                       # :273
-        L.check(lib.jt_sph_eval(C.byref(self._hash._desc), L.ptr(Q), M, C.c_float(radius),
-                                self.per_cell_cap, self.max_neighbors, L.ptr(self._m_dev),
-                                L.ptr(self._h_dev), C.c_float(F32(self._mean_h)),
-                                L.JT_SPH[self.kernel_type], L.ptr(rho), L.stream_handle()),
-                "jt_sph_eval")
+        if (self.tiled_queries and self.key_mode == "x32" and self._m_dev is None and self._h_dev is None
+                and self.per_cell_cap <= 64):
+            desc = C.byref(self._hash._desc)
+            ws_bytes = int(lib.jt_sph_tiled_workspace_size(desc, M))
+            ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=self.device)
+            L.check(lib.jt_sph_eval_tiled(desc, L.ptr(Q), M, C.c_float(radius), self.per_cell_cap,
+                                          self.max_neighbors, C.c_float(F32(self._mean_h)),
+                                          L.JT_SPH[self.kernel_type], L.ptr(rho), L.ptr(ws), ws_bytes,
+                                          L.stream_handle()), "jt_sph_eval_tiled")
+        else:
+            L.check(lib.jt_sph_eval(C.byref(self._hash._desc), L.ptr(Q), M, C.c_float(radius),
+                                    self.per_cell_cap, self.max_neighbors, L.ptr(self._m_dev),
+                                    L.ptr(self._h_dev), C.c_float(F32(self._mean_h)),
+                                    L.JT_SPH[self.kernel_type], L.ptr(rho), L.stream_handle()),
+                    "jt_sph_eval")
         rho = rho.cpu().numpy()
         if self.normalize and M > 0:                                          # :365-378
             if self.D == 2:

@@ -205,6 +205,63 @@ def test_kde_cutoff_tiles_sampled(gpu):
     np.testing.assert_allclose(d[idx][m], want[m], rtol=1e-5)
 
 
+@pytest.mark.parametrize("dims", ["3d", "2d"])
+def test_sph_query_tiles_match_per_query_path(gpu, dims):
+    """x32 keys, uniform masses: the column-tiled evaluation
+    (jt_sph_eval_tiled) and one wave per query (jt_sph_eval) sum the same
+    terms.  Queries: a block of grid points (many per column), random points
+    in and around the particle bounds (columns probed partly or not at all),
+    and a dense cluster (more than max_neighbors valid candidates: the top-K
+    pass).  Both paths against the oracle too."""
+    rng = np.random.default_rng(31)
+    P = np.concatenate([W.gaussian_mixture(150_000, seed=30),
+                        (0.5 + 0.002 * rng.standard_normal((20_000, 3)))]).astype(F32)
+    h = 2.0 / 256
+    a = np.linspace(0.3, 0.6, 40, dtype=F32)
+    X, Y, Z = np.meshgrid(a, a, a, indexing="xy")
+    Q = np.concatenate([np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1), rng.uniform(-0.1, 1.1, (3000, 3)),
+                        (0.5 + 0.003 * rng.standard_normal((500, 3)))]).astype(F32)
+    kw = dict(plane="xy", position=0.5, slab_thickness=0.1) if dims == "2d" else {}
+    res = {}
+    for tiled in (True, False):
+        est = jt.SPHDensityEstimator(P, dimensions=dims, smoothing_length=h, key_mode="x32", max_neighbors=300,
+                                     normalize=False, tiled_queries=tiled, **kw)
+        res[tiled] = est.evaluate(Q)
+    np.testing.assert_allclose(res[True], res[False], rtol=2e-6, atol=1e-6 * np.abs(res[False]).max())
+    sel = np.concatenate([np.arange(0, len(a) ** 3, 97), np.arange(len(a) ** 3, len(Q), 7)])
+    want, _, _ = OD.sph_density(P, Q[sel], h, 64, 300, "x32", False, dimensions=dims, **kw)
+    floor = 1e-3 * np.abs(want).max()
+    for tiled in (True, False):
+        got = res[tiled][sel]
+        m = np.abs(want) >= floor
+        np.testing.assert_allclose(got[m], want[m], rtol=1e-5)
+        np.testing.assert_allclose(got[~m], want[~m], rtol=1e-5, atol=1e-5 * floor)
+
+
+def test_full_size_sph_tiles(gpu):
+    """The benchmark's SPH workload (2^24 mixture particles, h = 2/256,
+    x32, 512 neighbours) on a 64 x 64 x 64 block of its 256^3 query grid:
+    the tiled and the per-query paths agree, and 256 sampled queries match
+    the oracle."""
+    P = W.gaussian_mixture(1 << 24, seed=6)
+    h = 2.0 / 256
+    a = np.linspace(0.0, 1.0, 256, dtype=F32)[96:160]
+    X, Y, Z = np.meshgrid(a, a, a, indexing="xy")
+    Q = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(F32)
+    Pd = torch.from_numpy(P).cuda()
+    res = {}
+    for tiled in (True, False):
+        est = jt.SPHDensityEstimator(Pd, dimensions="3d", smoothing_length=h, key_mode="x32", normalize=False,
+                                     tiled_queries=tiled)
+        res[tiled] = est.evaluate(Q)
+        del est
+    np.testing.assert_allclose(res[True], res[False], rtol=2e-6, atol=1e-6 * np.abs(res[False]).max())
+    sel = np.random.default_rng(32).choice(len(Q), 256, replace=False)
+    want, _, _ = OD.sph_density(P, Q[sel], h, 64, 512, "x32", False)
+    m = np.abs(want) >= 1e-3 * np.abs(want).max()
+    np.testing.assert_allclose(res[True][sel][m], want[m], rtol=1e-5)
+
+
 def test_full_size_sph_sampled_queries(gpu):
     """2^24 Gaussian-mixture particles, h = 2/256: neighbour lists of 2048
     sampled grid queries bit-exact with the oracle in both key modes."""

@@ -515,6 +515,30 @@ def bench_seeding(torch, jt, W, L, lib, dev, res=256, reps=20):
     del out
     out_d["random_seeds"] = _bench_random_seeds(torch, L, lib, dev, n, reps)
     out_d["inlet_grid_particles"] = _bench_inlet(torch, L, lib, dev, 1 << 22, reps)
+    if CPU_BASELINES["on"]:
+        from oracle import seeding as OS
+        info = cpu_host_info()
+        t0 = time.perf_counter()
+        OS.uniform_grid_seeds(res, [[0, 0, 0], [1, 1, 1]])
+        el = time.perf_counter() - t0
+        out_d["cpu_baseline"] = {"value": n / el, "unit": "seeds/s", "cores": 1, "kind": "port",
+                                 "sample": f"the oracle's uniform_grid_seeds {res}^3 (NumPy meshgrid, {el:.2f} s)",
+                                 **info}
+        nr = 1 << 20
+        t0 = time.perf_counter()
+        OS.random_seeds(nr, [[0, 0, 0], [1, 1, 1]], 42)
+        el = time.perf_counter() - t0
+        out_d["random_seeds"]["cpu_baseline"] = {
+            "value": nr / el, "unit": "seeds/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's threefry2x32 random_seeds for {nr} seeds (NumPy, {el:.2f} s)", **info}
+        ni = 1 << 16
+        t0 = time.perf_counter()
+        OS.inlet_grid_particles(ni, 0.0, 0, [[0, 0, 0], [1, 1, 1]], {"x": 64, "y": 64, "z": 64})
+        el = time.perf_counter() - t0
+        out_d["inlet_grid_particles"]["cpu_baseline"] = {
+            "value": ni / el, "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle's inlet face grid for {ni} particles (the reference's index loops, {el:.2f} s)",
+            **info}
     return out_d
 
 
@@ -655,7 +679,17 @@ def bench_nearest(torch, jt, W, L, lib, dev, m=1 << 20):
     pairs = float(m) * pos.shape[0]
     rate = pairs / (ms * 1e-3)
     peak = VALU_LANE_OPS / 13.0
-    return {"value": rate, "unit": "pairs/s", "ms": ms, "queries": m, "nodes": int(pos.shape[0]),
+    extra = {}
+    if CPU_BASELINES["on"]:
+        from oracle import fields as OF
+        qh = q[:4096].cpu().numpy()
+        t0 = time.perf_counter()
+        OF.nearest_node_sample(data.reshape(T, -1, 3), times, pos, qh, 0.5)
+        el = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": len(qh) * pos.shape[0] / el, "unit": "pairs/s", "cores": 1, "kind": "port",
+                                 "sample": f"{len(qh)} queries against all {pos.shape[0]} nodes, the oracle's "
+                                           f"exhaustive argmin ({el:.1f} s)", **cpu_host_info()}
+    return {**extra, "value": rate, "unit": "pairs/s", "ms": ms, "queries": m, "nodes": int(pos.shape[0]),
             "roofline": {"bound": "valu", "achieved": rate, "peak": peak, "unit": "pairs/s", "frac": rate / peak},
             "workload": "ABC flow 32^3 nodes x 4 snapshots, 2^20 uniform queries, nearest node (exhaustive)"}
 

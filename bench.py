@@ -768,10 +768,10 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
     of the float64 exhaustive sum; every dropped term is below
     (2 pi)^(-3/2) exp(-c^2/2), so their mean is too, and the line reports
     that bound relative to the density floor (1e-3 of the maximum); timed: the
-    jt_kde_grid_cutoff launch.  Rate: pairs within the cut-off per second
+    jt_kde_grid_cutoff_ws call (tile kernel and slice sum).  Rate: pairs within the cut-off per second
     (counted by jt_hashgrid_query_stats on a sample).  Roofline: the dense
     BF16 MFMA peak against 8 flops (four bfloat16 products of the split
-    float32 operands) per (grid point, particle) pair of each 16^3 tile and
+    float32 operands) per (grid point, particle) pair of each 16 x 16 x 32 tile and
     the particles within R of the tile's box, the products the kernel's
     matrix cores evaluate (estimated from 512 sampled tiles).
     parts: which of 'hashgrid', 'sph', 'kde' to run."""
@@ -872,10 +872,13 @@ def _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out):
     gx, gy, gz = (torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in (x, y, z))
     D = torch.empty((len(y), len(x), len(z)), dtype=torch.float32, device=dev)
 
+    ws_bytes = int(lib.jt_kde_grid_cutoff_workspace_size(len(x), len(y), len(z)))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+
     def kd():
-        L.check(lib.jt_kde_grid_cutoff(C.byref(hg._desc), kde.N, L.ptr(gx), len(x), L.ptr(gy), len(y), L.ptr(gz),
-                                       len(z), C.c_float(F(kde.h)), C.c_float(R), L.ptr(D), L.stream_handle()),
-                "jt_kde_grid_cutoff")
+        L.check(lib.jt_kde_grid_cutoff_ws(C.byref(hg._desc), kde.N, L.ptr(gx), len(x), L.ptr(gy), len(y), L.ptr(gz),
+                                          len(z), C.c_float(F(kde.h)), C.c_float(R), L.ptr(D), L.ptr(ws), ws_bytes,
+                                          L.stream_handle()), "jt_kde_grid_cutoff_ws")
     kd()
     torch.cuda.synchronize()
     e0.record()
@@ -896,16 +899,18 @@ def _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out):
                                         L.ptr(stk), L.stream_handle()), "jt_hashgrid_query_stats")
     pairs = float(stk[:, 1].double().mean()) * mk
     rate = pairs / (kde_ms * 1e-3)
-    # tile pairs: (grid points of a 16^3 tile) x (particles within R of the
-    # tile's box), the products the kernel's matrix cores evaluate, summed
-    # over 512 tiles drawn at random (seed 14) and scaled to all tiles
-    nt = [(len(v) + 15) // 16 for v in (x, y, z)]
+    del ws
+    # tile pairs: (grid points of a 16 x 16 x 32 tile) x (particles within R
+    # of the tile's box), the products the kernel's matrix cores evaluate,
+    # summed over 512 tiles drawn at random (seed 14) and scaled to all tiles
+    ext = (16, 16, 32)
+    nt = [(len(v) + e - 1) // e for v, e in zip((x, y, z), ext)]
     n_tiles = nt[0] * nt[1] * nt[2]
     trng = np.random.default_rng(14)
     pts = 0.0
     for _ in range(512):
         tb = [int(trng.integers(0, k)) for k in nt]
-        seg = [v[t * 16:min(t * 16 + 16, len(v))] for v, t in zip((x, y, z), tb)]
+        seg = [v[t * e:min(t * e + e, len(v))] for v, t, e in zip((x, y, z), tb, ext)]
         lo_t = torch.tensor([float(g_.min()) for g_ in seg], device=dev)
         hi_t = torch.tensor([float(g_.max()) for g_ in seg], device=dev)
         d = torch.clamp(torch.maximum(lo_t - Pd, Pd - hi_t), min=0.0)
@@ -920,7 +925,7 @@ def _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out):
                   "dropped_tail_bound_rel_at_floor": tail,
                   "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
                                "frac": ach / MFMA_BF16_TFLOPS, "tile_pairs": tile_pairs,
-                               "tile_pairs_sample": "512 random 16^3 tiles (seed 14), scaled to all tiles",
+                               "tile_pairs_sample": "512 random 16 x 16 x 32 tiles (seed 14), scaled to all tiles",
                                "hbm": hbm_roofline("k_kde_tile_mfma", 1, kde_ms),
                                "flops_per_tile_pair": 8,
                                "note": "dense bfloat16 MFMA peak; each float32 operand is split into bfloat16 hi/lo "

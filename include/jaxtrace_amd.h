@@ -301,6 +301,14 @@ int jt_kde_eval(const float* P, int64_t n, int32_t dim, const float* Q, int64_t 
 int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
                        const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
                        float cutoff, float* out, void* stream);
+/* jt_kde_grid_cutoff with a workspace (jt_kde_grid_cutoff_workspace_size
+ * bytes): each block's partial densities go to their own slice and the
+ * slices of a grid tile are added in a fixed order, so the result is the
+ * same on every call and no float atomics are used (the estimator's path). */
+size_t jt_kde_grid_cutoff_workspace_size(int32_t nx, int32_t ny, int32_t nz);
+int jt_kde_grid_cutoff_ws(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
+                          const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
+                          float cutoff, float* out, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- unstructured FEM field: element octree + P1 tetrahedra ---------------- */
 /* Opaque host octree (OctreeMeshOptimized, octree_fem_interpolator_optimized.py:18-38). */

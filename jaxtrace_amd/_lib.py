@@ -145,6 +145,10 @@ _SIGS = {
     "jt_kde_grid_cutoff": (C.c_int, [C.POINTER(HashGridDesc), C.c_int64, C.c_void_p, C.c_int32,
                                      C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, C.c_float,
                                      C.c_void_p, C.c_void_p]),
+    "jt_kde_grid_cutoff_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
+    "jt_kde_grid_cutoff_ws": (C.c_int, [C.POINTER(HashGridDesc), C.c_int64, C.c_void_p, C.c_int32,
+                                        C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, C.c_float,
+                                        C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "jt_octree_build": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_void_p)]),
     "jt_octree_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),

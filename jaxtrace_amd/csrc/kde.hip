@@ -5,7 +5,7 @@
 //   query; particles stream through LDS in tiles of 256 and are reused by all
 //   256 queries of the block.  Terms are accumulated in float64.
 // k_kde_tile_mfma (below; jt_kde_grid_cutoff runs it): the cut-off sum on
-//   the structured grid on the matrix cores, 16 x 16 x 16 tiles.
+//   the structured grid on the matrix cores, 16 x 16 x 32 tiles.
 //   Each dropped particle is farther than R from the grid point, so each
 //   dropped term is below norm*exp(-R^2/(2h^2)).
 #include "jt_common.h"
@@ -51,7 +51,7 @@ struct KGrid {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// k_kde_tile_mfma: the cut-off sum with 16 x 16 x 16 tiles on the matrix
+// k_kde_tile_mfma: the cut-off sum with 16 x 16 x 32 tiles on the matrix
 // cores.  For a tile and a pass of particles p the densities are a matrix
 // product,
 //   D[(x, y), z] += sum_p (ex[x, p] * ey[y, p]) * ez[p, z],
@@ -59,7 +59,7 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // coordinate.  Each of the 4 waves owns 4 of the tile's 16 y rows.
 //
 // Precision: the products run on the bfloat16 matrix cores
-// (v_mfma_f32_16x16x32_bf16, 16 times the float32 MFMA rate) with each
+// (v_mfma_f32_32x32x16_bf16, 16 times the float32 MFMA rate) with each
 // float32 operand split into two bfloat16 halves, hi = bf16(v) and
 // lo = bf16(v - hi) (both round-to-nearest; v - hi is exact), so
 // v = hi + lo within 2^-18 |v|.  The four products hi*hi + hi*lo + lo*hi +
@@ -74,20 +74,22 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // point), and hash-cell columns whose (x, y) distance to the box exceeds R
 // are not read at all; a column's z range is cut to the z half-width left
 // over by its (x, y) distance.
-typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-#define KDE_T 16
+#define KDE_TX 16                                     // tile extent in x (16 of the 32 rows of an M block)
+#define KDE_TY 16                                     // in y: 4 waves x 4 rows
+#define KDE_TZ 32                                     // in z: the MFMA N dimension
 #define KDE_BATCH 256                                 // particles read per round (one per thread)
 #ifndef KDE_P
-#define KDE_P 256                                     // particles per MFMA pass (a multiple of 32)
+#define KDE_P 256                                     // particles per MFMA pass (a multiple of 16)
 #endif
 #define KDE_STAGE (KDE_P + KDE_BATCH - 1)             // staged positions at most
 #define KDE_LDP (KDE_P + 8)                           // padded row length of the factor tables
 #ifndef KDE_TARGET_BLOCKS
-#define KDE_TARGET_BLOCKS 32768
+#define KDE_TARGET_BLOCKS 16384
 #endif
 #ifndef KDE_MAX_SPLIT
-#define KDE_MAX_SPLIT 16
+#define KDE_MAX_SPLIT 8
 #endif
 
 __global__ void __launch_bounds__(256) k_kde_zero(float* __restrict__ out, size_t n) {
@@ -121,36 +123,41 @@ __device__ __forceinline__ void split8(const float v[8], bf16x8& hi, bf16x8& lo)
 // adding its partial densities into out (zeroed first).  The particle
 // counts of the tiles' boxes differ widely on clustered data, so one block
 // per tile leaves a tail of heavy blocks running alone.
+// Tiles are KDE_TX x KDE_TY x KDE_TZ grid points; each pass of at most
+// KDE_P staged particles is D[(x, y), z] += A[(x, y), p] B[p, z] with
+// A = ex * ey (formed and split per wave) and B = ez (split once per
+// particle), on v_mfma_f32_32x32x16_bf16: the 32 z of the tile are the N
+// dimension, so each formed and split A value feeds 32 output columns.
+// Wave w owns the y rows 4w .. 4w + 3, as two 32-row M blocks (two y rows
+// of 16 x each).
 __global__ void __launch_bounds__(256)
 k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
                 const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
-                int nsplit, float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) float exs[KDE_T][KDE_LDP];        // x factors [x][p]
-    __shared__ __attribute__((aligned(16))) float eys[KDE_T][KDE_LDP];        // y factors [y][p]
-    __shared__ __attribute__((aligned(16))) __bf16 ezh[KDE_T][KDE_LDP];       // z factors [z][p], hi half
-    __shared__ __attribute__((aligned(16))) __bf16 ezl[KDE_T][KDE_LDP];       // lo half
+                int nsplit, float* __restrict__ out, float* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) float exs[KDE_TX][KDE_LDP];       // x factors [x][p]
+    __shared__ __attribute__((aligned(16))) float eys[KDE_TY][KDE_LDP];       // y factors [y][p]
+    __shared__ __attribute__((aligned(16))) __bf16 ezh[KDE_TZ][KDE_LDP];      // z factors [z][p], hi half
+    __shared__ __attribute__((aligned(16))) __bf16 ezl[KDE_TZ][KDE_LDP];      // lo half
+    __shared__ __attribute__((aligned(16))) float tcs[KDE_TX + KDE_TY + KDE_TZ];   // the tile's x | y | z
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int tiles_z = (nz + KDE_T - 1) / KDE_T, tiles_x = (nx + KDE_T - 1) / KDE_T;
+    const int tiles_z = (nz + KDE_TZ - 1) / KDE_TZ, tiles_x = (nx + KDE_TX - 1) / KDE_TX;
     const int part = blockIdx.x % nsplit;
     int b = blockIdx.x / nsplit;
     const int bz = b % tiles_z; b /= tiles_z;
     const int bx = b % tiles_x; b /= tiles_x;
     const int by = b;
     // the tile's coordinates (clamped at the grid's end; clamped points are not written)
-    float xs[KDE_T], ys[KDE_T], zs[KDE_T];
-#pragma unroll
-    for (int i = 0; i < KDE_T; ++i) {
-        xs[i] = gx[min(bx * KDE_T + i, nx - 1)];
-        ys[i] = gy[min(by * KDE_T + i, ny - 1)];
-        zs[i] = gz[min(bz * KDE_T + i, nz - 1)];
-    }
+    if (tid < KDE_TX) tcs[tid] = gx[min(bx * KDE_TX + tid, nx - 1)];
+    else if (tid < KDE_TX + KDE_TY) tcs[tid] = gy[min(by * KDE_TY + tid - KDE_TX, ny - 1)];
+    else if (tid < KDE_TX + KDE_TY + KDE_TZ) tcs[tid] = gz[min(bz * KDE_TZ + tid - KDE_TX - KDE_TY, nz - 1)];
+    __syncthreads();
+    const float* xs = tcs;
+    const float* ys = tcs + KDE_TX;
+    const float* zs = tcs + KDE_TX + KDE_TY;
     float lo[3] = {xs[0], ys[0], zs[0]}, hi[3] = {xs[0], ys[0], zs[0]};
-#pragma unroll
-    for (int i = 1; i < KDE_T; ++i) {
-        lo[0] = fminf(lo[0], xs[i]); hi[0] = fmaxf(hi[0], xs[i]);
-        lo[1] = fminf(lo[1], ys[i]); hi[1] = fmaxf(hi[1], ys[i]);
-        lo[2] = fminf(lo[2], zs[i]); hi[2] = fmaxf(hi[2], zs[i]);
-    }
+    for (int i = 1; i < KDE_TX; ++i) { lo[0] = fminf(lo[0], xs[i]); hi[0] = fmaxf(hi[0], xs[i]); }
+    for (int i = 1; i < KDE_TY; ++i) { lo[1] = fminf(lo[1], ys[i]); hi[1] = fmaxf(hi[1], ys[i]); }
+    for (int i = 1; i < KDE_TZ; ++i) { lo[2] = fminf(lo[2], zs[i]); hi[2] = fmaxf(hi[2], zs[i]); }
     int c0[3], c1[3];
     for (int a = 0; a < 3; ++a) {
         c0[a] = clampi((int)floorf((lo[a] - R - g.origin[a]) * g.inv), 0, g.dims[a] - 1);
@@ -159,52 +166,56 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
     const float R2 = R * R;
     const float cell = 1.0f / g.inv;
     const float nh = -0.5f * inv_h2;
-    const int col = lane & 15, kg = lane >> 4;   // MFMA operand lane: A row / B column, and k group
+    const int row = lane & 31, kg = lane >> 5;   // MFMA operand lane: A row / B column, and k group
     __shared__ int wcount[2][4];
     __shared__ float sx[KDE_STAGE], sy[KDE_STAGE], sz[KDE_STAGE];
-    double accd[4][4];
+    double accd[2][16];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) accd[r][q] = 0.0;
+        for (int q = 0; q < 16; ++q) accd[mb][q] = 0.0;
     // Sum the first n (<= KDE_P) staged particles: their factors (one
     // particle per thread), then the MFMA passes over them.  Block-uniform.
     auto consume = [&](int n) {
-        const int n32 = (n + 31) & ~31;            // padded with zero x factors to whole 32-particle steps
-        if (tid < n32) {
+        const int n16 = (n + 15) & ~15;            // padded with zero x factors to whole 16-particle steps
+        if (tid < n16) {
             const bool real = tid < n;
             const float px = real ? sx[tid] : 0.0f, py = real ? sy[tid] : 0.0f, pz = real ? sz[tid] : 0.0f;
 #pragma unroll
-            for (int i = 0; i < KDE_T; i += 2) {
-                const float dx0 = xs[i] - px, dx1 = xs[i + 1] - px;
-                const float dy0 = ys[i] - py, dy1 = ys[i + 1] - py;
-                const float dz0 = zs[i] - pz, dz1 = zs[i + 1] - pz;
-                exs[i][tid] = real ? __expf(nh * (dx0 * dx0)) : 0.0f;
-                exs[i + 1][tid] = real ? __expf(nh * (dx1 * dx1)) : 0.0f;
-                eys[i][tid] = __expf(nh * (dy0 * dy0));
-                eys[i + 1][tid] = __expf(nh * (dy1 * dy1));
-                const float f0 = __expf(nh * (dz0 * dz0)), f1 = __expf(nh * (dz1 * dz1));
-                const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
-                ezh[i][tid] = h0;
-                ezh[i + 1][tid] = h1;
-                ezl[i][tid] = (__bf16)(f0 - (float)h0);
-                ezl[i + 1][tid] = (__bf16)(f1 - (float)h1);
+            for (int i = 0; i < KDE_TX; ++i) {
+                const float dx = xs[i] - px;
+                exs[i][tid] = real ? __expf(nh * (dx * dx)) : 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < KDE_TY; ++i) {
+                const float dy = ys[i] - py;
+                eys[i][tid] = __expf(nh * (dy * dy));
+            }
+#pragma unroll
+            for (int i = 0; i < KDE_TZ; ++i) {
+                const float dz = zs[i] - pz;
+                const float f = __expf(nh * (dz * dz));
+                const __bf16 hz = (__bf16)f;
+                ezh[i][tid] = hz;
+                ezl[i][tid] = (__bf16)(f - (float)hz);
             }
         }
         __syncthreads();
-        floatx4 acc[4];
+        floatx16 acc[2];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int p0 = 0; p0 < n32; p0 += 32) {
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[mb][q] = 0.0f;
+        for (int p0 = 0; p0 < n16; p0 += 16) {
             const int pk = p0 + 8 * kg;               // this lane's 8 particles of the step
-            const float4 xa = *reinterpret_cast<const float4*>(&exs[col][pk]);
-            const float4 xb = *reinterpret_cast<const float4*>(&exs[col][pk + 4]);
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&ezh[col][pk]);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&ezl[col][pk]);
+            const float4 xa = *reinterpret_cast<const float4*>(&exs[row & 15][pk]);
+            const float4 xb = *reinterpret_cast<const float4*>(&exs[row & 15][pk + 4]);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&ezh[row][pk]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&ezl[row][pk]);
             const float xv[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int y = 4 * wv + r;
+            for (int mb = 0; mb < 2; ++mb) {
+                const int y = 4 * wv + 2 * mb + (row >> 4);
                 const float4 ya = *reinterpret_cast<const float4*>(&eys[y][pk]);
                 const float4 yb = *reinterpret_cast<const float4*>(&eys[y][pk + 4]);
                 const float yv[8] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y, yb.z, yb.w};
@@ -213,16 +224,16 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
                 for (int j = 0; j < 8; ++j) av[j] = xv[j] * yv[j];
                 bf16x8 ah, al;
                 split8(av, ah, al);
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[r], 0, 0, 0);
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[r], 0, 0, 0);
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[r], 0, 0, 0);
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[r], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[mb], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) accd[r][q] += (double)acc[r][q];
+            for (int q = 0; q < 16; ++q) accd[mb][q] += (double)acc[mb][q];
         __syncthreads();                               // factors and staged positions are rewritten next
     };
     int staged = 0, round = 0;                         // block-uniform
@@ -245,10 +256,22 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
         // cells (cx, cy, z0..z1) are consecutive ids: their points are one run
         const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + z0;
         const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + z1;
+        // the run's ends: the lanes of each wave read the cells' ranges at
+        // once (64 cells a round) and reduce them, so every wave has them
         int s = 0x7fffffff, e = 0;
-        for (int id = id0; id <= id1; ++id) {
-            const int cs = g.cstart[id], ce = g.cend[id];
-            if (ce > cs) { s = min(s, cs); e = max(e, ce); }
+        for (int base = id0; base <= id1; base += 64) {
+            const int id = base + lane;
+            int s2 = 0x7fffffff, e2 = 0;
+            if (id <= id1) {
+                const int cs = g.cstart[id], ce = g.cend[id];
+                if (ce > cs) { s2 = cs; e2 = ce; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                s2 = min(s2, __shfl_xor(s2, o));
+                e2 = max(e2, __shfl_xor(e2, o));
+            }
+            s = min(s, s2);
+            e = max(e, e2);
         }
         if (s >= e) continue;                          // no particles in these cells (s is still INT_MAX)
         // this thread's particle of the next round, loaded one round ahead
@@ -306,18 +329,50 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
         __syncthreads();
         consume(staged);
     }
-    // D of row r (y = 4 wv + r): lane holds x = 4 (lane / 16) + q, z = lane % 16
-    const int iz = bz * KDE_T + col;
+    // D of M block mb: lane holds z = lane % 32 and, for q = 0 .. 15, the
+    // block row 8 (q / 4) + 4 (lane / 32) + q % 4 = (x = row % 16,
+    // y = 4 wv + 2 mb + row / 16)
+    // partial != nullptr: this block's sums go to its own slice of partial
+    // (tile-local order ((y * KDE_TX + x) * KDE_TZ + z), every point), and
+    // k_kde_reduce adds the slices of a tile in a fixed order; else they are
+    // added into out with float atomics
+    const int iz = bz * KDE_TZ + row;
+    float* slice = partial ? partial + (size_t)blockIdx.x * (KDE_TX * KDE_TY * KDE_TZ) : nullptr;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int iy = by * KDE_T + 4 * wv + r;
+    for (int mb = 0; mb < 2; ++mb) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int ix = bx * KDE_T + 4 * kg + q;
-            if (ix < nx && iy < ny && iz < nz && accd[r][q] != 0.0)
-                unsafeAtomicAdd(&out[((size_t)iy * nx + ix) * nz + iz], (float)(accd[r][q] * scale));
+        for (int q = 0; q < 16; ++q) {
+            const int r = 8 * (q >> 2) + 4 * kg + (q & 3);
+            const int lx = r & 15, ly = 4 * wv + 2 * mb + (r >> 4);
+            const float v = (float)(accd[mb][q] * scale);
+            if (slice) {
+                slice[(ly * KDE_TX + lx) * KDE_TZ + row] = v;
+            } else {
+                const int ix = bx * KDE_TX + lx, iy = by * KDE_TY + ly;
+                if (ix < nx && iy < ny && iz < nz && accd[mb][q] != 0.0)
+                    unsafeAtomicAdd(&out[((size_t)iy * nx + ix) * nz + iz], v);
+            }
         }
     }
+}
+
+// out (ny, nx, nz) = the sum over a tile's nsplit slices of partial, in
+// slice order (float64), so the result does not depend on the order the
+// blocks ran in.
+__global__ void __launch_bounds__(256)
+k_kde_reduce(const float* __restrict__ partial, int nx, int ny, int nz, int nsplit, float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)nx * ny * nz) return;
+    const int iz = (int)(i % nz);
+    const size_t t = i / nz;
+    const int ix = (int)(t % nx), iy = (int)(t / nx);
+    const int tiles_z = (nz + KDE_TZ - 1) / KDE_TZ, tiles_x = (nx + KDE_TX - 1) / KDE_TX;
+    const size_t tile = ((size_t)(iy / KDE_TY) * tiles_x + ix / KDE_TX) * tiles_z + iz / KDE_TZ;
+    const int local = ((iy % KDE_TY) * KDE_TX + ix % KDE_TX) * KDE_TZ + iz % KDE_TZ;
+    const float* p = partial + tile * nsplit * (size_t)(KDE_TX * KDE_TY * KDE_TZ) + local;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k) s += (double)p[(size_t)k * (KDE_TX * KDE_TY * KDE_TZ)];
+    out[i] = (float)s;
 }
 
 }  // namespace jt
@@ -340,9 +395,16 @@ extern "C" int jt_kde_eval(const float* P, int64_t n, int32_t dim, const float* 
     return JT_OK;
 }
 
-extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
-                                  const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
-                                  float cutoff, float* out, void* stream) {
+// The tile count and the blocks per tile of a grid.
+static void kde_tiling(int nx, int ny, int nz, long long& tiles, int& nsplit) {
+    tiles = (long long)((ny + KDE_TY - 1) / KDE_TY) * ((nx + KDE_TX - 1) / KDE_TX) * ((nz + KDE_TZ - 1) / KDE_TZ);
+    // about KDE_TARGET_BLOCKS blocks (the chip holds 512 at a time), at most KDE_MAX_SPLIT per tile
+    nsplit = (int)std::max(1LL, std::min((long long)KDE_MAX_SPLIT, (long long)KDE_TARGET_BLOCKS / tiles));
+}
+
+static int kde_cutoff_run(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx, const float* gy,
+                          int32_t ny, const float* gz, int32_t nz, float h, float cutoff, float* out,
+                          float* partial, void* stream) {
     JT_REQUIRE(grid && grid->dim == 3 && grid->key_mode == JT_KEY_X64, "needs a 3-D x64 hash grid");
     JT_REQUIRE(grid->inv > 0.0f && cutoff > 0.0f && (1.0f / grid->inv) >= cutoff * 0.124999f,
                "hash-grid cell size must be >= cutoff / 8");
@@ -355,16 +417,48 @@ extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, cons
     const double hh = (double)h;
     const float inv_h2 = (float)(1.0 / (hh * hh));
     const double norm = pow(2.0 * M_PI, -1.5);
-    const long long tiles = (long long)((ny + KDE_T - 1) / KDE_T) * ((nx + KDE_T - 1) / KDE_T) * ((nz + KDE_T - 1) / KDE_T);
-    // about KDE_TARGET_BLOCKS blocks (the chip holds 768 at a time), at most KDE_MAX_SPLIT per tile
-    const int nsplit = (int)std::max(1LL, std::min((long long)KDE_MAX_SPLIT, (long long)KDE_TARGET_BLOCKS / tiles));
+    long long tiles;
+    int nsplit;
+    kde_tiling(nx, ny, nz, tiles, nsplit);
     JT_REQUIRE(tiles * nsplit < (1LL << 31), "grid too large");
     const size_t npts = (size_t)nx * ny * nz;
-    hipLaunchKernelGGL(k_kde_zero, dim3((unsigned)std::min<size_t>((npts + 255) / 256, 4096)), dim3(256), 0,
-                       (hipStream_t)stream, out, npts);
-    JT_HIP(hipGetLastError());
+    if (!partial) {
+        hipLaunchKernelGGL(k_kde_zero, dim3((unsigned)std::min<size_t>((npts + 255) / 256, 4096)), dim3(256), 0,
+                           (hipStream_t)stream, out, npts);
+        JT_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_kde_tile_mfma, dim3((unsigned)(tiles * nsplit)), dim3(256), 0, (hipStream_t)stream,
-                       gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, nsplit, out);
+                       gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, nsplit, out, partial);
     JT_HIP(hipGetLastError());
+    if (partial) {
+        hipLaunchKernelGGL(k_kde_reduce, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const float*)partial, nx, ny, nz, nsplit, out);
+        JT_HIP(hipGetLastError());
+    }
     return JT_OK;
+}
+
+extern "C" int jt_kde_grid_cutoff(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
+                                  const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
+                                  float cutoff, float* out, void* stream) {
+    return kde_cutoff_run(grid, n_total, gx, nx, gy, ny, gz, nz, h, cutoff, out, nullptr, stream);
+}
+
+extern "C" size_t jt_kde_grid_cutoff_workspace_size(int32_t nx, int32_t ny, int32_t nz) {
+    if (nx < 1 || ny < 1 || nz < 1) return 0;
+    long long tiles;
+    int nsplit;
+    kde_tiling(nx, ny, nz, tiles, nsplit);
+    return (size_t)tiles * nsplit * (KDE_TX * KDE_TY * KDE_TZ) * sizeof(float) + 256;
+}
+
+extern "C" int jt_kde_grid_cutoff_ws(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
+                                     const float* gy, int32_t ny, const float* gz, int32_t nz, float h, float cutoff,
+                                     float* out, void* ws, size_t ws_bytes, void* stream) {
+    const size_t need = jt_kde_grid_cutoff_workspace_size(nx, ny, nz);
+    JT_REQUIRE(need > 0, "bad sizes");
+    Arena arena(ws, ws_bytes);
+    float* partial = arena.take<float>((need - 256) / sizeof(float));
+    if (!partial || !arena.ok()) return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", need, ws_bytes);
+    return kde_cutoff_run(grid, n_total, gx, nx, gy, ny, gz, nz, h, cutoff, out, partial, stream);
 }

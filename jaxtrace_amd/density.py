@@ -435,9 +435,11 @@ class KDEEstimator:
         hg = HashGridNeighbors(self._dev_points(self.P), float(R) / 2, 1, "x64", self.device)
         out = torch.empty((ny, nx, nz), dtype=torch.float32, device=self.device)
         gx, gy, gz = (_to_dev(a, self.device) for a in (x, y, z))
-        L.check(lib.jt_kde_grid_cutoff(C.byref(hg._desc), self.N, L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,
-                                       C.c_float(F32(self.h)), C.c_float(R), L.ptr(out), L.stream_handle()),
-                "jt_kde_grid_cutoff")
+        ws_bytes = int(lib.jt_kde_grid_cutoff_workspace_size(nx, ny, nz))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=self.device)
+        L.check(lib.jt_kde_grid_cutoff_ws(C.byref(hg._desc), self.N, L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,
+                                          C.c_float(F32(self.h)), C.c_float(R), L.ptr(out), L.ptr(ws), ws_bytes,
+                                          L.stream_handle()), "jt_kde_grid_cutoff_ws")
         return out
 
     def evaluate_3d(self):

@@ -190,9 +190,9 @@ def test_kde_exhaustive_and_cutoff(gpu):
 
 
 def test_kde_cutoff_tiles_sampled(gpu):
-    """Cut-off KDE on a 40^3 grid (16^3 tiles, the last one 8 points deep on
-    each axis) over 2^16 mixture particles: 4096 sampled grid points against
-    the float64 exhaustive sum (kde.py:243-253) at the 1e-5 bar."""
+    """Cut-off KDE on a 40^3 grid (16 x 16 x 32 tiles, the last ones 8 points
+    deep in x, y and z) over 2^16 mixture particles: 4096 sampled grid points
+    against the float64 exhaustive sum (kde.py:243-253) at the 1e-5 bar."""
     P = W.gaussian_mixture(1 << 16, seed=11)
     cut = jt.KDEEstimator(P, resolution=40, bounds=((0, 1),) * 3, cutoff_sigmas=7.0)
     d = cut.density_grid_3d().cpu().numpy().ravel()
@@ -203,6 +203,51 @@ def test_kde_cutoff_tiles_sampled(gpu):
     m = want >= 1e-3 * want.max()
     assert m.sum() > 1000
     np.testing.assert_allclose(d[idx][m], want[m], rtol=1e-5)
+
+
+@pytest.mark.parametrize("shape", [(37, 21, 70), (5, 3, 2), (64, 64, 64)])
+def test_kde_cutoff_workspace_path(gpu, shape):
+    """jt_kde_grid_cutoff_ws (per-block slices summed in a fixed order) on
+    grids with ragged tiles in every axis: the same bits on two calls, within
+    2e-6 of the float-atomics entry point, and within 1e-5 of the float64
+    exhaustive sum (kde.py:243-253) at 2048 sampled points above the floor."""
+    import ctypes as C
+    lib = L.load()
+    P = W.gaussian_mixture(1 << 15, seed=40)
+    Pd = torch.from_numpy(P).cuda()
+    h = float(OD.scott_bandwidth(P))
+    R = F32(7.0 * h)
+    hg = jt.HashGridNeighbors(Pd, float(R) / 2, 1, "x64", "cuda")
+    nx, ny, nz = shape
+    axes = [np.linspace(0.05, 0.95, k, dtype=F32) for k in shape]
+    gx, gy, gz = (torch.from_numpy(a).cuda() for a in axes)
+    ws_bytes = int(lib.jt_kde_grid_cutoff_workspace_size(nx, ny, nz))
+    assert ws_bytes >= nx * ny * nz * 4
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    outs = []
+    for _ in range(2):
+        out = torch.full((ny, nx, nz), -1.0, dtype=torch.float32, device="cuda")
+        L.check(lib.jt_kde_grid_cutoff_ws(C.byref(hg._desc), len(P), L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,
+                                          C.c_float(F32(h)), C.c_float(R), L.ptr(out), L.ptr(ws), ws_bytes,
+                                          L.stream_handle()), "jt_kde_grid_cutoff_ws")
+        outs.append(out.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    at = torch.empty((ny, nx, nz), dtype=torch.float32, device="cuda")
+    L.check(lib.jt_kde_grid_cutoff(C.byref(hg._desc), len(P), L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,
+                                   C.c_float(F32(h)), C.c_float(R), L.ptr(at), L.stream_handle()),
+            "jt_kde_grid_cutoff")
+    at = at.cpu().numpy()
+    np.testing.assert_allclose(outs[0], at, rtol=2e-6, atol=1e-7 * np.abs(at).max())
+    with pytest.raises(MemoryError, match="workspace too small"):
+        L.check(lib.jt_kde_grid_cutoff_ws(C.byref(hg._desc), len(P), L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,
+                                          C.c_float(F32(h)), C.c_float(R), L.ptr(out), L.ptr(ws), ws_bytes - 512,
+                                          L.stream_handle()), "jt_kde_grid_cutoff_ws")
+    X, Y, Z = np.meshgrid(*axes, indexing="xy")
+    G = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    idx = np.random.default_rng(41).choice(len(G), min(2048, len(G)), replace=False)
+    want = OD.kde_eval(G[idx], P, F32(h), dtype=np.float64)
+    m = want >= 1e-3 * want.max()
+    np.testing.assert_allclose(outs[0].ravel()[idx][m], want[m], rtol=1e-5)
 
 
 @pytest.mark.parametrize("dims", ["3d", "2d"])

@@ -4,10 +4,11 @@
 // k_kde_exhaustive: the reference's all-pairs algorithm.  One thread per
 //   query; particles stream through LDS in tiles of 256 and are reused by all
 //   256 queries of the block.  Terms are accumulated in float64.
-// k_kde_tile_mfma (below; jt_kde_grid_cutoff runs it): the cut-off sum on
-//   the structured grid on the matrix cores, 16 x 16 x 32 tiles.
-//   Each dropped particle is farther than R from the grid point, so each
-//   dropped term is below norm*exp(-R^2/(2h^2)).
+// k_kde_tile_mfma (below; jt_kde_grid_cutoff_ws and jt_kde_grid_cutoff run
+//   it): the cut-off sum on the structured grid on the matrix cores, 16 x 16
+//   x 32 tiles.  Each dropped particle is farther than R from the grid point,
+//   so each dropped term is below norm*exp(-R^2/(2h^2)).
+// k_kde_reduce: the sum of a tile's per-block slices in a fixed order.
 #include "jt_common.h"
 #include <algorithm>
 
@@ -119,8 +120,9 @@ __device__ __forceinline__ void split8(const float v[8], bf16x8& hi, bf16x8& lo)
 }
 
 // nsplit blocks share a tile: block b takes tile b / nsplit and, of the
-// tile's cell columns (cx, cy), those with index = b % nsplit modulo nsplit,
-// adding its partial densities into out (zeroed first).  The particle
+// tile's cell columns (cx, cy), those with index = b % nsplit modulo nsplit;
+// its partial densities go to its own slice of partial (or, with no
+// workspace, are added into out, zeroed first, with atomics).  The particle
 // counts of the tiles' boxes differ widely on clustered data, so one block
 // per tile leaves a tail of heavy blocks running alone.
 // Tiles are KDE_TX x KDE_TY x KDE_TZ grid points; each pass of at most

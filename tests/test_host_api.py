@@ -77,3 +77,17 @@ def test_streamed_frame_writer_ranges_and_layout(tmp_path):
     e.write(1, np.zeros((3, 0, 3), np.float32))
     e.close()
     assert np.load(tmp_path / "e.npy").shape == (4, 0, 3)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says why."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]

@@ -107,8 +107,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
-    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "hashgrid", "nearest", "seeding",
-                                       "vtk"],
+    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "kde2d", "hashgrid", "nearest",
+                                       "seeding", "vtk"],
                     help="run one section only and print its JSON (profiler runs)")
     ap.add_argument("--adjoint-reps", type=int, default=3,
                     help="adjoint repetitions per storage; the last one is reported")
@@ -267,7 +267,7 @@ def cpu_sph_baseline(P, Q, h):
                        f"512-neighbour selection, cubic-spline sum ({q_s:.1f} s)", **info})
 
 
-def cpu_kde_baseline(P, Q, h):
+def cpu_kde_baseline(P, Q, h, total="all 2^24 x 256^3 pairs"):
     """The reference's exhaustive KDE sum (the oracle's kde_eval) for a
     particle sample and a grid-point sample, single process."""
     from oracle import density as OD
@@ -276,7 +276,7 @@ def cpu_kde_baseline(P, Q, h):
     el = time.perf_counter() - t0
     return {"value": len(P) * len(Q) / el, "unit": "pairs/s", "cores": 1, "kind": "port",
             "sample": f"exhaustive Gaussian sum, {len(P)} particles x {len(Q)} grid points ({el:.1f} s); "
-                      f"the reference evaluates all 2^24 x 256^3 pairs", **cpu_host_info()}
+                      f"the reference evaluates {total}", **cpu_host_info()}
 
 
 def cpu_octree_baseline(field, pts, conn, data, times, x0, n_steps=2):
@@ -744,7 +744,7 @@ def bench_octree(torch, jt, W, L, lib, dev, steps=20):
             "workload": "128^3 x 6 jittered tets, 8 snapshots, 2^22 particles, RK4 dt=1e-3, continuous inlet"}
 
 
-def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde")):
+def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde", "kde2d")):
     """Density workload: 2^24 particles from the 64-component Gaussian mixture
     (seed 6).
     SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
@@ -774,7 +774,8 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
     float32 operands) per (grid point, particle) pair of each 16 x 16 x 32 tile and
     the particles within R of the tile's box, the products the kernel's
     matrix cores evaluate (estimated from 512 sampled tiles).
-    parts: which of 'hashgrid', 'sph', 'kde' to run."""
+    KDE 2-D: see _bench_kde2d.
+    parts: which of 'hashgrid', 'sph', 'kde', 'kde2d' to run."""
     F = np.float32
     P = W.gaussian_mixture(1 << 24, seed=6)
     Pd = torch.from_numpy(P).to(dev)
@@ -795,7 +796,11 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
         out["hashgrid_build"] = {"ms": hg_ms, "roofline": hbm_roofline("hashgrid_build", len(P), hg_ms)}
         return out
     if "sph" not in parts:
-        return _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
+        if "kde" in parts:
+            out = _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
+        if "kde2d" in parts:
+            out = _bench_kde2d(torch, jt, L, lib, dev, P, out)
+        return out
     # the hash grid alone (x32 keys, cell h): bounds, keys, radix sort, cell
     # table, sorted positions; after one warm-up build
     jt.HashGridNeighbors(Pd, h, 64, "x32", dev)
@@ -855,9 +860,68 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
         hb, sb = cpu_sph_baseline(P, Qd[torch.from_numpy(qs).to(dev)].cpu().numpy(), h)
         out["sph"]["cpu_baseline"] = sb
         out["sph"]["hashgrid_build"]["cpu_baseline"] = hb
-    if "kde" not in parts:
-        return out
-    return _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
+    if "kde" in parts:
+        out = _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
+    if "kde2d" in parts:
+        out = _bench_kde2d(torch, jt, L, lib, dev, P, out)
+    return out
+
+
+def _bench_kde2d(torch, jt, L, lib, dev, P, out, res=256):
+    """KDEEstimator.evaluate_2d's all-pairs sum (kde.py:205-221, 244-270) on
+    the (x, y) of the 2^24 mixture particles, a res^2 grid over the unit
+    square, Scott bandwidth of those 2-D points: jt_kde_grid2d (both factors
+    per particle, bf16 hi/lo, on the matrix cores; timed: the call, tile
+    kernel and slice sum) beside jt_kde_eval (one thread per grid point over
+    every particle, float64 sums).  Every pair is summed, so the pair count
+    is n x res^2; roofline: 8 flops per pair (four bfloat16 products) against
+    the dense bf16 MFMA peak."""
+    F = np.float32
+    P2 = np.ascontiguousarray(P[:, :2])
+    kde = jt.KDEEstimator(torch.from_numpy(P2).to(dev), resolution=res, bounds=((0, 1), (0, 1)), normalize=False)
+    x = np.linspace(0.0, 1.0, res, dtype=F)
+    e0, e1 = _events(torch)
+    kde.density_grid_2d(x, x, P2)
+    torch.cuda.synchronize()
+    e0.record()
+    D = kde.density_grid_2d(x, x, P2)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    n = len(P2)
+    pairs = float(n) * res * res
+    # the one-thread-per-point kernel on the same grid, for comparison
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    Gd = torch.from_numpy(np.stack([X.ravel(), Y.ravel()], 1).astype(F)).to(dev)
+    Pd2 = kde._dev_points(P2)
+    ref = torch.empty(res * res, dtype=torch.float32, device=dev)
+    e0.record()
+    L.check(lib.jt_kde_eval(L.ptr(Pd2), n, 2, L.ptr(Gd), res * res, C.c_float(F(kde.h)), L.ptr(ref),
+                            L.stream_handle()), "jt_kde_eval")
+    e1.record()
+    torch.cuda.synchronize()
+    ref_ms = e0.elapsed_time(e1)
+    Dn, Rn = D.cpu().numpy().ravel(), ref.cpu().numpy()
+    m = Rn >= 1e-3 * Rn.max()
+    ach = 8.0 * pairs / (ms * 1e-3) / 1e12
+    out["kde_2d"] = {"value": pairs / (ms * 1e-3), "unit": "pairs/s", "ms": ms, "grid": [res, res], "particles": n,
+                     "pairs": pairs, "bandwidth": float(kde.h),
+                     "per_point_kernel_ms": ref_ms,
+                     "max_rel_diff_vs_per_point_kernel_above_floor": float(np.max(np.abs(Dn[m] - Rn[m]) / Rn[m])),
+                     "parity_test": "tests/test_gpu_density.py::test_full_size_kde_grid2d",
+                     "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": ach / MFMA_BF16_TFLOPS, "flops_per_pair": 8,
+                                  "hbm": hbm_roofline("k_kde2d_tile", 1, ms),
+                                  "note": "dense bfloat16 MFMA peak; every pair is summed, each term as the four "
+                                          "bf16 products of the split float32 factors"},
+                     "workload": f"the 2^24 Gaussian-mixture particles' (x, y), {res}^2 grid on the unit square, "
+                                 f"Scott bandwidth, all pairs"}
+    if CPU_BASELINES["on"]:
+        out["kde_2d"]["cpu_baseline"] = cpu_kde_baseline(
+            P2[:1 << 16], np.stack([X.ravel(), Y.ravel()], 1)[np.random.default_rng(17).choice(res * res, 8192,
+                                                                                                replace=False)],
+            float(kde.h), total=f"all 2^24 x {res}^2 pairs")
+    return out
 
 
 def _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out):
@@ -972,6 +1036,7 @@ def main():
               "octree": lambda: bench_octree(torch, jt, W, L, lib, dev),
               "sph": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("sph",)),
               "kde": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde",)),
+              "kde2d": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde2d",)),
               "hashgrid": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("hashgrid",)),
               "nearest": lambda: bench_nearest(torch, jt, W, L, lib, dev),
               "seeding": lambda: bench_seeding(torch, jt, W, L, lib, dev),

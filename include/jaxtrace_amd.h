@@ -309,6 +309,15 @@ size_t jt_kde_grid_cutoff_workspace_size(int32_t nx, int32_t ny, int32_t nz);
 int jt_kde_grid_cutoff_ws(const jt_hashgrid* grid, int64_t n_total, const float* gx, int32_t nx,
                           const float* gy, int32_t ny, const float* gz, int32_t nz, float h,
                           float cutoff, float* out, void* ws, size_t ws_bytes, void* stream);
+/* KDEEstimator.evaluate_2d's all-pairs density (kde.py:205-221, 244-270):
+ * out (ny, nx) [iy][ix] = mean_n (2pi)^-1 exp(-0.5 |(gx[ix], gy[iy]) - P_n|^2
+ * f32(1/h^2)) for the n points P (n, 2), every pair summed, on the bf16 matrix
+ * cores (separable factors, each split into bf16 hi/lo halves).  Workspace:
+ * jt_kde_grid2d_workspace_size(n, nx, ny) bytes (one partial tile per block,
+ * summed in a fixed order: the same bits on every call). */
+size_t jt_kde_grid2d_workspace_size(int64_t n, int32_t nx, int32_t ny);
+int jt_kde_grid2d(const float* P, int64_t n, const float* gx, int32_t nx, const float* gy, int32_t ny,
+                  float h, float* out, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- unstructured FEM field: element octree + P1 tetrahedra ---------------- */
 /* Opaque host octree (OctreeMeshOptimized, octree_fem_interpolator_optimized.py:18-38). */

@@ -377,6 +377,145 @@ k_kde_reduce(const float* __restrict__ partial, int nx, int ny, int nz, int nspl
     out[i] = (float)s;
 }
 
+
+// ---- 2-D grid, all pairs, on the matrix cores --------------------------------
+// The reference's all-pairs sum on evaluate_2d's grid (kde.py:205-221,
+// 244-270) is a matrix product: with the separable Gaussian,
+//   D[y, x] = sum_p ey[y, p] * ex[x, p],  e*[., p] = exp(-0.5 d^2 / h^2),
+// so both operands are per-particle factors and no product has to be formed
+// per wave.  A block takes a 64 x 128 (y x x) tile of the grid and a chunk of
+// the particles; per pass of 64 particles its threads compute the 192 factor
+// rows (two particles a thread, so one packed conversion rounds both), split
+// into bfloat16 hi and lo halves, into LDS, then run 4 MFMA steps:
+// wave w owns the tile's x columns 32w .. 32w + 31 against both 32-row y
+// blocks (v_mfma_f32_32x32x16_bf16, the four hi/lo products of each term).
+// Pass sums are float32 over K2_FOLD passes, then float64; each block writes
+// its partial tile to its own slice and k_kde2d_reduce adds a tile's slices in
+// chunk order.
+#define K2_TY 64                                      // tile rows (y): two 32-row M blocks
+#define K2_TX 128                                     // tile columns (x): one 32-column N block per wave
+#define K2_P 64                                       // particles per pass (4 MFMA steps of 16)
+#ifndef K2_LDP
+#define K2_LDP (K2_P + 4)                             // padded factor row (bf16 elements): 52 KiB, 3 blocks a CU
+#endif
+#define K2_FOLD 4                                     // passes summed in float32 before the float64 fold
+#define K2_TARGET_BLOCKS 4096
+#ifndef K2_FU
+#define K2_FU 8                                       // factor rows unrolled per loop trip
+#endif
+
+__global__ void __launch_bounds__(256, 3)
+k_kde2d_tile(const float* __restrict__ P, int n, int chunk, int nchunk, const float* __restrict__ gx, int nx,
+             const float* __restrict__ gy, int ny, float nh, double scale, float* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) __bf16 eyh[K2_TY][K2_LDP];
+    __shared__ __attribute__((aligned(16))) __bf16 eyl[K2_TY][K2_LDP];
+    __shared__ __attribute__((aligned(16))) __bf16 exh[K2_TX][K2_LDP];
+    __shared__ __attribute__((aligned(16))) __bf16 exl[K2_TX][K2_LDP];
+    __shared__ float tcs[K2_TY + K2_TX];                                      // the tile's y | x
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int c = blockIdx.x % nchunk, t = blockIdx.x / nchunk;
+    const int tiles_x = (nx + K2_TX - 1) / K2_TX;
+    const int bx = t % tiles_x, by = t / tiles_x;
+    // coordinates clamped at the grid's end (those points are not read back)
+    if (tid < K2_TY) tcs[tid] = gy[min(by * K2_TY + tid, ny - 1)];
+    else if (tid < K2_TY + K2_TX) tcs[tid] = gx[min(bx * K2_TX + tid - K2_TY, nx - 1)];
+    __syncthreads();
+    const int row = lane & 31, kg = lane >> 5;         // MFMA operand lane: A row / B column, and k group
+    const float nl2 = nh * 1.44269504f;                // exp(nh d^2) = 2^(nl2 d^2)
+    const int pq = tid & 31, grp = tid >> 5;           // particle pair of a pass, factor rows grp mod 8
+    const long long p0 = (long long)c * chunk;
+    const long long p1 = min(p0 + chunk, (long long)n);
+    floatx16 acc[2];
+    double accd[2][16];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) { acc[mb][q] = 0.0f; accd[mb][q] = 0.0; }
+    // two factors of one row (particles 2 pq and 2 pq + 1) split into bf16 hi
+    // and lo pairs, one 32-bit store each
+    auto put2 = [&](__bf16 (*th)[K2_LDP], __bf16 (*tl)[K2_LDP], int r, float fa, float fb) {
+        const unsigned hp = pk_bf16(fa, fb);
+        const unsigned lp = pk_bf16(fa - bf16_lo_f32(hp), fb - bf16_hi_f32(hp));
+        *reinterpret_cast<unsigned*>(&th[r][2 * pq]) = hp;
+        *reinterpret_cast<unsigned*>(&tl[r][2 * pq]) = lp;
+    };
+    int since_fold = 0;
+    for (long long base = p0; base < p1; base += K2_P) {
+        // factors: thread (pair pq, group grp) takes rows grp, grp + 8, ... of
+        // the 64 y rows and the 128 x rows for particles 2 pq and 2 pq + 1; a
+        // particle past the chunk gets zero x factors
+        const long long ja = base + 2 * pq;
+        const bool ra = ja < p1, rb = ja + 1 < p1;
+        const float xa = ra ? P[2 * ja] : 0.0f, ya = ra ? P[2 * ja + 1] : 0.0f;
+        const float xb = rb ? P[2 * ja + 2] : 0.0f, yb = rb ? P[2 * ja + 3] : 0.0f;
+        const float ka = ra ? 1.0f : 0.0f, kb = rb ? 1.0f : 0.0f;
+#pragma unroll
+        for (int i = 0; i < K2_TY / 8; ++i) {
+            const int r = 8 * i + grp;
+            const float cy = tcs[r];
+            const float da = cy - ya, db = cy - yb;
+            put2(eyh, eyl, r, __builtin_amdgcn_exp2f(nl2 * (da * da)), __builtin_amdgcn_exp2f(nl2 * (db * db)));
+        }
+#pragma unroll K2_FU
+        for (int i = 0; i < K2_TX / 8; ++i) {
+            const int r = 8 * i + grp;
+            const float cx = tcs[K2_TY + r];
+            const float da = cx - xa, db = cx - xb;
+            put2(exh, exl, r, ka * __builtin_amdgcn_exp2f(nl2 * (da * da)), kb * __builtin_amdgcn_exp2f(nl2 * (db * db)));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < K2_P / 16; ++s) {
+            const int pk = 16 * s + 8 * kg;            // this lane's 8 particles of the step
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&exh[32 * wv + row][pk]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&exl[32 * wv + row][pk]);
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&eyh[32 * mb + row][pk]);
+                const bf16x8 al = *reinterpret_cast<const bf16x8*>(&eyl[32 * mb + row][pk]);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[mb], 0, 0, 0);
+            }
+        }
+        __syncthreads();                               // the factor tables are rewritten next
+        if (++since_fold == K2_FOLD) {
+            since_fold = 0;
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) { accd[mb][q] += (double)acc[mb][q]; acc[mb][q] = 0.0f; }
+        }
+    }
+    // D of y block mb: lane holds x = 32 wv + lane % 32 and, for q = 0 .. 15,
+    // y = 32 mb + 8 (q / 4) + 4 (lane / 32) + q % 4
+    float* slice = partial + (size_t)blockIdx.x * (K2_TY * K2_TX);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int ly = 32 * mb + 8 * (q >> 2) + 4 * kg + (q & 3);
+            slice[ly * K2_TX + 32 * wv + row] = (float)((accd[mb][q] + (double)acc[mb][q]) * scale);
+        }
+}
+
+// out (ny, nx) = the sum over a tile's nchunk slices of partial, in chunk
+// order (float64).
+__global__ void __launch_bounds__(256)
+k_kde2d_reduce(const float* __restrict__ partial, int nx, int ny, int nchunk, float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)nx * ny) return;
+    const int ix = (int)(i % nx), iy = (int)(i / nx);
+    const int tiles_x = (nx + K2_TX - 1) / K2_TX;
+    const size_t t = (size_t)(iy / K2_TY) * tiles_x + ix / K2_TX;
+    const int local = (iy % K2_TY) * K2_TX + ix % K2_TX;
+    const float* p = partial + t * nchunk * (size_t)(K2_TY * K2_TX) + local;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; ++k) s += (double)p[(size_t)k * (K2_TY * K2_TX)];
+    out[i] = (float)s;
+}
+
 }  // namespace jt
 
 using namespace jt;
@@ -463,4 +602,48 @@ extern "C" int jt_kde_grid_cutoff_ws(const jt_hashgrid* grid, int64_t n_total, c
     float* partial = arena.take<float>((need - 256) / sizeof(float));
     if (!partial || !arena.ok()) return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", need, ws_bytes);
     return kde_cutoff_run(grid, n_total, gx, nx, gy, ny, gz, nz, h, cutoff, out, partial, stream);
+}
+
+// The tile count and the particle chunks of a 2-D grid evaluation.
+static void kde2d_tiling(int64_t n, int nx, int ny, long long& tiles, int& nchunk, long long& chunk) {
+    tiles = (long long)((ny + K2_TY - 1) / K2_TY) * ((nx + K2_TX - 1) / K2_TX);
+    const long long passes = (n + K2_P - 1) / K2_P;
+    long long want = std::max(1LL, std::min((long long)K2_TARGET_BLOCKS / tiles, passes));
+    chunk = ((n + want - 1) / want + K2_P - 1) / K2_P * K2_P;      // whole passes
+    nchunk = (int)((n + chunk - 1) / chunk);
+}
+
+extern "C" size_t jt_kde_grid2d_workspace_size(int64_t n, int32_t nx, int32_t ny) {
+    if (n < 1 || nx < 1 || ny < 1) return 0;
+    long long tiles, chunk;
+    int nchunk;
+    kde2d_tiling(n, nx, ny, tiles, nchunk, chunk);
+    return (size_t)tiles * nchunk * (K2_TY * K2_TX) * sizeof(float) + 256;
+}
+
+extern "C" int jt_kde_grid2d(const float* P, int64_t n, const float* gx, int32_t nx, const float* gy, int32_t ny,
+                             float h, float* out, void* ws, size_t ws_bytes, void* stream) {
+    JT_REQUIRE(n >= 1 && n < (1LL << 31), "n out of range");
+    JT_REQUIRE(nx >= 1 && ny >= 1 && (int64_t)nx * ny < (1LL << 31), "bad grid sizes");
+    JT_REQUIRE(h > 0.0f, "bandwidth must be positive");
+    JT_REQUIRE(P && gx && gy && out, "NULL pointer");
+    long long tiles, chunk;
+    int nchunk;
+    kde2d_tiling(n, nx, ny, tiles, nchunk, chunk);
+    JT_REQUIRE(tiles * nchunk < (1LL << 31), "grid too large");
+    const size_t need = jt_kde_grid2d_workspace_size(n, nx, ny);
+    Arena arena(ws, ws_bytes);
+    float* partial = arena.take<float>((need - 256) / sizeof(float));
+    if (!partial || !arena.ok()) return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", need, ws_bytes);
+    const double hh = (double)h;
+    const float inv_h2 = (float)(1.0 / (hh * hh));                 // kde.py:251
+    const double norm = 1.0 / (2.0 * M_PI);                         // kernels.py:91, d = 2
+    hipLaunchKernelGGL(k_kde2d_tile, dim3((unsigned)(tiles * nchunk)), dim3(256), 0, (hipStream_t)stream,
+                       P, (int)n, (int)chunk, nchunk, gx, nx, gy, ny, -0.5f * inv_h2, norm / (double)n, partial);
+    JT_HIP(hipGetLastError());
+    const size_t npts = (size_t)nx * ny;
+    hipLaunchKernelGGL(k_kde2d_reduce, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)partial, nx, ny, nchunk, out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
 }

@@ -290,12 +290,16 @@ class KDEEstimator:
     sum.  A number c makes ``evaluate_3d`` sum, for each grid point, every
     particle within c*h of it (and some farther ones: those in the hash-grid
     cells, of size c*h/2, around the point's block of the grid); every
-    dropped term is below (2pi)^(-3/2) exp(-c^2/2), and so is their mean."""
+    dropped term is below (2pi)^(-3/2) exp(-c^2/2), and so is their mean.
+    ``evaluate_2d`` sums every pair; with ``matrix_cores=True`` (the default)
+    it does so as a product of per-particle x and y factors on the bf16
+    matrix cores (each term to about 2^-17 relative), else one thread per
+    grid point over all particles."""
 
     def __init__(self, positions, bandwidth: Optional[float] = None, bandwidth_rule: str = "scott",
                  normalize: bool = True, plane: str = "xy", slab_position: float = 0.0,
                  slab_thickness: float = 0.1, resolution: Union[int, Tuple[int, int, int]] = 100,
-                 bounds=None, cutoff_sigmas: Optional[float] = None, device=None):
+                 bounds=None, cutoff_sigmas: Optional[float] = None, device=None, matrix_cores: bool = True):
         self.device = torch.device(device) if device is not None else L.device()
         P = positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else positions
         P = np.asarray(P, dtype=F32)
@@ -306,6 +310,7 @@ class KDEEstimator:
         self.bandwidth, self.bandwidth_rule, self.normalize = bandwidth, bandwidth_rule, normalize
         self.plane, self.slab_position, self.slab_thickness = plane, slab_position, slab_thickness
         self.resolution, self.bounds, self.cutoff_sigmas = resolution, bounds, cutoff_sigmas
+        self.matrix_cores = bool(matrix_cores)
         if bandwidth is None:                                                  # :81-97
             if bandwidth_rule == "scott":
                 self.h = float(scott_bandwidth(P))
@@ -403,11 +408,28 @@ class KDEEstimator:
         return (np.linspace(lo[0], hi[0], nx, dtype=np.float32), np.linspace(lo[1], hi[1], ny, dtype=np.float32),
                 np.linspace(lo[2], hi[2], nz, dtype=np.float32))
 
+    def density_grid_2d(self, x, y, pts) -> torch.Tensor:
+        """All-pairs densities (kde.py:244-270) on the grid x (nx) by y (ny),
+        (ny, nx) on the device, on the matrix cores."""
+        lib = L.load()
+        P = self._dev_points(np.ascontiguousarray(pts, dtype=F32))
+        gx, gy = (_to_dev(np.ascontiguousarray(a, dtype=F32), self.device) for a in (x, y))
+        nx, ny, n = len(x), len(y), int(P.shape[0])
+        out = torch.empty((ny, nx), dtype=torch.float32, device=self.device)
+        ws_bytes = int(lib.jt_kde_grid2d_workspace_size(n, nx, ny))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=self.device)
+        L.check(lib.jt_kde_grid2d(L.ptr(P), n, L.ptr(gx), nx, L.ptr(gy), ny, C.c_float(F32(self.h)), L.ptr(out),
+                                  L.ptr(ws), ws_bytes, L.stream_handle()), "jt_kde_grid2d")
+        return out
+
     def evaluate_2d(self):
         """kde.py:205-221."""
         X, Y, pts = self._make_grid_2d()
-        grid = np.stack([X.ravel(), Y.ravel()], axis=1)
-        Z = self._evaluate_grid(grid, pts).reshape(X.shape)
+        if self.matrix_cores:
+            Z = self.density_grid_2d(X[0], Y[:, 0], pts).cpu().numpy()
+        else:
+            grid = np.stack([X.ravel(), Y.ravel()], axis=1)
+            Z = self._evaluate_grid(grid, pts).reshape(X.shape)
         if self.normalize:
             dx = (X.max() - X.min()) / max(X.shape[1] - 1, 1)
             dy = (Y.max() - Y.min()) / max(Y.shape[0] - 1, 1)

@@ -250,6 +250,68 @@ def test_kde_cutoff_workspace_path(gpu, shape):
     np.testing.assert_allclose(outs[0].ravel()[idx][m], want[m], rtol=1e-5)
 
 
+@pytest.mark.parametrize("n,shape", [(1, (5, 3)), (63, (37, 21)), (4097, (130, 70)), (70_001, (128, 64)),
+                                     (200_003, (300, 129))])
+def test_kde_grid2d_all_pairs(gpu, n, shape):
+    """jt_kde_grid2d (every pair, on the matrix cores) on grids with ragged
+    tiles and particle counts off the 64-particle passes: within 1e-5 of the
+    float64 all-pairs sum (kde.py:244-270) above 1e-3 of the maximum, the
+    same bits on two calls, within 1e-5 of the one-thread-per-point kernel,
+    and MemoryError on a short workspace."""
+    import ctypes as C
+    lib = L.load()
+    rng = np.random.default_rng(50 + n)
+    P = (0.5 + 0.15 * rng.standard_normal((n, 2))).astype(F32)
+    h = float(OD.scott_bandwidth(P)) if n > 1 else 0.05
+    nx, ny = shape
+    x = np.linspace(0.0, 1.0, nx, dtype=F32)
+    y = np.linspace(0.1, 0.9, ny, dtype=F32)
+    Pd, gx, gy = (torch.from_numpy(a).cuda() for a in (P, x, y))
+    ws_bytes = int(lib.jt_kde_grid2d_workspace_size(n, nx, ny))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    outs = []
+    for _ in range(2):
+        out = torch.full((ny, nx), -1.0, dtype=torch.float32, device="cuda")
+        L.check(lib.jt_kde_grid2d(L.ptr(Pd), n, L.ptr(gx), nx, L.ptr(gy), ny, C.c_float(F32(h)), L.ptr(out),
+                                  L.ptr(ws), ws_bytes, L.stream_handle()), "jt_kde_grid2d")
+        outs.append(out.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    G = np.stack([X.ravel(), Y.ravel()], 1)
+    idx = np.random.default_rng(51).choice(len(G), min(1500, len(G)), replace=False)
+    want = OD.kde_eval(G[idx], P, F32(h), dtype=np.float64)
+    got = outs[0].ravel()[idx]
+    m = want >= 1e-3 * want.max()
+    np.testing.assert_allclose(got[m], want[m], rtol=1e-5)
+    np.testing.assert_allclose(got[~m], want[~m], rtol=0, atol=1e-5 * 1e-3 * want.max())
+    ref = torch.empty(nx * ny, dtype=torch.float32, device="cuda")
+    Gd = torch.from_numpy(np.ascontiguousarray(G.astype(F32))).cuda()
+    L.check(lib.jt_kde_eval(L.ptr(Pd), n, 2, L.ptr(Gd), nx * ny, C.c_float(F32(h)), L.ptr(ref), L.stream_handle()),
+            "jt_kde_eval")
+    ref = ref.cpu().numpy().reshape(ny, nx)
+    np.testing.assert_allclose(outs[0], ref, rtol=1e-5, atol=1e-8 * ref.max())
+    with pytest.raises(MemoryError, match="workspace too small"):
+        L.check(lib.jt_kde_grid2d(L.ptr(Pd), n, L.ptr(gx), nx, L.ptr(gy), ny, C.c_float(F32(h)), L.ptr(out),
+                                  L.ptr(ws), ws_bytes - 512, L.stream_handle()), "jt_kde_grid2d")
+
+
+def test_full_size_kde_grid2d(gpu):
+    """The 2-D benchmark workload: 2^24 mixture particles' (x, y) on a 256^2
+    grid, every pair; 24 sampled grid points against the float64 sum."""
+    P = np.ascontiguousarray(W.gaussian_mixture(1 << 24, seed=6)[:, :2])
+    kde = jt.KDEEstimator(torch.from_numpy(P).cuda(), resolution=256, bounds=((0, 1), (0, 1)), normalize=False)
+    x = np.linspace(0, 1, 256, dtype=F32)
+    D = kde.density_grid_2d(x, x, P).cpu().numpy()
+    rng = np.random.default_rng(52)
+    iy, ix = rng.integers(0, 256, 24), rng.integers(0, 256, 24)
+    G = np.stack([x[ix], x[iy]], 1)
+    want = np.concatenate([OD.kde_eval(G, P[k:k + (1 << 21)], F32(kde.h), dtype=np.float64) * (1 << 21)
+                           for k in range(0, 1 << 24, 1 << 21)]).reshape(8, 24).sum(0) / (1 << 24)
+    m = want >= 1e-3 * want.max()
+    assert m.sum() >= 12
+    np.testing.assert_allclose(D[iy, ix][m], want[m], rtol=1e-5)
+
+
 @pytest.mark.parametrize("dims", ["3d", "2d"])
 def test_sph_query_tiles_match_per_query_path(gpu, dims):
     """x32 keys, uniform masses: the column-tiled evaluation

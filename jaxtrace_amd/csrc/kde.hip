@@ -103,7 +103,14 @@ __device__ __forceinline__ unsigned pk_bf16(float a, float b) {
     const bf16x2 v = {(__bf16)a, (__bf16)b};
     return __builtin_bit_cast(unsigned, v);
 }
-__device__ __forceinline__ float bf16_lo_f32(unsigned w) { return __uint_as_float(w << 16); }
+// The low element as float32: one shift of the packed word, written as one
+// instruction (as plain C the compiler rounds the element a second time to
+// form it).
+__device__ __forceinline__ float bf16_lo_f32(unsigned w) {
+    unsigned r;
+    asm("v_lshlrev_b32 %0, 16, %1" : "=v"(r) : "v"(w));
+    return __uint_as_float(r);
+}
 __device__ __forceinline__ float bf16_hi_f32(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
 
 // Eight float32 values split into their bfloat16 hi and lo halves.
@@ -443,12 +450,11 @@ k_kde2d_tile(const float* __restrict__ P, int n, int chunk, int nchunk, const fl
     for (long long base = p0; base < p1; base += K2_P) {
         // factors: thread (pair pq, group grp) takes rows grp, grp + 8, ... of
         // the 64 y rows and the 128 x rows for particles 2 pq and 2 pq + 1; a
-        // particle past the chunk gets zero x factors
+        // particle past the chunk sits at 1e30, where its factors are exactly 0
         const long long ja = base + 2 * pq;
         const bool ra = ja < p1, rb = ja + 1 < p1;
-        const float xa = ra ? P[2 * ja] : 0.0f, ya = ra ? P[2 * ja + 1] : 0.0f;
-        const float xb = rb ? P[2 * ja + 2] : 0.0f, yb = rb ? P[2 * ja + 3] : 0.0f;
-        const float ka = ra ? 1.0f : 0.0f, kb = rb ? 1.0f : 0.0f;
+        const float xa = ra ? P[2 * ja] : 1e30f, ya = ra ? P[2 * ja + 1] : 1e30f;
+        const float xb = rb ? P[2 * ja + 2] : 1e30f, yb = rb ? P[2 * ja + 3] : 1e30f;
 #pragma unroll
         for (int i = 0; i < K2_TY / 8; ++i) {
             const int r = 8 * i + grp;
@@ -461,7 +467,7 @@ k_kde2d_tile(const float* __restrict__ P, int n, int chunk, int nchunk, const fl
             const int r = 8 * i + grp;
             const float cx = tcs[K2_TY + r];
             const float da = cx - xa, db = cx - xb;
-            put2(exh, exl, r, ka * __builtin_amdgcn_exp2f(nl2 * (da * da)), kb * __builtin_amdgcn_exp2f(nl2 * (db * db)));
+            put2(exh, exl, r, __builtin_amdgcn_exp2f(nl2 * (da * da)), __builtin_amdgcn_exp2f(nl2 * (db * db)));
         }
         __syncthreads();
 #pragma unroll

@@ -107,8 +107,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
-    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "kde2d", "hashgrid", "nearest",
-                                       "seeding", "vtk"],
+    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "kde2d", "kde3d", "hashgrid",
+                                       "nearest", "seeding", "vtk"],
                     help="run one section only and print its JSON (profiler runs)")
     ap.add_argument("--adjoint-reps", type=int, default=3,
                     help="adjoint repetitions per storage; the last one is reported")
@@ -744,7 +744,7 @@ def bench_octree(torch, jt, W, L, lib, dev, steps=20):
             "workload": "128^3 x 6 jittered tets, 8 snapshots, 2^22 particles, RK4 dt=1e-3, continuous inlet"}
 
 
-def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde", "kde2d")):
+def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde", "kde2d", "kde3d")):
     """Density workload: 2^24 particles from the 64-component Gaussian mixture
     (seed 6).
     SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
@@ -774,8 +774,8 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
     float32 operands) per (grid point, particle) pair of each 16 x 16 x 32 tile and
     the particles within R of the tile's box, the products the kernel's
     matrix cores evaluate (estimated from 512 sampled tiles).
-    KDE 2-D: see _bench_kde2d.
-    parts: which of 'hashgrid', 'sph', 'kde', 'kde2d' to run."""
+    KDE 2-D and all-pairs 3-D: see _bench_kde2d and _bench_kde3d.
+    parts: which of 'hashgrid', 'sph', 'kde', 'kde2d', 'kde3d' to run."""
     F = np.float32
     P = W.gaussian_mixture(1 << 24, seed=6)
     Pd = torch.from_numpy(P).to(dev)
@@ -800,6 +800,8 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
             out = _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
         if "kde2d" in parts:
             out = _bench_kde2d(torch, jt, L, lib, dev, P, out)
+        if "kde3d" in parts:
+            out = _bench_kde3d(torch, jt, L, lib, dev, P, out)
         return out
     # the hash grid alone (x32 keys, cell h): bounds, keys, radix sort, cell
     # table, sorted positions; after one warm-up build
@@ -864,6 +866,54 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
         out = _bench_kde(torch, jt, L, lib, dev, P, Pd, kde_res, out)
     if "kde2d" in parts:
         out = _bench_kde2d(torch, jt, L, lib, dev, P, out)
+    if "kde3d" in parts:
+        out = _bench_kde3d(torch, jt, L, lib, dev, P, out)
+    return out
+
+
+def _bench_kde3d(torch, jt, L, lib, dev, P, out, res=100):
+    """KDEEstimator.evaluate_3d's default (no cut-off: every pair, kde.py:223-240,
+    244-270) at the reference's default resolution 100 over the 2^24 mixture
+    particles, Scott bandwidth: jt_kde_grid3d (the 16 x 16 x 32 tile kernel
+    over chunks of the particles, bf16 hi/lo products on the matrix cores;
+    timed: the call) beside jt_kde_eval (one thread per grid point over every
+    particle).  Roofline: 8 flops per pair against the dense bf16 MFMA peak
+    (the tiles' padding past the grid's end is not counted)."""
+    F = np.float32
+    kde = jt.KDEEstimator(torch.from_numpy(P).to(dev), resolution=res, bounds=((0, 1),) * 3, normalize=False)
+    e0, e1 = _events(torch)
+    kde.density_grid_3d()
+    torch.cuda.synchronize()
+    e0.record()
+    D = kde.density_grid_3d()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    n = len(P)
+    pairs = float(n) * res ** 3
+    x, y, z = kde._axes_3d()
+    X, Y, Z = np.meshgrid(x, y, z, indexing="xy")
+    Gd = torch.from_numpy(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(F)).to(dev)
+    ref = torch.empty(res ** 3, dtype=torch.float32, device=dev)
+    e0.record()
+    L.check(lib.jt_kde_eval(L.ptr(kde._dev_points(kde.P)), n, 3, L.ptr(Gd), res ** 3, C.c_float(F(kde.h)), L.ptr(ref),
+                            L.stream_handle()), "jt_kde_eval")
+    e1.record()
+    torch.cuda.synchronize()
+    ref_ms = e0.elapsed_time(e1)
+    Dn, Rn = D.cpu().numpy().ravel(), ref.cpu().numpy()
+    m = Rn >= 1e-3 * Rn.max()
+    ach = 8.0 * pairs / (ms * 1e-3) / 1e12
+    out["kde_3d_all_pairs"] = {
+        "value": pairs / (ms * 1e-3), "unit": "pairs/s", "ms": ms, "grid": [res] * 3, "particles": n, "pairs": pairs,
+        "bandwidth": float(kde.h), "per_point_kernel_ms": ref_ms,
+        "max_rel_diff_vs_per_point_kernel_above_floor": float(np.max(np.abs(Dn[m] - Rn[m]) / Rn[m])),
+        "parity_test": "tests/test_gpu_density.py::test_full_size_kde_grid3d",
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / MFMA_BF16_TFLOPS, "flops_per_pair": 8,
+                     "note": "dense bfloat16 MFMA peak; every pair summed, each term as four bf16 products"},
+        "workload": f"2^24 Gaussian-mixture particles, {res}^3 grid (the reference's default resolution), "
+                    f"Scott bandwidth, all pairs"}
     return out
 
 
@@ -1037,6 +1087,7 @@ def main():
               "sph": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("sph",)),
               "kde": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde",)),
               "kde2d": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde2d",)),
+              "kde3d": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde3d",)),
               "hashgrid": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("hashgrid",)),
               "nearest": lambda: bench_nearest(torch, jt, W, L, lib, dev),
               "seeding": lambda: bench_seeding(torch, jt, W, L, lib, dev),

@@ -315,6 +315,16 @@ int jt_kde_grid_cutoff_ws(const jt_hashgrid* grid, int64_t n_total, const float*
  * cores (separable factors, each split into bf16 hi/lo halves).  Workspace:
  * jt_kde_grid2d_workspace_size(n, nx, ny) bytes (one partial tile per block,
  * summed in a fixed order: the same bits on every call). */
+/* KDEEstimator.evaluate_3d's all-pairs density (kde.py:223-240, 244-270):
+ * out (ny, nx, nz) = mean_n (2pi)^(-3/2) exp(-0.5 |q - P_n|^2 f32(1/h^2)) for
+ * the n points P (n, 3) and the grid (gx, gy, gz) ('xy' meshgrid order),
+ * every pair summed, on the bf16 matrix cores (the cut-off kernel's tiles
+ * with chunks of the points instead of hash-grid columns).  Workspace:
+ * jt_kde_grid3d_workspace_size(n, nx, ny, nz) bytes; the result is the same
+ * on every call. */
+size_t jt_kde_grid3d_workspace_size(int64_t n, int32_t nx, int32_t ny, int32_t nz);
+int jt_kde_grid3d(const float* P, int64_t n, const float* gx, int32_t nx, const float* gy, int32_t ny,
+                  const float* gz, int32_t nz, float h, float* out, void* ws, size_t ws_bytes, void* stream);
 size_t jt_kde_grid2d_workspace_size(int64_t n, int32_t nx, int32_t ny);
 int jt_kde_grid2d(const float* P, int64_t n, const float* gx, int32_t nx, const float* gy, int32_t ny,
                   float h, float* out, void* ws, size_t ws_bytes, void* stream);

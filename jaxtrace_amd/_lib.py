@@ -146,6 +146,9 @@ _SIGS = {
                                      C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, C.c_float,
                                      C.c_void_p, C.c_void_p]),
     "jt_kde_grid_cutoff_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
+    "jt_kde_grid3d_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+    "jt_kde_grid3d": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                C.c_int32, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "jt_kde_grid2d_workspace_size": (C.c_size_t, [C.c_int64, C.c_int32, C.c_int32]),
     "jt_kde_grid2d": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float,
                                 C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

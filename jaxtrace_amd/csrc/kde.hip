@@ -144,7 +144,8 @@ __device__ __forceinline__ void split8(const float v[8], bf16x8& hi, bf16x8& lo)
 __global__ void __launch_bounds__(256)
 k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ gy, int ny,
                 const float* __restrict__ gz, int nz, KGrid g, float R, float inv_h2, double scale,
-                int nsplit, float* __restrict__ out, float* __restrict__ partial) {
+                int nsplit, float* __restrict__ out, float* __restrict__ partial,
+                const float* __restrict__ pall, int n_all, int chunk) {
     __shared__ __attribute__((aligned(16))) float exs[KDE_TX][KDE_LDP];       // x factors [x][p]
     __shared__ __attribute__((aligned(16))) float eys[KDE_TY][KDE_LDP];       // y factors [y][p]
     __shared__ __attribute__((aligned(16))) __bf16 ezh[KDE_TZ][KDE_LDP];      // z factors [z][p], hi half
@@ -248,62 +249,28 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
         __syncthreads();                               // factors and staged positions are rewritten next
     };
     int staged = 0, round = 0;                         // block-uniform
-    const int ncy = c1[1] - c0[1] + 1, ncol = (c1[0] - c0[0] + 1) * ncy;
-    for (int k = part; k < ncol; k += nsplit) {
-        const int cx = c0[0] + k / ncy, cy = c0[1] + k % ncy;
-        // the column's (x, y) distance to the box, from its cells' bounds
-        // (widened by a thousandth of a cell against rounding)
-        const float slack = 1e-3f * cell;
-        const float xl = g.origin[0] + (float)cx * cell - slack, xh = g.origin[0] + (float)(cx + 1) * cell + slack;
-        const float yl = g.origin[1] + (float)cy * cell - slack, yh = g.origin[1] + (float)(cy + 1) * cell + slack;
-        const float gx_ = fmaxf(fmaxf(lo[0] - xh, xl - hi[0]), 0.0f);
-        const float gy_ = fmaxf(fmaxf(lo[1] - yh, yl - hi[1]), 0.0f);
-        const float rem = R2 - (gx_ * gx_ + gy_ * gy_);
-        if (rem < 0.0f) continue;                      // every point of the column is farther than R
-        const float rz = sqrtf(rem) + slack;
-        const int z0 = max(c0[2], clampi((int)floorf((lo[2] - rz - g.origin[2]) * g.inv), 0, g.dims[2] - 1));
-        const int z1 = min(c1[2], clampi((int)floorf((hi[2] + rz - g.origin[2]) * g.inv), 0, g.dims[2] - 1));
-        if (z0 > z1) continue;
-        // cells (cx, cy, z0..z1) are consecutive ids: their points are one run
-        const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + z0;
-        const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + z1;
-        // the run's ends: the lanes of each wave read the cells' ranges at
-        // once (64 cells a round) and reduce them, so every wave has them
-        int s = 0x7fffffff, e = 0;
-        for (int base = id0; base <= id1; base += 64) {
-            const int id = base + lane;
-            int s2 = 0x7fffffff, e2 = 0;
-            if (id <= id1) {
-                const int cs = g.cstart[id], ce = g.cend[id];
-                if (ce > cs) { s2 = cs; e2 = ce; }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                s2 = min(s2, __shfl_xor(s2, o));
-                e2 = max(e2, __shfl_xor(e2, o));
-            }
-            s = min(s, s2);
-            e = max(e, e2);
-        }
-        if (s >= e) continue;                          // no particles in these cells (s is still INT_MAX)
+    // Stage the points [s, e) of src (x, y, z records), with cull those
+    // within R of the tile's box, consuming every KDE_P of them.
+    auto stage_run = [&](const float* __restrict__ src, int s, int e, bool cull) {
         // this thread's particle of the next round, loaded one round ahead
         float qx = 0.f, qy = 0.f, qz = 0.f;
         if ((long long)s + tid < (long long)e) {
-            qx = g.spos[(size_t)(s + tid) * 3]; qy = g.spos[(size_t)(s + tid) * 3 + 1];
-            qz = g.spos[(size_t)(s + tid) * 3 + 2];
+            qx = src[(size_t)(s + tid) * 3]; qy = src[(size_t)(s + tid) * 3 + 1];
+            qz = src[(size_t)(s + tid) * 3 + 2];
         }
         for (int base = s; base < e; base += KDE_BATCH) {
             const int j = base + tid;
             const float px = qx, py = qy, pz = qz;
             if ((long long)j + KDE_BATCH < (long long)e) {
-                qx = g.spos[(size_t)(j + KDE_BATCH) * 3]; qy = g.spos[(size_t)(j + KDE_BATCH) * 3 + 1];
-                qz = g.spos[(size_t)(j + KDE_BATCH) * 3 + 2];
+                qx = src[(size_t)(j + KDE_BATCH) * 3]; qy = src[(size_t)(j + KDE_BATCH) * 3 + 1];
+                qz = src[(size_t)(j + KDE_BATCH) * 3 + 2];
             }
             // keep the particles within R of the tile's box: each one
             // dropped is farther than R from every tile point
             const float dx = fmaxf(fmaxf(lo[0] - px, px - hi[0]), 0.0f);
             const float dy = fmaxf(fmaxf(lo[1] - py, py - hi[1]), 0.0f);
             const float dz = fmaxf(fmaxf(lo[2] - pz, pz - hi[2]), 0.0f);
-            const bool keep = j < e && (dx * dx + dy * dy) + dz * dz <= R2;
+            const bool keep = j < e && (!cull || (dx * dx + dy * dy) + dz * dz <= R2);
             // staging slots in thread order (deterministic): wave counts, then
             // offsets; the counts alternate between two rows, as a wave may
             // write the next round's count while another still reads these
@@ -334,6 +301,52 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
                 if (tid < rest) { sx[tid] = a; sy[tid] = b2; sz[tid] = c2; }
                 staged = rest;
             }
+        }
+    };
+    if (pall) {
+        // every pair: this block's chunk of the particles, in their order
+        const long long s0 = (long long)part * chunk;
+        const long long e0 = min(s0 + chunk, (long long)n_all);
+        if (s0 < e0) stage_run(pall, (int)s0, (int)e0, false);
+    } else {
+        const int ncy = c1[1] - c0[1] + 1, ncol = (c1[0] - c0[0] + 1) * ncy;
+        for (int k = part; k < ncol; k += nsplit) {
+            const int cx = c0[0] + k / ncy, cy = c0[1] + k % ncy;
+            // the column's (x, y) distance to the box, from its cells' bounds
+            // (widened by a thousandth of a cell against rounding)
+            const float slack = 1e-3f * cell;
+            const float xl = g.origin[0] + (float)cx * cell - slack, xh = g.origin[0] + (float)(cx + 1) * cell + slack;
+            const float yl = g.origin[1] + (float)cy * cell - slack, yh = g.origin[1] + (float)(cy + 1) * cell + slack;
+            const float gx_ = fmaxf(fmaxf(lo[0] - xh, xl - hi[0]), 0.0f);
+            const float gy_ = fmaxf(fmaxf(lo[1] - yh, yl - hi[1]), 0.0f);
+            const float rem = R2 - (gx_ * gx_ + gy_ * gy_);
+            if (rem < 0.0f) continue;                      // every point of the column is farther than R
+            const float rz = sqrtf(rem) + slack;
+            const int z0 = max(c0[2], clampi((int)floorf((lo[2] - rz - g.origin[2]) * g.inv), 0, g.dims[2] - 1));
+            const int z1 = min(c1[2], clampi((int)floorf((hi[2] + rz - g.origin[2]) * g.inv), 0, g.dims[2] - 1));
+            if (z0 > z1) continue;
+            // cells (cx, cy, z0..z1) are consecutive ids: their points are one run
+            const int id0 = (cx * g.dims[1] + cy) * g.dims[2] + z0;
+            const int id1 = (cx * g.dims[1] + cy) * g.dims[2] + z1;
+            // the run's ends: the lanes of each wave read the cells' ranges at
+            // once (64 cells a round) and reduce them, so every wave has them
+            int s = 0x7fffffff, e = 0;
+            for (int base = id0; base <= id1; base += 64) {
+                const int id = base + lane;
+                int s2 = 0x7fffffff, e2 = 0;
+                if (id <= id1) {
+                    const int cs = g.cstart[id], ce = g.cend[id];
+                    if (ce > cs) { s2 = cs; e2 = ce; }
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    s2 = min(s2, __shfl_xor(s2, o));
+                    e2 = max(e2, __shfl_xor(e2, o));
+                }
+                s = min(s, s2);
+                e = max(e, e2);
+            }
+            if (s >= e) continue;                          // no particles in these cells (s is still INT_MAX)
+            stage_run(g.spos, s, e, true);
         }
     }
     if (staged > 0) {
@@ -577,7 +590,8 @@ static int kde_cutoff_run(const jt_hashgrid* grid, int64_t n_total, const float*
         JT_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_kde_tile_mfma, dim3((unsigned)(tiles * nsplit)), dim3(256), 0, (hipStream_t)stream,
-                       gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, nsplit, out, partial);
+                       gx, nx, gy, ny, gz, nz, g, cutoff, inv_h2, norm / (double)n_total, nsplit, out, partial,
+                       (const float*)nullptr, 0, 0);
     JT_HIP(hipGetLastError());
     if (partial) {
         hipLaunchKernelGGL(k_kde_reduce, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
@@ -610,6 +624,55 @@ extern "C" int jt_kde_grid_cutoff_ws(const jt_hashgrid* grid, int64_t n_total, c
     float* partial = arena.take<float>((need - 256) / sizeof(float));
     if (!partial || !arena.ok()) return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", need, ws_bytes);
     return kde_cutoff_run(grid, n_total, gx, nx, gy, ny, gz, nz, h, cutoff, out, partial, stream);
+}
+
+// The tile count and the particle chunks of an all-pairs 3-D evaluation.
+static void kde3d_all_tiling(int64_t n, int nx, int ny, int nz, long long& tiles, int& nchunk, long long& chunk) {
+    tiles = (long long)((ny + KDE_TY - 1) / KDE_TY) * ((nx + KDE_TX - 1) / KDE_TX) * ((nz + KDE_TZ - 1) / KDE_TZ);
+    const long long rounds = (n + KDE_BATCH - 1) / KDE_BATCH;
+    const long long want = std::max(1LL, std::min((long long)KDE_TARGET_BLOCKS / tiles, rounds));
+    chunk = ((n + want - 1) / want + KDE_BATCH - 1) / KDE_BATCH * KDE_BATCH;
+    nchunk = (int)((n + chunk - 1) / chunk);
+}
+
+extern "C" size_t jt_kde_grid3d_workspace_size(int64_t n, int32_t nx, int32_t ny, int32_t nz) {
+    if (n < 1 || nx < 1 || ny < 1 || nz < 1) return 0;
+    long long tiles, chunk;
+    int nchunk;
+    kde3d_all_tiling(n, nx, ny, nz, tiles, nchunk, chunk);
+    return (size_t)tiles * nchunk * (KDE_TX * KDE_TY * KDE_TZ) * sizeof(float) + 256;
+}
+
+extern "C" int jt_kde_grid3d(const float* P, int64_t n, const float* gx, int32_t nx, const float* gy, int32_t ny,
+                             const float* gz, int32_t nz, float h, float* out, void* ws, size_t ws_bytes,
+                             void* stream) {
+    JT_REQUIRE(n >= 1 && n < (1LL << 31), "n out of range");
+    JT_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "bad grid sizes");
+    JT_REQUIRE(h > 0.0f, "bandwidth must be positive");
+    JT_REQUIRE(P && gx && gy && gz && out, "NULL pointer");
+    long long tiles, chunk;
+    int nchunk;
+    kde3d_all_tiling(n, nx, ny, nz, tiles, nchunk, chunk);
+    JT_REQUIRE(tiles * nchunk < (1LL << 31), "grid too large");
+    const size_t need = jt_kde_grid3d_workspace_size(n, nx, ny, nz);
+    Arena arena(ws, ws_bytes);
+    float* partial = arena.take<float>((need - 256) / sizeof(float));
+    if (!partial || !arena.ok()) return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", need, ws_bytes);
+    KGrid g{};                                        // no hash grid: the blocks read particle chunks
+    g.inv = 1.0f;
+    g.dims[0] = g.dims[1] = g.dims[2] = 1;
+    const double hh = (double)h;
+    const float inv_h2 = (float)(1.0 / (hh * hh));                 // kde.py:251
+    const double norm = pow(2.0 * M_PI, -1.5);                      // kernels.py:91, d = 3
+    hipLaunchKernelGGL(k_kde_tile_mfma, dim3((unsigned)(tiles * nchunk)), dim3(256), 0, (hipStream_t)stream,
+                       gx, nx, gy, ny, gz, nz, g, 0.0f, inv_h2, norm / (double)n, nchunk, (float*)nullptr, partial,
+                       P, (int)n, (int)chunk);
+    JT_HIP(hipGetLastError());
+    const size_t npts = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_kde_reduce, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)partial, nx, ny, nz, nchunk, out);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
 }
 
 // The tile count and the particle chunks of a 2-D grid evaluation.

@@ -291,10 +291,10 @@ class KDEEstimator:
     particle within c*h of it (and some farther ones: those in the hash-grid
     cells, of size c*h/2, around the point's block of the grid); every
     dropped term is below (2pi)^(-3/2) exp(-c^2/2), and so is their mean.
-    ``evaluate_2d`` sums every pair; with ``matrix_cores=True`` (the default)
-    it does so as a product of per-particle x and y factors on the bf16
-    matrix cores (each term to about 2^-17 relative), else one thread per
-    grid point over all particles."""
+    ``evaluate_2d`` and ``evaluate_3d`` without a cut-off sum every pair;
+    with ``matrix_cores=True`` (the default) they do so as products of
+    per-particle x, y (and z) factors on the bf16 matrix cores (each term to
+    about 2^-17 relative), else one thread per grid point over all particles."""
 
     def __init__(self, positions, bandwidth: Optional[float] = None, bandwidth_rule: str = "scott",
                  normalize: bool = True, plane: str = "xy", slab_position: float = 0.0,
@@ -450,6 +450,16 @@ class KDEEstimator:
         if self.D != 3:
             raise ValueError("evaluate_3d needs 3-D positions")
         if self.cutoff_sigmas is None:
+            if self.matrix_cores:
+                P = self._dev_points(self.P)
+                gx, gy, gz = (_to_dev(a, self.device) for a in (x, y, z))
+                out = torch.empty((ny, nx, nz), dtype=torch.float32, device=self.device)
+                ws_bytes = int(lib.jt_kde_grid3d_workspace_size(self.N, nx, ny, nz))
+                ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=self.device)
+                L.check(lib.jt_kde_grid3d(L.ptr(P), self.N, L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz,
+                                          C.c_float(F32(self.h)), L.ptr(out), L.ptr(ws), ws_bytes, L.stream_handle()),
+                        "jt_kde_grid3d")
+                return out
             X, Y, Z = np.meshgrid(x, y, z, indexing="xy")
             grid = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
             return torch.from_numpy(self._evaluate_grid(grid, self.P).reshape(ny, nx, nz))

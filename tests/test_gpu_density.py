@@ -295,6 +295,70 @@ def test_kde_grid2d_all_pairs(gpu, n, shape):
                                   L.ptr(ws), ws_bytes - 512, L.stream_handle()), "jt_kde_grid2d")
 
 
+@pytest.mark.parametrize("n,shape", [(1, (3, 5, 2)), (255, (17, 33, 40)), (70_001, (40, 40, 40)),
+                                     (300_007, (33, 20, 70))])
+def test_kde_grid3d_all_pairs(gpu, n, shape):
+    """jt_kde_grid3d (every pair, the tile kernel over chunks of the points)
+    on grids with ragged tiles: within 1e-5 of the float64 all-pairs sum
+    (kde.py:244-270) above 1e-3 of the maximum, the same bits on two calls,
+    within 1e-5 of the one-thread-per-point kernel, MemoryError on a short
+    workspace."""
+    import ctypes as C
+    lib = L.load()
+    rng = np.random.default_rng(60 + n)
+    P = (0.5 + 0.15 * rng.standard_normal((n, 3))).astype(F32)
+    h = float(OD.scott_bandwidth(P)) if n > 1 else 0.05
+    nx, ny, nz = shape
+    axes = [np.linspace(0.0, 1.0, nx, dtype=F32), np.linspace(0.1, 0.9, ny, dtype=F32),
+            np.linspace(0.2, 0.8, nz, dtype=F32)]
+    Pd = torch.from_numpy(P).cuda()
+    gx, gy, gz = (torch.from_numpy(a).cuda() for a in axes)
+    ws_bytes = int(lib.jt_kde_grid3d_workspace_size(n, nx, ny, nz))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    outs = []
+    for _ in range(2):
+        out = torch.full((ny, nx, nz), -1.0, dtype=torch.float32, device="cuda")
+        L.check(lib.jt_kde_grid3d(L.ptr(Pd), n, L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz, C.c_float(F32(h)),
+                                  L.ptr(out), L.ptr(ws), ws_bytes, L.stream_handle()), "jt_kde_grid3d")
+        outs.append(out.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    X, Y, Z = np.meshgrid(*axes, indexing="xy")
+    G = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    idx = np.random.default_rng(61).choice(len(G), min(1500, len(G)), replace=False)
+    want = OD.kde_eval(G[idx], P, F32(h), dtype=np.float64)
+    got = outs[0].ravel()[idx]
+    m = want >= 1e-3 * want.max()
+    np.testing.assert_allclose(got[m], want[m], rtol=1e-5)
+    np.testing.assert_allclose(got[~m], want[~m], rtol=0, atol=1e-5 * 1e-3 * want.max())
+    ref = torch.empty(len(G), dtype=torch.float32, device="cuda")
+    Gd = torch.from_numpy(np.ascontiguousarray(G.astype(F32))).cuda()
+    L.check(lib.jt_kde_eval(L.ptr(Pd), n, 3, L.ptr(Gd), len(G), C.c_float(F32(h)), L.ptr(ref), L.stream_handle()),
+            "jt_kde_eval")
+    ref = ref.cpu().numpy().reshape(ny, nx, nz)
+    np.testing.assert_allclose(outs[0], ref, rtol=1e-5, atol=1e-8 * ref.max())
+    with pytest.raises(MemoryError, match="workspace too small"):
+        L.check(lib.jt_kde_grid3d(L.ptr(Pd), n, L.ptr(gx), nx, L.ptr(gy), ny, L.ptr(gz), nz, C.c_float(F32(h)),
+                                  L.ptr(out), L.ptr(ws), ws_bytes - 512, L.stream_handle()), "jt_kde_grid3d")
+
+
+def test_full_size_kde_grid3d(gpu):
+    """evaluate_3d's default (no cut-off: every pair) at the reference's
+    default resolution 100 over the 2^24 mixture particles; 16 sampled grid
+    points against the float64 sum."""
+    P = W.gaussian_mixture(1 << 24, seed=6)
+    kde = jt.KDEEstimator(torch.from_numpy(P).cuda(), resolution=100, bounds=((0, 1),) * 3, normalize=False)
+    D = kde.density_grid_3d().cpu().numpy()
+    x, y, z = kde._axes_3d()
+    rng = np.random.default_rng(62)
+    iy, ix, iz = (rng.integers(0, 100, 16) for _ in range(3))
+    G = np.stack([x[ix], y[iy], z[iz]], 1)
+    want = np.concatenate([OD.kde_eval(G, P[k:k + (1 << 21)], F32(kde.h), dtype=np.float64) * (1 << 21)
+                           for k in range(0, 1 << 24, 1 << 21)]).reshape(8, 16).sum(0) / (1 << 24)
+    m = want >= 1e-3 * want.max()
+    assert m.sum() >= 8
+    np.testing.assert_allclose(D[iy, ix, iz][m], want[m], rtol=1e-5)
+
+
 def test_full_size_kde_grid2d(gpu):
     """The 2-D benchmark workload: 2^24 mixture particles' (x, y) on a 256^2
     grid, every pair; 24 sampled grid points against the float64 sum."""

@@ -107,8 +107,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the octree and density measurements")
-    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "kde2d", "kde3d", "hashgrid",
-                                       "nearest", "seeding", "vtk"],
+    ap.add_argument("--only", choices=["track", "adjoint", "octree", "sph", "kde", "kde2d", "kde3d", "kdepts",
+                                       "hashgrid", "nearest", "seeding", "vtk"],
                     help="run one section only and print its JSON (profiler runs)")
     ap.add_argument("--adjoint-reps", type=int, default=3,
                     help="adjoint repetitions per storage; the last one is reported")
@@ -744,7 +744,7 @@ def bench_octree(torch, jt, W, L, lib, dev, steps=20):
             "workload": "128^3 x 6 jittered tets, 8 snapshots, 2^22 particles, RK4 dt=1e-3, continuous inlet"}
 
 
-def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde", "kde2d", "kde3d")):
+def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=("sph", "kde", "kde2d", "kde3d", "kdepts")):
     """Density workload: 2^24 particles from the 64-component Gaussian mixture
     (seed 6).
     SPH: the 256^3 evaluate_3d grid as queries, 3-D cubic spline, h = 2/256,
@@ -774,8 +774,9 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
     float32 operands) per (grid point, particle) pair of each 16 x 16 x 32 tile and
     the particles within R of the tile's box, the products the kernel's
     matrix cores evaluate (estimated from 512 sampled tiles).
-    KDE 2-D and all-pairs 3-D: see _bench_kde2d and _bench_kde3d.
-    parts: which of 'hashgrid', 'sph', 'kde', 'kde2d', 'kde3d' to run."""
+    KDE 2-D, all-pairs 3-D and evaluate(points): see _bench_kde2d,
+    _bench_kde3d and _bench_kde_points.
+    parts: which of 'hashgrid', 'sph', 'kde', 'kde2d', 'kde3d', 'kdepts' to run."""
     F = np.float32
     P = W.gaussian_mixture(1 << 24, seed=6)
     Pd = torch.from_numpy(P).to(dev)
@@ -802,6 +803,8 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
             out = _bench_kde2d(torch, jt, L, lib, dev, P, out)
         if "kde3d" in parts:
             out = _bench_kde3d(torch, jt, L, lib, dev, P, out)
+        if "kdepts" in parts:
+            out = _bench_kde_points(torch, jt, L, lib, dev, P, out)
         return out
     # the hash grid alone (x32 keys, cell h): bounds, keys, radix sort, cell
     # table, sorted positions; after one warm-up build
@@ -868,6 +871,52 @@ def bench_density(torch, jt, W, L, lib, dev, kde_res=256, sample=1 << 16, parts=
         out = _bench_kde2d(torch, jt, L, lib, dev, P, out)
     if "kde3d" in parts:
         out = _bench_kde3d(torch, jt, L, lib, dev, P, out)
+    if "kdepts" in parts:
+        out = _bench_kde_points(torch, jt, L, lib, dev, P, out)
+    return out
+
+
+def _bench_kde_points(torch, jt, L, lib, dev, P, out, res=50):
+    """KDEEstimator.evaluate(points) (kde.py:128-184, 244-270) as the
+    reference's workflow calls it: a res x res grid of (x, y) points on the
+    z = 0.5 plane, given as 3-D points, over the 2^24 mixture particles,
+    every pair: jt_kde_eval_ws (the particles split over enough blocks to
+    fill the chip, float64 chunk sums added in order; the estimator's path)
+    beside jt_kde_eval (one thread per point over every particle)."""
+    F = np.float32
+    kde = jt.KDEEstimator(torch.from_numpy(P).to(dev), normalize=False)
+    a = np.linspace(0.0, 1.0, res, dtype=F)
+    X, Y = np.meshgrid(a, a)
+    Q = np.stack([X.ravel(), Y.ravel(), np.full(res * res, 0.5, F)], 1).astype(F)
+    e0, e1 = _events(torch)
+    kde.evaluate(Q)
+    Qd = torch.from_numpy(Q).to(dev)
+    Pd = kde._dev_points(kde.P)
+    n, m = len(P), len(Q)
+    ws_bytes = int(lib.jt_kde_eval_workspace_size(n, m))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    o1 = torch.empty(m, dtype=torch.float32, device=dev)
+    o2 = torch.empty(m, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    e0.record()
+    L.check(lib.jt_kde_eval_ws(L.ptr(Pd), n, 3, L.ptr(Qd), m, C.c_float(F(kde.h)), L.ptr(o1), L.ptr(ws), ws_bytes,
+                               L.stream_handle()), "jt_kde_eval_ws")
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    e0.record()
+    L.check(lib.jt_kde_eval(L.ptr(Pd), n, 3, L.ptr(Qd), m, C.c_float(F(kde.h)), L.ptr(o2), L.stream_handle()),
+            "jt_kde_eval")
+    e1.record()
+    torch.cuda.synchronize()
+    ref_ms = e0.elapsed_time(e1)
+    a1, a2 = o1.cpu().numpy(), o2.cpu().numpy()
+    out["kde_points"] = {"value": float(n) * m / (ms * 1e-3), "unit": "pairs/s", "ms": ms, "points": m,
+                         "particles": n, "one_thread_per_point_ms": ref_ms,
+                         "max_rel_diff_vs_one_thread_per_point": float(np.max(np.abs(a1 - a2) / np.maximum(a2, 1e-30))),
+                         "parity_test": "tests/test_gpu_density.py::test_kde_eval_chunked",
+                         "workload": f"evaluate() at a {res}x{res} grid of points on the z = 0.5 plane over the 2^24 "
+                                     f"Gaussian-mixture particles, Scott bandwidth, every pair"}
     return out
 
 
@@ -1088,6 +1137,7 @@ def main():
               "kde": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde",)),
               "kde2d": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde2d",)),
               "kde3d": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kde3d",)),
+              "kdepts": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("kdepts",)),
               "hashgrid": lambda: bench_density(torch, jt, W, L, lib, dev, parts=("hashgrid",)),
               "nearest": lambda: bench_nearest(torch, jt, W, L, lib, dev),
               "seeding": lambda: bench_seeding(torch, jt, W, L, lib, dev),

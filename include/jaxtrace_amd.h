@@ -291,6 +291,13 @@ int jt_sph_eval_tiled(const jt_hashgrid* g, const float* Q, int64_t m, float rad
  * exhaustive over all n points (the reference algorithm). */
 int jt_kde_eval(const float* P, int64_t n, int32_t dim, const float* Q, int64_t m,
                 float h, float* out, void* stream);
+/* jt_kde_eval with the particles split over more blocks when the queries
+ * are few (the per-chunk sums in float64, then added in chunk order):
+ * workspace jt_kde_eval_workspace_size(n, m) bytes (0 when one chunk fills
+ * the chip; the call is then jt_kde_eval).  The estimator's evaluate(). */
+size_t jt_kde_eval_workspace_size(int64_t n, int64_t m);
+int jt_kde_eval_ws(const float* P, int64_t n, int32_t dim, const float* Q, int64_t m, float h, float* out,
+                   void* ws, size_t ws_bytes, void* stream);
 /* The same density on the structured grid of KDEEstimator.evaluate_3d
  * (kde.py:188-240; out is (ny, nx, nz), 'xy' meshgrid order), summing every
  * point within `cutoff` of each grid point (and some farther ones), found

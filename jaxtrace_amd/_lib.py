@@ -142,6 +142,9 @@ _SIGS = {
                                     C.c_void_p]),
     "jt_kde_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64, C.c_float,
                               C.c_void_p, C.c_void_p]),
+    "jt_kde_eval_workspace_size": (C.c_size_t, [C.c_int64, C.c_int64]),
+    "jt_kde_eval_ws": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64, C.c_float, C.c_void_p,
+                                 C.c_void_p, C.c_size_t, C.c_void_p]),
     "jt_kde_grid_cutoff": (C.c_int, [C.POINTER(HashGridDesc), C.c_int64, C.c_void_p, C.c_int32,
                                      C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, C.c_float,
                                      C.c_void_p, C.c_void_p]),

@@ -16,20 +16,22 @@
 namespace jt {
 
 __global__ void __launch_bounds__(256)
-k_kde_exhaustive(const float* __restrict__ Q, int m, int D, const float* __restrict__ P, int n,
-                 float inv_h2, double scale, float* __restrict__ out) {
+k_kde_exhaustive(const float* __restrict__ Q, int m, int D, const float* __restrict__ P, int n, int chunk,
+                 float inv_h2, double scale, float* __restrict__ out, double* __restrict__ partial) {
     __shared__ float sp[256 * 3];
     const int tid = threadIdx.x;
     const int i = blockIdx.x * 256 + tid;
+    // this block's particles: chunk blockIdx.y (all of them with one chunk)
+    const int p0 = blockIdx.y * chunk, p1 = min(n, p0 + chunk);
     float q[3] = {0.f, 0.f, 0.f};
     if (i < m)
         for (int a = 0; a < D; ++a) q[a] = Q[(size_t)i * D + a];
     double acc = 0.0;
-    for (int base = 0; base < n; base += 256) {
+    for (int base = p0; base < p1; base += 256) {
         const int j = base + tid;
-        for (int a = 0; a < D; ++a) sp[tid * 3 + a] = (j < n) ? P[(size_t)j * D + a] : 0.0f;
+        for (int a = 0; a < D; ++a) sp[tid * 3 + a] = (j < p1) ? P[(size_t)j * D + a] : 0.0f;
         __syncthreads();
-        const int cnt = min(256, n - base);
+        const int cnt = min(256, p1 - base);
         for (int k = 0; k < cnt; ++k) {
             float dx = q[0] - sp[k * 3], dy = q[1] - sp[k * 3 + 1];
             float r2 = dx * dx + dy * dy;
@@ -38,7 +40,21 @@ k_kde_exhaustive(const float* __restrict__ Q, int m, int D, const float* __restr
         }
         __syncthreads();
     }
-    if (i < m) out[i] = (float)(acc * scale);
+    if (i < m) {
+        if (partial) partial[(size_t)blockIdx.y * m + i] = acc;
+        else out[i] = (float)(acc * scale);
+    }
+}
+
+// out[i] = scale * the sum of the chunks' partial sums of query i, in chunk
+// order.
+__global__ void __launch_bounds__(256)
+k_kde_chunk_sum(const double* __restrict__ partial, int m, int nchunk, double scale, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[(size_t)c * m + i];
+    out[i] = (float)(s * scale);
 }
 
 struct KGrid {
@@ -552,7 +568,52 @@ extern "C" int jt_kde_eval(const float* P, int64_t n, int32_t dim, const float* 
     const float inv_h2 = (float)(1.0 / (hh * hh));                 // kde.py:251
     const double norm = pow(2.0 * M_PI, -0.5 * dim);                // kernels.py:91
     hipLaunchKernelGGL(k_kde_exhaustive, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       Q, (int)m, dim, P, (int)n, inv_h2, norm / (double)n, out);
+                       Q, (int)m, dim, P, (int)n, (int)n, inv_h2, norm / (double)n, out, (double*)nullptr);
+    JT_HIP(hipGetLastError());
+    return JT_OK;
+}
+
+// The particle chunks of a per-point evaluation: enough blocks to fill the
+// chip when the queries are few.
+#define KDE_EVAL_TARGET_BLOCKS 2048
+static void kde_eval_chunks(int64_t n, int64_t m, int& nchunk, long long& chunk) {
+    const long long qblocks = std::max(1LL, (long long)((m + 255) / 256));
+    const long long want = std::max(1LL, std::min((long long)KDE_EVAL_TARGET_BLOCKS / qblocks, (long long)(n + 255) / 256));
+    chunk = ((n + want - 1) / want + 255) / 256 * 256;
+    nchunk = (int)((n + chunk - 1) / chunk);
+}
+
+extern "C" size_t jt_kde_eval_workspace_size(int64_t n, int64_t m) {
+    if (n < 1 || m < 1) return 0;
+    int nchunk;
+    long long chunk;
+    kde_eval_chunks(n, m, nchunk, chunk);
+    return nchunk > 1 ? (size_t)nchunk * m * sizeof(double) + 256 : 0;
+}
+
+extern "C" int jt_kde_eval_ws(const float* P, int64_t n, int32_t dim, const float* Q, int64_t m, float h,
+                              float* out, void* ws, size_t ws_bytes, void* stream) {
+    JT_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    JT_REQUIRE(n >= 1 && n < (1LL << 31) && m >= 0 && m < (1LL << 31), "sizes out of range");
+    JT_REQUIRE(h > 0.0f, "bandwidth must be positive");
+    if (m == 0) return JT_OK;
+    JT_REQUIRE(P && Q && out, "NULL pointer");
+    int nchunk;
+    long long chunk;
+    kde_eval_chunks(n, m, nchunk, chunk);
+    if (nchunk == 1) return jt_kde_eval(P, n, dim, Q, m, h, out, stream);
+    const size_t need = jt_kde_eval_workspace_size(n, m);
+    Arena arena(ws, ws_bytes);
+    double* partial = arena.take<double>((size_t)nchunk * m);
+    if (!partial || !arena.ok()) return fail(JT_ENOMEM, "workspace too small: need %zu, got %zu", need, ws_bytes);
+    const double hh = (double)h;
+    const float inv_h2 = (float)(1.0 / (hh * hh));                 // kde.py:251
+    const double norm = pow(2.0 * M_PI, -0.5 * dim);                // kernels.py:91
+    hipLaunchKernelGGL(k_kde_exhaustive, dim3((unsigned)((m + 255) / 256), (unsigned)nchunk), dim3(256), 0,
+                       (hipStream_t)stream, Q, (int)m, dim, P, (int)n, (int)chunk, inv_h2, 0.0, out, partial);
+    JT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_kde_chunk_sum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const double*)partial, (int)m, nchunk, norm / (double)n, out);
     JT_HIP(hipGetLastError());
     return JT_OK;
 }

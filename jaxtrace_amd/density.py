@@ -353,8 +353,10 @@ class KDEEstimator:
         Q = _to_dev(np.ascontiguousarray(np.asarray(grid_pts, dtype=F32)), self.device)
         M, d = int(Q.shape[0]), int(Q.shape[1])
         out = torch.empty(M, dtype=torch.float32, device=self.device)
-        L.check(lib.jt_kde_eval(L.ptr(P), int(P.shape[0]), d, L.ptr(Q), M, C.c_float(F32(self.h)),
-                                L.ptr(out), L.stream_handle()), "jt_kde_eval")
+        ws_bytes = int(lib.jt_kde_eval_workspace_size(int(P.shape[0]), M))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=self.device)
+        L.check(lib.jt_kde_eval_ws(L.ptr(P), int(P.shape[0]), d, L.ptr(Q), M, C.c_float(F32(self.h)),
+                                   L.ptr(out), L.ptr(ws), ws_bytes, L.stream_handle()), "jt_kde_eval_ws")
         return out.cpu().numpy()
 
     def evaluate(self, points) -> np.ndarray:

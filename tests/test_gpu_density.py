@@ -341,6 +341,46 @@ def test_kde_grid3d_all_pairs(gpu, n, shape):
                                   L.ptr(out), L.ptr(ws), ws_bytes - 512, L.stream_handle()), "jt_kde_grid3d")
 
 
+@pytest.mark.parametrize("n,m,dim", [(1, 1, 3), (1000, 7, 2), (200_003, 2500, 3), (1 << 20, 300, 2),
+                                     (70_001, 70_000, 3)])
+def test_kde_eval_chunked(gpu, n, m, dim):
+    """jt_kde_eval_ws (the particles split over blocks when the queries are
+    few, float64 chunk sums added in chunk order) against jt_kde_eval (one
+    float64 sum per query) to 2 float32 ulps, and against the float64 sum of
+    the oracle (kde.py:244-270) to 1e-5 above 1e-3 of the maximum; the same
+    bits on two calls; MemoryError on a short workspace."""
+    import ctypes as C
+    lib = L.load()
+    rng = np.random.default_rng(70 + n)
+    P = (0.5 + 0.15 * rng.standard_normal((n, dim))).astype(F32)
+    Q = rng.uniform(0.0, 1.0, (m, dim)).astype(F32)
+    h = F32(0.03)
+    Pd, Qd = torch.from_numpy(P).cuda(), torch.from_numpy(Q).cuda()
+    ws_bytes = int(lib.jt_kde_eval_workspace_size(n, m))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    outs = []
+    for _ in range(2):
+        out = torch.full((m,), -1.0, dtype=torch.float32, device="cuda")
+        L.check(lib.jt_kde_eval_ws(L.ptr(Pd), n, dim, L.ptr(Qd), m, C.c_float(h), L.ptr(out), L.ptr(ws), ws_bytes,
+                                   L.stream_handle()), "jt_kde_eval_ws")
+        outs.append(out.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    ref = torch.empty(m, dtype=torch.float32, device="cuda")
+    L.check(lib.jt_kde_eval(L.ptr(Pd), n, dim, L.ptr(Qd), m, C.c_float(h), L.ptr(ref), L.stream_handle()),
+            "jt_kde_eval")
+    ref = ref.cpu().numpy()
+    np.testing.assert_allclose(outs[0], ref, rtol=2.5e-7, atol=0)
+    sel = np.random.default_rng(71).choice(m, min(m, 400), replace=False)
+    want = OD.kde_eval(Q[sel], P, h, dtype=np.float64)
+    got = outs[0][sel]
+    mk = want >= max(1e-3 * want.max(), 1e-30)           # float32 holds the value
+    np.testing.assert_allclose(got[mk], want[mk], rtol=1e-5)
+    if ws_bytes > 0:
+        with pytest.raises(MemoryError, match="workspace too small"):
+            L.check(lib.jt_kde_eval_ws(L.ptr(Pd), n, dim, L.ptr(Qd), m, C.c_float(h), L.ptr(out), L.ptr(ws),
+                                       ws_bytes // 4, L.stream_handle()), "jt_kde_eval_ws")
+
+
 def test_full_size_kde_grid3d(gpu):
     """evaluate_3d's default (no cut-off: every pair) at the reference's
     default resolution 100 over the 2^24 mixture particles; 16 sampled grid

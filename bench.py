@@ -917,6 +917,9 @@ def _bench_kde_points(torch, jt, L, lib, dev, P, out, res=50):
                          "parity_test": "tests/test_gpu_density.py::test_kde_eval_chunked",
                          "workload": f"evaluate() at a {res}x{res} grid of points on the z = 0.5 plane over the 2^24 "
                                      f"Gaussian-mixture particles, Scott bandwidth, every pair"}
+    if CPU_BASELINES["on"]:
+        out["kde_points"]["cpu_baseline"] = cpu_kde_baseline(P[:1 << 16], Q, float(kde.h),
+                                                             total=f"all 2^24 x {m} pairs")
     return out
 
 
@@ -963,6 +966,11 @@ def _bench_kde3d(torch, jt, L, lib, dev, P, out, res=100):
                      "note": "dense bfloat16 MFMA peak; every pair summed, each term as four bf16 products"},
         "workload": f"2^24 Gaussian-mixture particles, {res}^3 grid (the reference's default resolution), "
                     f"Scott bandwidth, all pairs"}
+    if CPU_BASELINES["on"]:
+        G = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+        out["kde_3d_all_pairs"]["cpu_baseline"] = cpu_kde_baseline(
+            P[:1 << 16], G[np.random.default_rng(18).choice(len(G), 4096, replace=False)], float(kde.h),
+            total=f"all 2^24 x {res}^3 pairs")
     return out
 
 

@@ -193,7 +193,7 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
     }
     const float R2 = R * R;
     const float cell = 1.0f / g.inv;
-    const float nh = -0.5f * inv_h2;
+    const float nl2 = -0.5f * inv_h2 * 1.44269504f;    // exp(-0.5 d^2 / h^2) = 2^(nl2 d^2)
     const int row = lane & 31, kg = lane >> 5;   // MFMA operand lane: A row / B column, and k group
     __shared__ int wcount[2][4];
     __shared__ float sx[KDE_STAGE], sy[KDE_STAGE], sz[KDE_STAGE];
@@ -208,21 +208,22 @@ k_kde_tile_mfma(const float* __restrict__ gx, int nx, const float* __restrict__ 
         const int n16 = (n + 15) & ~15;            // padded with zero x factors to whole 16-particle steps
         if (tid < n16) {
             const bool real = tid < n;
-            const float px = real ? sx[tid] : 0.0f, py = real ? sy[tid] : 0.0f, pz = real ? sz[tid] : 0.0f;
+            // a padding slot sits at 1e30: its factors are exactly 0
+            const float px = real ? sx[tid] : 1e30f, py = real ? sy[tid] : 1e30f, pz = real ? sz[tid] : 1e30f;
 #pragma unroll
             for (int i = 0; i < KDE_TX; ++i) {
                 const float dx = xs[i] - px;
-                exs[i][tid] = real ? __expf(nh * (dx * dx)) : 0.0f;
+                exs[i][tid] = __builtin_amdgcn_exp2f(nl2 * (dx * dx));
             }
 #pragma unroll
             for (int i = 0; i < KDE_TY; ++i) {
                 const float dy = ys[i] - py;
-                eys[i][tid] = __expf(nh * (dy * dy));
+                eys[i][tid] = __builtin_amdgcn_exp2f(nl2 * (dy * dy));
             }
 #pragma unroll
             for (int i = 0; i < KDE_TZ; ++i) {
                 const float dz = zs[i] - pz;
-                const float f = __expf(nh * (dz * dz));
+                const float f = __builtin_amdgcn_exp2f(nl2 * (dz * dz));
                 const __bf16 hz = (__bf16)f;
                 ezh[i][tid] = hz;
                 ezl[i][tid] = (__bf16)(f - (float)hz);

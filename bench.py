@@ -911,8 +911,18 @@ def _bench_kde_points(torch, jt, L, lib, dev, P, out, res=50):
     torch.cuda.synchronize()
     ref_ms = e0.elapsed_time(e1)
     a1, a2 = o1.cpu().numpy(), o2.cpu().numpy()
+    # 28 VALU instructions per pair in the kernel's inner loop (3-D: the
+    # differences, squares and sums, the scale, expf's range reduction and
+    # exp, the float64 conversion and add), against MI355X's published FP32
+    # vector rate counted in lane-operations (157.3 TFLOP/s = 78.65 T FMA)
+    valu_per_pair, peak_tops = 28, 78.65
+    ach = valu_per_pair * float(n) * m / (ms * 1e-3) / 1e12
     out["kde_points"] = {"value": float(n) * m / (ms * 1e-3), "unit": "pairs/s", "ms": ms, "points": m,
                          "particles": n, "one_thread_per_point_ms": ref_ms,
+                         "roofline": {"bound": "valu", "achieved": ach, "peak": peak_tops, "unit": "T lane-ops/s",
+                                      "frac": ach / peak_tops, "valu_per_pair": valu_per_pair,
+                                      "note": "VALU instructions per pair counted in the built kernel's inner loop; "
+                                              "peak: the published FP32 vector rate as lane-operations"},
                          "max_rel_diff_vs_one_thread_per_point": float(np.max(np.abs(a1 - a2) / np.maximum(a2, 1e-30))),
                          "parity_test": "tests/test_gpu_density.py::test_kde_eval_chunked",
                          "workload": f"evaluate() at a {res}x{res} grid of points on the z = 0.5 plane over the 2^24 "
